@@ -1,0 +1,342 @@
+/*
+ * Precision-generic body of the oracle (TEST INFRASTRUCTURE, see flb_oracle.h).
+ * Included twice by flb_oracle.c with REAL = float / double and SFX = f32 / f64.
+ *
+ * Expression order follows the reference's unrolled specialisations
+ * (compiled with -D_UNROLLFOR, as benchmarks/cavity3d/Makefile:9 does), so
+ * that sums and products round exactly as FreeLB's do.
+ */
+
+#define CAT_(a, b) a##_##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* Lattice weights as the reference computes them: T(1)/T(3) etc.
+ * (lattice_set.h:168-194, 259-296). */
+static void FN(weights)(int q, REAL* w) {
+  if (q == 19) {
+    w[0] = (REAL)1 / (REAL)3;
+    for (int i = 1; i <= 6; ++i) w[i] = (REAL)1 / (REAL)18;
+    for (int i = 7; i <= 18; ++i) w[i] = (REAL)1 / (REAL)36;
+  } else {
+    w[0] = (REAL)4 / (REAL)9;
+    for (int i = 1; i <= 4; ++i) w[i] = (REAL)1 / (REAL)9;
+    for (int i = 5; i <= 8; ++i) w[i] = (REAL)1 / (REAL)36;
+  }
+}
+
+/* moment::rhoUImpl<D3Q19> (moment.ur.h:505-517) and <D2Q9> (moment.ur.h:461-472). */
+static void FN(rhou_cell)(int q, const REAL* c, REAL* rho, REAL* u) {
+  if (q == 19) {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] + c[11] +
+             c[12] + c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+    *rho = r;
+    u[0] = (c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16]) / r;
+    u[1] = (c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18]) / r;
+    u[2] = (c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18]) / r;
+  } else {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+    *rho = r;
+    u[0] = (c[1] - c[2] + c[5] - c[6] + c[7] - c[8]) / r;
+    u[1] = (c[3] - c[4] + c[5] - c[6] - c[7] + c[8]) / r;
+  }
+}
+
+/* equilibrium::SecondOrderImpl<D3Q19> (equilibrium.ur.h:165-214) and <D2Q9>
+ * (equilibrium.ur.h:65-94). */
+static void FN(feq_cell)(int q, const REAL* w, REAL rho, const REAL* u, REAL* feq) {
+  const REAL InvCs2 = (REAL)3;
+  const REAL InvCs2x = (REAL)0.5 * InvCs2;
+  if (q == 19) {
+    const REAL var0 = (REAL)1 - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * InvCs2x;
+    const REAL rhowk0 = w[0] * rho;
+    const REAL rhowk1 = w[1] * rho;
+    const REAL rhowk7 = w[7] * rho;
+    /* c.u for the first direction of each opposite pair 1,3,5,7,9,11,13,15,17 */
+    const REAL cu[9] = {u[0],        u[1],        u[2],        u[0] + u[1], u[0] + u[2],
+                        u[1] + u[2], u[0] - u[1], u[0] - u[2], u[1] - u[2]};
+    feq[0] = rhowk0 * var0;
+    for (int p = 0; p < 9; ++p) {
+      const REAL a = InvCs2 * cu[p];
+      const REAL v = var0 + a * a * (REAL)0.5;
+      const REAL rw = p < 3 ? rhowk1 : rhowk7;
+      feq[2 * p + 1] = rw * (v + a);
+      feq[2 * p + 2] = rw * (v - a);
+    }
+  } else {
+    const REAL var0 = (REAL)1 - (u[0] * u[0] + u[1] * u[1]) * InvCs2x;
+    const REAL rhowk0 = w[0] * rho;
+    const REAL rhowk1 = w[1] * rho;
+    const REAL rhowk5 = w[5] * rho;
+    const REAL cu[4] = {u[0], u[1], u[0] + u[1], u[0] - u[1]};
+    feq[0] = rhowk0 * var0;
+    for (int p = 0; p < 4; ++p) {
+      const REAL a = InvCs2 * cu[p];
+      const REAL v = var0 + a * a * (REAL)0.5;
+      const REAL rw = p < 2 ? rhowk1 : rhowk5;
+      feq[2 * p + 1] = rw * (v + a);
+      feq[2 * p + 2] = rw * (v - a);
+    }
+  }
+}
+
+/* c_i . u for direction i, summed over axes in order from zero the way
+ * Vector<T,D>::operator* does (Vector.h:229-236). */
+static REAL FN(cdotu)(int q, int i, const REAL* u) {
+  const int* ci = (q == 19) ? &C19[i][0] : &C9[i][0];
+  const int d = (q == 19) ? 3 : 2;
+  REAL r = (REAL)0;
+  for (int a = 0; a < d; ++a) r += u[a] * (REAL)ci[a];
+  return r;
+}
+
+/* Fetch RHO / VELOCITY field values of a cell (fields default to rho = 1, u = 0). */
+static void FN(fields)(const flbo_lattice* L, size_t id, size_t N, REAL* rho, REAL* u) {
+  const int d = (L->q == 19) ? 3 : 2;
+  *rho = L->rho_field ? (REAL)L->rho_field[id] : (REAL)1;
+  for (int a = 0; a < d; ++a) u[a] = L->u_field ? (REAL)L->u_field[(size_t)a * N + id] : (REAL)0;
+}
+
+/* Smagorinsky effective omega (collisionLES.h:36-59): Pi_ab_neq from
+ * moment.ur.h:708-718 (D2Q9) / 750-763 (D3Q19), NormSquare from util.h:418-434. */
+static REAL FN(smag_omega)(int q, const REAL* c, REAL rho, const REAL* u, REAL omega, REAL cs,
+                           REAL rho_field) {
+  const REAL cs2 = (REAL)1 / (REAL)3;
+  REAL ns;
+  if (q == 19) {
+    REAL t0 = c[1] + c[2] + c[7] + c[8] + c[9] + c[10] + c[13] + c[14] + c[15] + c[16] -
+              rho * (u[0] * u[0] + cs2);
+    REAL t1 = c[7] + c[8] - c[13] - c[14] - rho * (u[0] * u[1]);
+    REAL t2 = c[9] + c[10] - c[15] - c[16] - rho * (u[0] * u[2]);
+    REAL t3 = c[3] + c[4] + c[7] + c[8] + c[11] + c[12] + c[13] + c[14] + c[17] + c[18] -
+              rho * (u[1] * u[1] + cs2);
+    REAL t4 = c[11] + c[12] - c[17] - c[18] - rho * (u[1] * u[2]);
+    REAL t5 = c[5] + c[6] + c[9] + c[10] + c[11] + c[12] + c[15] + c[16] + c[17] + c[18] -
+              rho * (u[2] * u[2] + cs2);
+    ns = t0 * t0 + t1 * t1 * 2 + t2 * t2 * 2 + t3 * t3 + t4 * t4 * 2 + t5 * t5;
+  } else {
+    REAL t0 = c[1] + c[2] + c[5] + c[6] + c[7] + c[8] - rho * (u[0] * u[0] + cs2);
+    REAL t1 = c[5] + c[6] - c[7] - c[8] - rho * (u[0] * u[1]);
+    REAL t2 = c[3] + c[4] + c[5] + c[6] + c[7] + c[8] - rho * (u[1] * u[1] + cs2);
+    ns = t0 * t0 + t1 * t1 * 2 + t2 * t2;
+  }
+  const REAL norm = SQRT(ns);
+  /* smagorinsky * smagorinsky * InvCs2 * InvCs2 * 2 * std::sqrt(2) / RHO_field;
+   * std::sqrt(2) of an int argument is a double, so the product is formed in
+   * double from that point on and rounded back to REAL on assignment. */
+  const REAL coeff = (REAL)((double)(cs * cs * (REAL)3 * (REAL)3 * (REAL)2) * sqrt(2.0) /
+                            (double)rho_field);
+  const REAL rt_mol = (REAL)1 / omega;
+  const REAL rt_tur = (REAL)0.5 * (SQRT(rt_mol * rt_mol + coeff * norm) - rt_mol);
+  return (REAL)1 / (rt_mol + rt_tur);
+}
+
+/* Apply one cell's dynamics in place on the gathered values c[q]. */
+static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const REAL* w, REAL omega,
+                             REAL _omega, REAL* c, size_t id, size_t N) {
+  const int q = L->q;
+  const int d = (q == 19) ? 3 : 2;
+  REAL rho, u[3] = {0, 0, 0}, feq[19];
+  switch (t->dynamics) {
+    case FLBO_DYN_BGK:
+    case FLBO_DYN_BGK_FIELD:
+      /* collision::BGK (collision.h:39-56) */
+      if (t->dynamics == FLBO_DYN_BGK) {
+        FN(rhou_cell)(q, c, &rho, u);
+      } else {
+        FN(fields)(L, id, N, &rho, u);
+      }
+      FN(feq_cell)(q, w, rho, u, feq);
+      for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i];
+      break;
+    case FLBO_DYN_BOUNCEBACK:
+      /* collision::BounceBack<D3Q19/D2Q9> (collision.ur.h:44-93) */
+      for (int i = 1; i < q; i += 2) {
+        const REAL tmp = c[i];
+        c[i] = c[i + 1];
+        c[i + 1] = tmp;
+      }
+      break;
+    case FLBO_DYN_BOUNCEBACK_MW: {
+      /* collision::BounceBackMovingWall (collision.ur.h:97-187):
+       * rhox = 2*InvCs2*RHO; uc_i = (c_i.u) * w_i * rhox */
+      REAL rf, uf[3] = {0, 0, 0};
+      FN(fields)(L, id, N, &rf, uf);
+      const REAL rhox = (REAL)6 * rf;
+      for (int i = 1; i < q; i += 2) {
+        const int* ci = (q == 19) ? &C19[i][0] : &C9[i][0];
+        /* the unrolled form sums only the non-zero components: u0, u1, u0+u1, u0-u1, ... */
+        REAL cu = 0;
+        int first = 1;
+        for (int a = 0; a < d; ++a) {
+          if (ci[a] == 0) continue;
+          if (first) {
+            cu = ci[a] > 0 ? uf[a] : -uf[a];
+            first = 0;
+          } else {
+            cu = ci[a] > 0 ? cu + uf[a] : cu - uf[a];
+          }
+        }
+        const REAL uc = cu * w[i] * rhox;
+        const REAL tmp = c[i];
+        c[i] = c[i + 1] + uc;
+        c[i + 1] = tmp - uc;
+      }
+      break;
+    }
+    case FLBO_DYN_SMAG_BGK: {
+      /* collision::SmagorinskyBGK (collisionLES.h:36-66); RHO field in the coefficient */
+      REAL rf, uf[3];
+      FN(fields)(L, id, N, &rf, uf);
+      FN(rhou_cell)(q, c, &rho, u);
+      const REAL eo = FN(smag_omega)(q, c, rho, u, omega, (REAL)t->smagorinsky, rf);
+      FN(feq_cell)(q, w, rho, u, feq);
+      const REAL _eo = (REAL)1 - eo;
+      for (int i = 0; i < q; ++i) c[i] = eo * feq[i] + _eo * c[i];
+      break;
+    }
+    case FLBO_DYN_MRT:
+    case FLBO_DYN_SMAG_MRT:
+      FN(mrt_cell)(q, w, omega, t, L, c, id, N);
+      break;
+    default:
+      break;
+  }
+}
+
+/* One reference time step on the SoA arrays A (in: post-stream state of the
+ * previous step; out: post-stream state of this step). B is scratch.
+ *   1. BlockLattice::ApplyCellDynamics (block_lattice.hh:72-92): every cell
+ *      id in [0, N), first-match task (tmp.h:283-309), in place.
+ *   2. BlockLattice::Stream (block_lattice.hh:64-70) via CyclicArray::rotate
+ *      (field.h:676-685): arr_i[id] <- old arr_i[(id - Delta_i) mod N],
+ *      Delta_i = c_i . (1, Nx, Nx*Ny) (block_lattice_base.h:126-127).
+ *   3. BlockBoundaryManager::Apply (basic_boundary.h:345-347) for each
+ *      halfway manager in order (boundary.h:121-128): for every boundary cell
+ *      of the inner region and every k with a void neighbour at id + Delta_k,
+ *      f_{opp k}(id) <- getPrevious_k(id) [+ 2 InvCs2 w_o RHO (VELOCITY . c_o)]
+ *      (bounce_back_boundary.h:169-181, 240-256; getPrevious at field.h:645-656).
+ */
+static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
+  const int q = L->q;
+  const size_t nx = (size_t)L->nx, ny = (size_t)L->ny, nz = (size_t)L->nz;
+  const size_t N = nx * ny * nz;
+  const int (*C)[3] = (q == 19) ? C19 : C9;
+  const int* OPP = (q == 19) ? OPP19 : OPP9;
+  REAL w[19];
+  FN(weights)(q, w);
+  const REAL omega = (REAL)L->omega;
+  const REAL _omega = (REAL)1 - omega;
+  const uint8_t* flags = L->flags;
+
+#pragma omp parallel for schedule(static) num_threads(L->nthreads > 0 ? L->nthreads : omp_get_max_threads())
+  for (size_t id = 0; id < N; ++id) {
+    const uint8_t fl = flags[id];
+    const flbo_task* t = NULL;
+    for (int k = 0; k < L->ntasks; ++k) {
+      if (fl & L->tasks[k].mask) {
+        t = &L->tasks[k];
+        break;
+      }
+    }
+    if (!t || t->dynamics == FLBO_DYN_NONE) continue;
+    REAL c[19];
+    for (int i = 0; i < q; ++i) c[i] = A[(size_t)i * N + id];
+    FN(collide_cell)(L, t, w, omega, _omega, c, id, N);
+    for (int i = 0; i < q; ++i) A[(size_t)i * N + id] = c[i];
+  }
+
+  ptrdiff_t delta[19];
+  for (int i = 0; i < q; ++i)
+    delta[i] = (ptrdiff_t)C[i][0] + (ptrdiff_t)C[i][1] * (ptrdiff_t)nx +
+               (ptrdiff_t)C[i][2] * (ptrdiff_t)(nx * ny);
+
+#pragma omp parallel for schedule(static) num_threads(L->nthreads > 0 ? L->nthreads : omp_get_max_threads())
+  for (size_t id = 0; id < N; ++id) {
+    for (int i = 0; i < q; ++i) {
+      ptrdiff_t src = (ptrdiff_t)id - delta[i];
+      if (src < 0) src += (ptrdiff_t)N;
+      if (src >= (ptrdiff_t)N) src -= (ptrdiff_t)N;
+      B[(size_t)i * N + id] = A[(size_t)i * N + (size_t)src];
+    }
+  }
+
+  const int ov = L->overlap;
+  const int d = (q == 19) ? 3 : 2;
+  for (int b = 0; b < L->nbc; ++b) {
+    const flbo_hwbc* bc = &L->bcs[b];
+    const size_t zlo = d == 3 ? (size_t)ov : 0, zhi = d == 3 ? nz - (size_t)ov : 1;
+#pragma omp parallel for schedule(static) num_threads(L->nthreads > 0 ? L->nthreads : omp_get_max_threads())
+    for (size_t z = zlo; z < zhi; ++z) {
+      for (size_t y = (size_t)ov; y < ny - (size_t)ov; ++y) {
+        for (size_t x = (size_t)ov; x < nx - (size_t)ov; ++x) {
+          const size_t id = x + nx * (y + ny * z);
+          if (!(flags[id] & bc->cell_mask)) continue;
+          REAL rf = 1, uf[3] = {0, 0, 0};
+          if (bc->kind == FLBO_HWBC_MOVING) FN(fields)(L, id, N, &rf, uf);
+          for (int k = 1; k < q; ++k) {
+            const size_t nb = (size_t)((ptrdiff_t)id + delta[k]);
+            if (!(flags[nb] & bc->void_mask)) continue;
+            const int o = OPP[k];
+            REAL v = A[(size_t)k * N + id];
+            if (bc->kind == FLBO_HWBC_MOVING) {
+              /* 2 * InvCs2 * w_o * RHO * (VELOCITY . c_o) */
+              v = v + (REAL)6 * w[o] * rf * FN(cdotu)(q, o, uf);
+            }
+            B[(size_t)o * N + id] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+int FN(run_impl)(const flbo_lattice* L, REAL* pops, REAL* scratch, int steps) {
+  if (!L || !pops || !scratch || (L->q != 19 && L->q != 9) || steps < 0) return -1;
+  const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
+  REAL* A = pops;
+  REAL* B = scratch;
+  for (int s = 0; s < steps; ++s) {
+    FN(step)(L, A, B);
+    REAL* t = A;
+    A = B;
+    B = t;
+  }
+  if (A != pops) memcpy(pops, A, sizeof(REAL) * (size_t)L->q * N);
+  return 0;
+}
+
+void FN(rhou_impl)(const flbo_lattice* L, const REAL* pops, uint8_t mask, REAL* rhou) {
+  const int q = L->q;
+  const int d = (q == 19) ? 3 : 2;
+  const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
+#pragma omp parallel for schedule(static)
+  for (size_t id = 0; id < N; ++id) {
+    REAL c[19], rho = 0, u[3] = {0, 0, 0};
+    if (L->flags[id] & mask) {
+      for (int i = 0; i < q; ++i) c[i] = pops[(size_t)i * N + id];
+      FN(rhou_cell)(q, c, &rho, u);
+    }
+    rhou[id] = rho;
+    for (int a = 0; a < d; ++a) rhou[(size_t)(a + 1) * N + id] = u[a];
+  }
+}
+
+void FN(init_feq_impl)(const flbo_lattice* L, const REAL* rhou, REAL* pops) {
+  const int q = L->q;
+  const int d = (q == 19) ? 3 : 2;
+  const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
+  REAL w[19];
+  FN(weights)(q, w);
+#pragma omp parallel for schedule(static)
+  for (size_t id = 0; id < N; ++id) {
+    REAL u[3] = {0, 0, 0}, feq[19];
+    for (int a = 0; a < d; ++a) u[a] = rhou[(size_t)(a + 1) * N + id];
+    FN(feq_cell)(q, w, rhou[id], u, feq);
+    for (int i = 0; i < q; ++i) pops[(size_t)i * N + id] = feq[i];
+  }
+}
+
+#undef FN
+#undef CAT
+#undef CAT_

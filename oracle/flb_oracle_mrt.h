@@ -1,0 +1,93 @@
+/*
+ * MRT collision of the oracle (TEST INFRASTRUCTURE, see flb_oracle.h).
+ * Included by flb_oracle.c with REAL / SFX / SQRT defined, before
+ * flb_oracle_impl.h.
+ */
+
+#define MCAT_(a, b) a##_##b
+#define MCAT(a, b) MCAT_(a, b)
+#define MFN(name) MCAT(name, SFX)
+
+/* Forward declarations of helpers defined in flb_oracle_impl.h. */
+static void MFN(rhou_cell)(int q, const REAL* c, REAL* rho, REAL* u);
+static void MFN(feq_cell)(int q, const REAL* w, REAL rho, const REAL* u, REAL* feq);
+static void MFN(fields)(const flbo_lattice* L, size_t id, size_t N, REAL* rho, REAL* u);
+static REAL MFN(smag_omega)(int q, const REAL* c, REAL rho, const REAL* u, REAL omega, REAL cs,
+                            REAL rho_field);
+
+/* D2Q9: collision::MRT_Feq_RhoU<D2Q9> as unrolled in collisionMRT.ur.h:45-129
+ * (closed-form equilibrium moments, rates {0, 1.1, 1.1, 0, 1.1, 0, 1.1, w, w}).
+ * D3Q19: m = M f, m_eq = M feq(SecondOrder), f <- f - M^T diag(s_j / |M_j|^2)
+ * (m - m_eq) on the orthogonal d'Humieres basis (no reference form). */
+static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
+                          const flbo_lattice* L, REAL* c, size_t id, size_t N) {
+  REAL rho, u[3] = {0, 0, 0};
+  MFN(rhou_cell)(q, c, &rho, u);
+  REAL om = omega;
+  if (t->dynamics == FLBO_DYN_SMAG_MRT) {
+    REAL rf, uf[3];
+    MFN(fields)(L, id, N, &rf, uf);
+    om = MFN(smag_omega)(q, c, rho, u, omega, (REAL)t->smagorinsky, rf);
+  }
+  if (q == 9) {
+    REAL rt[9];
+    if (L->mrt_rates) {
+      for (int j = 0; j < 9; ++j) rt[j] = isnan(L->mrt_rates[j]) ? om : (REAL)L->mrt_rates[j];
+    } else {
+      const REAL s11 = (REAL)(11. / 10.);
+      const REAL d[9] = {0, s11, s11, 0, s11, 0, s11, om, om};
+      for (int j = 0; j < 9; ++j) rt[j] = d[j];
+    }
+    REAL ims[9][9];
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 9; ++j)
+        ims[i][j] = ((REAL)INVM9_N[i][j] / (REAL)INVM9_D[i][j]) * rt[j];
+    REAL m[9];
+    for (int i = 0; i < 9; ++i) {
+      m[i] = (REAL)0;
+      for (int j = 0; j < 9; ++j) m[i] += (REAL)M9[i][j] * c[j];
+    }
+    REAL dm[9];
+    const REAL ux2 = u[0] * u[0];
+    const REAL uy2 = u[1] * u[1];
+    const REAL rhoux = rho * u[0];
+    const REAL rhouy = rho * u[1];
+    const REAL cse0 = (REAL)3 * rho * (ux2 + uy2);
+    dm[0] = m[0] - rho;
+    dm[1] = m[1] + (REAL)2 * rho - cse0;
+    dm[2] = m[2] - (REAL)9 * rho * ux2 * uy2 + cse0 - rho;
+    dm[3] = m[3] - rhoux;
+    dm[4] = m[4] - rhoux * ((REAL)3 * uy2 - (REAL)1);
+    dm[5] = m[5] - rhouy;
+    dm[6] = m[6] - rhouy * ((REAL)3 * ux2 - (REAL)1);
+    dm[7] = m[7] - rho * (ux2 - uy2);
+    dm[8] = m[8] - rho * u[0] * u[1];
+    for (int i = 0; i < 9; ++i) {
+      REAL s = (REAL)0;
+      for (int j = 0; j < 9; ++j) s += ims[i][j] * dm[j];
+      c[i] -= s;
+    }
+  } else {
+    REAL feq[19], g[19], k[19];
+    MFN(feq_cell)(q, w, rho, u, feq);
+    for (int i = 0; i < 19; ++i) g[i] = c[i] - feq[i];
+    for (int j = 0; j < 19; ++j) {
+      double sj = L->mrt_rates ? L->mrt_rates[j] : mrt19_default_rate(j);
+      const REAL s = isnan(sj) ? om : (REAL)sj;
+      REAL dm = (REAL)0;
+      for (int i = 0; i < 19; ++i)
+        if (M19[j][i] != 0) dm += (REAL)M19[j][i] * g[i];
+      k[j] = s * dm / (REAL)M19_NORM[j];
+    }
+    for (int i = 0; i < 19; ++i) {
+      REAL s = (REAL)0;
+      for (int j = 0; j < 19; ++j)
+        if (M19[j][i] != 0) s += (REAL)M19[j][i] * k[j];
+      c[i] -= s;
+    }
+  }
+}
+
+#undef MFN
+#undef MCAT
+#undef MCAT_

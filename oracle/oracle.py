@@ -1,0 +1,192 @@
+"""ctypes binding of the C oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module; the product (freelb_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK, DYN_SMAG_MRT = range(8)
+HWBC_NORMAL, HWBC_MOVING = 0, 1
+
+# Cavity flags of the reference drivers (benchmarks/cavity3d/cavity3d.cpp:121-124).
+VOID, AABB, BB, BBMW = 1, 2, 4, 8
+
+
+class _Task(C.Structure):
+    _fields_ = [("mask", C.c_uint8), ("dynamics", C.c_uint8), ("pad", C.c_uint8 * 2),
+                ("smagorinsky", C.c_double)]
+
+
+class _HWBC(C.Structure):
+    _fields_ = [("cell_mask", C.c_uint8), ("void_mask", C.c_uint8), ("kind", C.c_uint8),
+                ("pad", C.c_uint8)]
+
+
+class _Lattice(C.Structure):
+    _fields_ = [("q", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("overlap", C.c_int), ("flags", C.POINTER(C.c_uint8)), ("ntasks", C.c_int),
+                ("tasks", _Task * 8), ("nbc", C.c_int), ("bcs", _HWBC * 4),
+                ("omega", C.c_double), ("rho_field", C.POINTER(C.c_double)),
+                ("u_field", C.POINTER(C.c_double)), ("mrt_rates", C.POINTER(C.c_double)),
+                ("nthreads", C.c_int)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        for name, T in (("f32", C.c_float), ("f64", C.c_double)):
+            P = C.POINTER(T)
+            getattr(_lib, "flbo_run_" + name).argtypes = [C.POINTER(_Lattice), P, P, C.c_int]
+            getattr(_lib, "flbo_run_" + name).restype = C.c_int
+            getattr(_lib, "flbo_rhou_" + name).argtypes = [C.POINTER(_Lattice), P, C.c_uint8, P]
+            getattr(_lib, "flbo_init_feq_" + name).argtypes = [C.POINTER(_Lattice), P, P]
+        _lib.flbo_cavity_flags.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)]
+        _lib.flbo_convert_from_rt.argtypes = [C.c_float, C.c_float, C.c_float,
+                                              C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    return _lib
+
+
+def _ptr(a, T):
+    return a.ctypes.data_as(C.POINTER(T))
+
+
+class Lattice:
+    """One block of the reference lattice: flags, task table, halfway BCs, fields."""
+
+    def __init__(self, q, dims, flags, tasks, bcs=(), omega=1.0, rho_field=None, u_field=None,
+                 mrt_rates=None, overlap=1, nthreads=0):
+        self.q = q
+        self.nx, self.ny, self.nz = dims
+        self.N = self.nx * self.ny * self.nz
+        self.flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        assert self.flags.size == self.N
+        self._keep = [self.flags]
+        L = _Lattice()
+        L.q, L.nx, L.ny, L.nz, L.overlap = q, self.nx, self.ny, self.nz, overlap
+        L.flags = _ptr(self.flags, C.c_uint8)
+        assert len(tasks) <= 8 and len(bcs) <= 4
+        L.ntasks = len(tasks)
+        for k, t in enumerate(tasks):
+            mask, dyn = t[0], t[1]
+            L.tasks[k].mask, L.tasks[k].dynamics = mask, dyn
+            L.tasks[k].smagorinsky = t[2] if len(t) > 2 else 0.0
+        L.nbc = len(bcs)
+        for k, (cm, vm, kind) in enumerate(bcs):
+            L.bcs[k].cell_mask, L.bcs[k].void_mask, L.bcs[k].kind = cm, vm, kind
+        L.omega = float(omega)
+        for name, arr in (("rho_field", rho_field), ("u_field", u_field), ("mrt_rates", mrt_rates)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+                self._keep.append(a)
+                setattr(L, name, _ptr(a, C.c_double))
+        L.nthreads = nthreads
+        self.L = L
+
+    @property
+    def d(self):
+        return 3 if self.q == 19 else 2
+
+    def run(self, pops, steps):
+        """Advance pops[q][N] (float32 or float64, natural layout) in place."""
+        T = C.c_float if pops.dtype == np.float32 else C.c_double
+        name = "f32" if pops.dtype == np.float32 else "f64"
+        assert pops.flags.c_contiguous and pops.size == self.q * self.N
+        scratch = np.empty_like(pops)
+        rc = getattr(lib(), "flbo_run_" + name)(C.byref(self.L), _ptr(pops, T), _ptr(scratch, T), steps)
+        if rc != 0:
+            raise RuntimeError(f"flbo_run_{name} failed ({rc})")
+        return pops
+
+    def rhou(self, pops, mask=0xFF & ~VOID):
+        T = C.c_float if pops.dtype == np.float32 else C.c_double
+        name = "f32" if pops.dtype == np.float32 else "f64"
+        out = np.zeros((self.d + 1, self.N), dtype=pops.dtype)
+        getattr(lib(), "flbo_rhou_" + name)(C.byref(self.L), _ptr(pops, T), mask, _ptr(out, T))
+        return out
+
+    def init_feq(self, rhou):
+        rhou = np.ascontiguousarray(rhou)
+        T = C.c_float if rhou.dtype == np.float32 else C.c_double
+        name = "f32" if rhou.dtype == np.float32 else "f64"
+        pops = np.empty((self.q, self.N), dtype=rhou.dtype)
+        getattr(lib(), "flbo_init_feq_" + name)(C.byref(self.L), _ptr(rhou, T), _ptr(pops, T))
+        return pops
+
+
+def cavity_flags(Ni, Nj, Nk=0):
+    n = (Ni + 2) * (Nj + 2) * ((Nk + 2) if Nk > 0 else 1)
+    out = np.empty(n, dtype=np.uint8)
+    lib().flbo_cavity_flags(Ni, Nj, Nk, _ptr(out, C.c_uint8))
+    return out
+
+
+def convert_from_rt(RT, visc, u_phys):
+    om, ul = C.c_float(), C.c_float()
+    lib().flbo_convert_from_rt(RT, visc, u_phys, C.byref(om), C.byref(ul))
+    return np.float32(om.value), np.float32(ul.value)
+
+
+def weights(q, dtype=np.float32):
+    one = dtype(1)
+    if q == 19:
+        return np.array([one / dtype(3)] + [one / dtype(18)] * 6 + [one / dtype(36)] * 12, dtype=dtype)
+    return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
+
+
+def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float32):
+    """The reference cavity driver as an oracle Lattice plus its initial pops.
+
+    mode 'fw'      : benchmarks/cavity3d/cavity3d.cpp:170-182 (wall cells swap).
+    mode 'hw'      : every non-void cell BGK<rhoU>; halfway normal / moving wall.
+    mode 'hwfield' : examples/couette2d/couette2d.cpp:183-216 (walls BGK<useFieldrhoU>).
+    """
+    q = 19 if Nk > 0 else 9
+    flags = cavity_flags(Ni, Nj, Nk)
+    dims = (Ni + 2, Nj + 2, (Nk + 2) if Nk > 0 else 1)
+    omega, ulat = convert_from_rt(np.float32(RT), np.float32(visc), np.float32(u_phys))
+    d = 3 if q == 19 else 2
+    N = flags.size
+    rho_field = np.ones(N)
+    u_field = np.zeros((d, N))
+    u_field[0, flags == BBMW] = float(ulat)
+    if mode == "fw":
+        tasks = [(AABB, DYN_BGK), (BB, DYN_BOUNCEBACK), (BBMW, DYN_BOUNCEBACK_MW)]
+        bcs = []
+    elif mode == "hw":
+        tasks = [(AABB | BB | BBMW, DYN_BGK)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+    elif mode == "hwfield":
+        tasks = [(AABB, DYN_BGK), (BB | BBMW, DYN_BGK_FIELD)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+    else:
+        raise ValueError(mode)
+    if dtype == np.float64:
+        om = 1.0 / RT
+        ul = float(u_phys) / (1.0 / ((RT - 0.5) * (1.0 / 3.0) / visc))
+        u_field[0, flags == BBMW] = ul
+    else:
+        om = float(omega)
+    lat = Lattice(q, dims, flags, tasks, bcs, omega=om, rho_field=rho_field, u_field=u_field)
+    w = weights(q, dtype)
+    pops = np.repeat(w[:, None], N, axis=1).astype(dtype)
+    pops = np.ascontiguousarray(pops)
+    return lat, pops
