@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Throughput of the fused D3Q19 lattice update on one MI355X.
+
+Workload: the 256^3 lid-driven cavity (D3Q19, BGK, tau = 0.53, halfway
+bounce-back on five walls, moving lid u = 0.1), populations resident in HBM.
+One step = one fused collide-and-stream launch over the 256^3 fluid cells.
+`value` is MLUPS of FP32 storage; the same workload with FP16 / BF16 shifted
+storage is reported beside it, with each run's HBM roofline fraction
+(152 B per cell update in FP32, 76 B in the shifted half formats).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+N_EDGE = 256
+
+
+class HipTimer:
+    """hipEvent pair on a given stream (plain HIP runtime through ctypes)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(C.byref(self.b)) == 0
+
+    def sync(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+    def start(self, stream=None):
+        assert self.hip.hipEventRecord(self.a, stream) == 0
+
+    def stop(self, stream=None) -> float:
+        assert self.hip.hipEventRecord(self.b, stream) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), self.a, self.b) == 0
+        return float(ms.value)
+
+
+def cavity_flags(n):
+    """Flag layout of the reference cavity driver: void shell 1, fluid 2,
+    walls 4, lid 8 (benchmarks/cavity3d/cavity3d.cpp:121-142)."""
+    m = n + 2
+    f = np.full((m, m, m), 1, dtype=np.uint8)
+    f[1:-1, 1:-1, 1:-1] = 4
+    f[2:-2, 2:-2, 2:-2] = 2
+    f[n, 2:-2, 2:-2] = 8  # top layer z = n, x/y in [2, n-1]
+    return f.reshape(-1)
+
+
+def make_block(storage, n=N_EDGE):
+    from freelb_amd import BlockLattice, HalfwayBB, Key_TypePair
+    from freelb_amd import abi
+    omega = float(np.float32(1.0) / np.float32(0.53))
+    ulid = 0.1
+    lat = BlockLattice(19, (n + 2, n + 2, n + 2), omega=omega, storage=storage)
+    flags = cavity_flags(n)
+    lat.upload_flags(flags)
+    lat.set_tasks([Key_TypePair(2 | 4 | 8, abi.DYN_BGK)])
+    lat.set_halfway_bc([HalfwayBB(4, 1, False), HalfwayBB(8, 1, True)])
+    N = flags.size
+    u = np.zeros((3, N))
+    u[0, flags == 8] = ulid
+    lat.set_field_rho(np.ones(N))
+    lat.set_field_u(u)
+    lat.init_uniform(1.0)
+    return lat
+
+
+def time_storage(storage, steps, warmup, timer):
+    lat = make_block(storage)
+    n_active = N_EDGE ** 3
+    lat.step(warmup)
+    lat.synchronize()
+    timer.sync()
+    t0 = time.perf_counter()
+    timer.start()
+    lat.step(steps)
+    ms = timer.stop()
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    mass = lat.mass_momentum()
+    finite = bool(np.isfinite(mass).all())
+    lat.close()
+    bytes_per = 152 if storage == "f32" else (76 if storage in ("f16s", "bf16s") else 304)
+    ms_step = ms / steps
+    mlups = n_active * steps / (ms / 1e3) / 1e6
+    achieved = n_active * bytes_per / (ms_step / 1e3) / 1e9  # GB/s, algorithmic bytes per launch
+    return {"storage": storage, "ms_per_step": ms_step, "mlups": mlups, "wall_s": wall,
+            "achieved_gbs": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": bytes_per,
+            "finite": finite}
+
+
+def cpu_baseline(n=N_EDGE, steps=20):
+    """FreeLB's own benchmarks/cavity3d driver (built from the reference sources
+    into oracle/_ref by oracle/Makefile) on the host cores, bounded sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cavity3d_bench")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if os.path.exists(exe):
+        work = tempfile.mkdtemp(prefix="flb_cpu_")
+        try:
+            ini = (f"[workdir]\nworkdir_ = ./output\n[parallel]\nthread_num = {threads}\nblock_num = 1\n"
+                   f"[Mesh]\nNi = {n}\nNj = {n}\nNk = {n}\nCell_Len = 1\n"
+                   "[Physical_Property]\nrho_ref = 0.001\nKine_Visc = 0.01\n"
+                   "[Init_Conditions]\nU_Ini0 = 0\nU_Ini1 = 0\nU_Ini2 = 0\nU_Max = 0.1\n"
+                   "[Boundary_Conditions]\nVelo_Wall0 = 0.1\nVelo_Wall1 = 0\nVelo_Wall2 = 0\n"
+                   f"[LB]\nRT = 0.53\n[Simulation_Settings]\nTotalStep = {steps}\nOutputStep = 100000\n"
+                   "[tolerance]\ntol = 1e-4\n")
+            with open(os.path.join(work, "cavity3d.ini"), "w") as fh:
+                fh.write(ini)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+            out = subprocess.run([exe], cwd=work, env=env, capture_output=True, text=True,
+                                 timeout=600).stdout
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        mlups_total = None
+        for line in out.splitlines():
+            if line.strip().startswith("Average_MLUPs:"):  # utils/timer.h:93
+                mlups_total = float(line.split(":", 1)[1].split()[0])
+        if mlups_total is not None:
+            active = mlups_total * n ** 3 / (n + 2) ** 3
+            return {"value": active, "unit": "MLUPS", "cores": threads, "kind": "reference",
+                    "sample": f"FreeLB benchmarks/cavity3d (fullway walls), {n}^3 fluid cells, "
+                              f"{steps} timed steps after 10 warm-up, OpenMP {threads} threads; "
+                              f"MLUPS over fluid cells ({mlups_total:.1f} counting the ghost layer)"}
+    # fall back to the C oracle (same algorithm restated), timed the same way
+    from oracle import oracle as O
+    case, p = O.cavity_case("hw", n, n, n)
+    case.L.nthreads = threads
+    t0 = time.perf_counter()
+    case.run(p, steps)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C oracle, halfway cavity {n}^3, {steps} steps, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storages", default="f32,f16s,bf16s")
+    ap.add_argument("--no-probe", action="store_true", help="skip the device copy-bandwidth probe")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each GPU would run its own share of blocks; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    timer = HipTimer()
+    runs = {}
+    for s in args.storages.split(","):
+        runs[s] = time_storage(s, args.steps, args.warmup, timer)
+        print(f"# {s}: {runs[s]['mlups']:.1f} MLUPS, {runs[s]['ms_per_step']:.4f} ms/step, "
+              f"{runs[s]['achieved_gbs']:.0f} GB/s ({100 * runs[s]['frac']:.1f}% of HBM peak)",
+              file=sys.stderr, flush=True)
+    f32 = runs["f32"]
+    line = {
+        "metric": "MLUPS D3Q19 256^3 FP32 (lid-driven cavity, BGK)",
+        "value": round(f32["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (cavity flags, uniform equilibrium start)",
+        "config": {"workload": "cavity D3Q19 BGK 256^3, halfway walls + moving lid, tau 0.53",
+                   "cells": N_EDGE ** 3, "lattice": "D3Q19", "collision": "BGK",
+                   "storage": "f32", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(f32["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(f32["frac"], 4),
+                     "traffic": None},
+    }
+    for s, r in runs.items():
+        if s == "f32":
+            continue
+        line[f"{s}_mlups"] = round(r["mlups"], 1)
+        line[f"{s}_speedup_vs_f32"] = round(r["mlups"] / f32["mlups"], 3)
+        line[f"{s}_roofline_frac"] = round(r["frac"], 4)
+    if not all(r["finite"] for r in runs.values()):
+        line["warning"] = "non-finite mass after the timed run"
+    if not args.no_probe:
+        from freelb_amd import abi
+        gbs = C.c_double()
+        abi.check(abi.lib().flb_bandwidth_probe(C.c_size_t(1 << 30), 20, C.byref(gbs)))
+        line["device_copy_gbs"] = round(gbs.value, 1)
+        line["roofline"]["frac_of_copy"] = round(f32["achieved_gbs"] / gbs.value, 4)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,12 @@
+"""MI355X (gfx950) engine for FreeLB's per-block lattice update.
+
+The hot path -- the fused AA-pattern collide-and-stream step with halfway
+bounce-back -- runs in hand-written HIP kernels behind the C ABI of
+include/flb_abi.h (libflb_hip.so, built in-tree). This package holds the
+kernels' sources (csrc/), the build recipe and a thin host-side mirror of
+FreeLB's BlockLattice interface.
+"""
+from .abi import FLBError  # noqa: F401
+from .lattice import BlockLattice, HalfwayBB, Key_TypePair, weights  # noqa: F401
+
+__all__ = ["BlockLattice", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]

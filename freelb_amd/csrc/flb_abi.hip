@@ -1,0 +1,771 @@
+// C-ABI host layer: block handles, validation, staging and launches.
+// Declarations and reference citations are in include/flb_abi.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flb_abi.h"
+#include "flb_cell.hpp"
+#include "flb_internal.hpp"
+
+using flb::KArgs;
+
+struct flb_block_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  flb_block_desc desc{};
+  int d = 3;
+  int64_t N = 0;        // host cells nx*ny*nz
+  int64_t stride = 0;   // slots between two per-cell arrays (pads included)
+  int pitch = 0;        // slots per row
+  int64_t plane = 0;    // slots per plane
+  int64_t dpop = 0, zpop = 0;  // population strides (see flb_internal.hpp)
+  int ypop = 0;
+  int64_t pop_elems = 0;       // elements allocated for the populations
+  size_t elem = 4;   // bytes per stored population
+  size_t real = 4;   // bytes of the compute type
+  void* d_pops = nullptr;
+  uint8_t* d_flags = nullptr;
+  void* d_rho = nullptr;
+  void* d_u = nullptr;
+  double* d_partials = nullptr;
+  double* d_red = nullptr;
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  std::vector<uint8_t> h_flags;
+  bool flags_set = false;
+  flb_task tasks[flb::kMaxTasks]{};
+  int ntasks = 0;
+  flb_hwbc bcs[flb::kMaxBc]{};
+  int nbc = 0;
+  double rates[19]{};
+  int64_t step = 0;
+  int parity = 0;
+  int pending = 0;  // 1: cell dynamics applied, stream not yet called
+  int vec = -1;  // step kernel form: -1 cell form, 1/2/4 row form with that many cells per thread
+  int wpb = 4;   // wavefronts per workgroup
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(FLB_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+      (void)hipSetDevice(dev);
+    } else {
+      prev = -1;
+    }
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+bool storage_ok(int s) {
+  return s == FLB_STORE_F32 || s == FLB_STORE_F16S || s == FLB_STORE_BF16S || s == FLB_STORE_F64;
+}
+
+hipError_t launch_step(const flb_block_s* b, int family, const KArgs& a) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_step_f32(b->desc.q, b->parity, family, a, b->stream);
+    case FLB_STORE_F16S: return flb::launch_step_f16s(b->desc.q, b->parity, family, a, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_step_bf16s(b->desc.q, b->parity, family, a, b->stream);
+    default: return flb::launch_step_f64(b->desc.q, b->parity, family, a, b->stream);
+  }
+}
+
+hipError_t launch_gather(const flb_block_s* b, const KArgs& a, void* out) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_gather_f32(b->desc.q, b->parity, a, out, b->stream);
+    case FLB_STORE_F16S: return flb::launch_gather_f16s(b->desc.q, b->parity, a, out, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_gather_bf16s(b->desc.q, b->parity, a, out, b->stream);
+    default: return flb::launch_gather_f64(b->desc.q, b->parity, a, out, b->stream);
+  }
+}
+
+hipError_t launch_scatter(const flb_block_s* b, const KArgs& a, const void* in) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_scatter_f32(b->desc.q, a, in, b->stream);
+    case FLB_STORE_F16S: return flb::launch_scatter_f16s(b->desc.q, a, in, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_scatter_bf16s(b->desc.q, a, in, b->stream);
+    default: return flb::launch_scatter_f64(b->desc.q, a, in, b->stream);
+  }
+}
+
+hipError_t launch_init_feq(const flb_block_s* b, const KArgs& a, const void* rhou) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_init_feq_f32(b->desc.q, a, rhou, b->stream);
+    case FLB_STORE_F16S: return flb::launch_init_feq_f16s(b->desc.q, a, rhou, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_init_feq_bf16s(b->desc.q, a, rhou, b->stream);
+    default: return flb::launch_init_feq_f64(b->desc.q, a, rhou, b->stream);
+  }
+}
+
+hipError_t launch_moments(const flb_block_s* b, const KArgs& a, uint8_t mask, void* out) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_moments_f32(b->desc.q, b->parity, a, mask, out, b->stream);
+    case FLB_STORE_F16S: return flb::launch_moments_f16s(b->desc.q, b->parity, a, mask, out, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_moments_bf16s(b->desc.q, b->parity, a, mask, out, b->stream);
+    default: return flb::launch_moments_f64(b->desc.q, b->parity, a, mask, out, b->stream);
+  }
+}
+
+hipError_t launch_reduce(const flb_block_s* b, const KArgs& a, uint8_t mask, double* out) {
+  switch (b->desc.storage) {
+    case FLB_STORE_F32: return flb::launch_reduce_f32(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+    case FLB_STORE_F16S: return flb::launch_reduce_f16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_reduce_bf16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+    default: return flb::launch_reduce_f64(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+  }
+}
+
+void default_rates(flb_block b) {
+  if (b->desc.q == 19) {
+    for (int j = 0; j < 19; ++j) b->rates[j] = flb::m19_default_rate(j);
+  } else {
+    // collisionMRT.ur.h:51: {0, 1.1, 1.1, 0, 1.1, 0, 1.1, omega, omega}
+    const double s[9] = {0.0, 1.1, 1.1, 0.0, 1.1, 0.0, 1.1, -1.0, -1.0};
+    for (int j = 0; j < 9; ++j) b->rates[j] = s[j];
+    for (int j = 9; j < 19; ++j) b->rates[j] = 0.0;
+  }
+}
+
+int axis_lo(const flb_block_s* b, int ax) { return b->desc.periodic[ax] ? 0 : b->desc.overlap; }
+int axis_ext(const flb_block_s* b, int ax) {
+  const int n = ax == 0 ? b->desc.nx : (ax == 1 ? b->desc.ny : b->desc.nz);
+  if (ax == 2 && b->d == 2) return 1;
+  return b->desc.periodic[ax] ? n : n - 2 * b->desc.overlap;
+}
+
+int family_of(const flb_block_s* b);
+
+// Step kernel form and cells per thread along x. 0 selects the cell form
+// (one cell per thread, any periodic axis); 1, 2, 4 the row form, where one
+// workgroup of at most 4 wavefronts covers a whole row.
+int vec_of(const flb_block_s* b) {
+  const bool per = b->desc.periodic[0] || b->desc.periodic[1] || b->desc.periodic[2];
+  if (per) return 0;
+  int v = b->vec;
+  if (v <= 0) return 0;
+  if (family_of(b) == flb::FAM_FULL && v > 2) v = 2;
+  const int ex = b->desc.nx - 2 * b->desc.overlap;
+  while (v <= 4 && (ex + flb::kWave * v - 1) / (flb::kWave * v) > flb::kMaxThreads / flb::kWave) v *= 2;
+  if (v > 4) return 0;
+  return v;
+}
+
+KArgs make_args(const flb_block_s* b) {
+  KArgs a{};
+  a.pops = static_cast<char*>(b->d_pops) + flb::kFrontPad * b->elem;
+  a.dpop = b->dpop;
+  a.ypop = b->ypop;
+  a.zpop = b->zpop;
+  a.flags = b->d_flags + flb::kFrontPad;
+  a.rho_f = b->d_rho ? static_cast<char*>(b->d_rho) + flb::kFrontPad * b->real : nullptr;
+  a.u_f = b->d_u ? static_cast<char*>(b->d_u) + flb::kFrontPad * b->real : nullptr;
+  a.fstride = b->stride;
+  a.nx = b->desc.nx;
+  a.ny = b->desc.ny;
+  a.nz = b->desc.nz;
+  a.xs = axis_lo(b, 0);
+  a.ys = axis_lo(b, 1);
+  a.zs = b->d == 3 ? axis_lo(b, 2) : 0;
+  a.ex = axis_ext(b, 0);
+  a.ey = axis_ext(b, 1);
+  a.ez = axis_ext(b, 2);
+  a.pitch = b->pitch;
+  a.plane = b->plane;
+  a.px = b->desc.periodic[0] != 0;
+  a.py = b->desc.periodic[1] != 0;
+  a.pz = b->d == 3 && b->desc.periodic[2] != 0;
+  a.vec = vec_of(b);
+  if (a.vec > 0) {
+    // row form: one workgroup per row
+    a.groups = (a.ex + a.vec - 1) / a.vec;
+    a.wpr = (a.groups + flb::kWave - 1) / flb::kWave;
+    a.wpb = a.wpr;
+    a.nblocks = a.ey * a.ez;
+  } else {
+    a.groups = a.ex;
+    a.wpr = (a.ex + flb::kWave - 1) / flb::kWave;
+    a.wpb = b->wpb;
+    const int64_t waves = int64_t(a.wpr) * a.ey * a.ez;
+    a.nblocks = int((waves + a.wpb - 1) / a.wpb);
+  }
+  a.ntasks = b->ntasks;
+  for (int t = 0; t < b->ntasks; ++t) {
+    a.tmask[t] = b->tasks[t].mask;
+    a.tdyn[t] = b->tasks[t].dynamics;
+    a.tcs[t] = b->tasks[t].smagorinsky;
+  }
+  a.nbc = b->nbc;
+  for (int k = 0; k < b->nbc; ++k) {
+    a.bcmask[k] = b->bcs[k].cell_mask;
+    a.bcvoid[k] = b->bcs[k].void_mask;
+    a.bckind[k] = b->bcs[k].kind;
+  }
+  a.omega = b->desc.omega;
+  for (int j = 0; j < 19; ++j) a.rates[j] = b->rates[j];
+  return a;
+}
+
+// Slot of host cell (x, y, z) in the padded device layout.
+int64_t slot_of(const flb_block_s* b, int x, int y, int z) {
+  return int64_t(x - axis_lo(b, 0)) + int64_t(b->pitch) * y + b->plane * z;
+}
+
+// Host dense array -> padded slot layout (values outside cells stay `fill`).
+template <class T, class U>
+void to_slots(const flb_block_s* b, const U* dense, T* slots, T fill) {
+  for (int64_t k = 0; k < b->stride; ++k) slots[k] = fill;
+  const int nx = b->desc.nx, ny = b->desc.ny, nz = b->desc.nz;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        slots[flb::kFrontPad + slot_of(b, x, y, z)] =
+            T(dense[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * z)]);
+}
+
+int family_of(const flb_block_s* b) {
+  for (int t = 0; t < b->ntasks; ++t) {
+    const int d = b->tasks[t].dynamics;
+    if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT) return flb::FAM_FULL;
+  }
+  return flb::FAM_BGK;
+}
+
+// A non-periodic axis's overlap layer is not visited by the kernels; a cell
+// there that matches a task would be updated by the reference (its loop runs
+// over every cell), so such a layout is rejected rather than half-served.
+int check_layout(const flb_block_s* b) {
+  if (!b->flags_set) return FLB_OK;
+  const int nx = b->desc.nx, ny = b->desc.ny, nz = b->desc.nz;
+  const int ov = b->desc.overlap;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const bool outer = (!b->desc.periodic[0] && (x < ov || x >= nx - ov)) ||
+                           (!b->desc.periodic[1] && (y < ov || y >= ny - ov)) ||
+                           (b->d == 3 && !b->desc.periodic[2] && (z < ov || z >= nz - ov));
+        (void)outer;
+        const uint8_t fl = b->h_flags[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * z)];
+        int dyn = FLB_DYN_NONE, task = -1;
+        for (int t = 0; t < b->ntasks; ++t) {
+          if (fl & b->tasks[t].mask) {
+            dyn = b->tasks[t].dynamics;
+            task = t;
+            break;
+          }
+        }
+        if (dyn == FLB_DYN_NONE) continue;
+        if (outer)
+          return fail(FLB_ERR_LAYOUT,
+                      "cell (%d,%d,%d) in the overlap layer has flag 0x%02x matching task %d",
+                      x, y, z, fl, task);
+        // a halfway wall neighbour must not be updated itself: its slots take
+        // the ordinary writes of the links that bounce off it
+        for (int k = 0; k < b->nbc; ++k)
+          if (fl & b->bcs[k].void_mask)
+            return fail(FLB_ERR_LAYOUT,
+                        "cell (%d,%d,%d) with flag 0x%02x is updated by task %d but is a wall "
+                        "(void mask 0x%02x) of halfway boundary %d",
+                        x, y, z, fl, task, b->bcs[k].void_mask, k);
+      }
+  return FLB_OK;
+}
+
+int ensure_stage(flb_block b, size_t bytes) {
+  if (b->stage_bytes >= bytes) return FLB_OK;
+  if (b->d_stage) HIP_TRY(hipFree(b->d_stage));
+  b->d_stage = nullptr;
+  b->stage_bytes = 0;
+  if (hipMalloc(&b->d_stage, bytes) != hipSuccess)
+    return fail(FLB_ERR_NOMEM, "staging allocation of %zu bytes failed", bytes);
+  b->stage_bytes = bytes;
+  return FLB_OK;
+}
+
+// Host doubles (ncomp dense components of N cells) -> device compute-type
+// arrays in the slot layout, one `stride` apart.
+int upload_field(flb_block b, const double* host, int ncomp, void** dst) {
+  const size_t bytes = size_t(b->stride) * ncomp * b->real;
+  if (!*dst) {
+    if (hipMalloc(dst, bytes) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "field allocation of %zu bytes failed", bytes);
+  }
+  if (b->real == 8) {
+    std::vector<double> tmp(size_t(b->stride) * ncomp);
+    for (int c = 0; c < ncomp; ++c) to_slots<double, double>(b, host + c * b->N, tmp.data() + c * b->stride, 0.0);
+    HIP_TRY(hipMemcpyAsync(*dst, tmp.data(), bytes, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  } else {
+    std::vector<float> tmp(size_t(b->stride) * ncomp);
+    for (int c = 0; c < ncomp; ++c) to_slots<float, double>(b, host + c * b->N, tmp.data() + c * b->stride, 0.0f);
+    HIP_TRY(hipMemcpyAsync(*dst, tmp.data(), bytes, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  }
+  return FLB_OK;
+}
+
+#define CHECK_BLOCK(b) \
+  if (!(b)) return fail(FLB_ERR_ARG, "null block handle")
+
+}  // namespace
+
+extern "C" {
+
+const char* flb_last_error(void) { return g_err.c_str(); }
+
+int flb_abi_version(void) { return FLB_ABI_VERSION; }
+
+int flb_block_create(const flb_block_desc* desc, flb_block* out) {
+  if (!desc || !out) return fail(FLB_ERR_ARG, "null descriptor or output");
+  *out = nullptr;
+  if (desc->q != FLB_D2Q9 && desc->q != FLB_D3Q19)
+    return fail(FLB_ERR_ARG, "q must be 9 (D2Q9) or 19 (D3Q19), got %d", desc->q);
+  if (!storage_ok(desc->storage)) return fail(FLB_ERR_ARG, "unknown storage kind %d", desc->storage);
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1)
+    return fail(FLB_ERR_ARG, "extents must be positive (%d,%d,%d)", desc->nx, desc->ny, desc->nz);
+  if (desc->q == FLB_D2Q9 && desc->nz != 1) return fail(FLB_ERR_ARG, "D2Q9 needs nz == 1");
+  if (desc->overlap < 1) return fail(FLB_ERR_ARG, "overlap must be >= 1");
+  const int d = desc->q == FLB_D3Q19 ? 3 : 2;
+  const int n[3] = {desc->nx, desc->ny, desc->nz};
+  for (int ax = 0; ax < d; ++ax) {
+    const int need = desc->periodic[ax] ? 1 : 2 * desc->overlap + 1;
+    if (n[ax] < need) return fail(FLB_ERR_ARG, "axis %d extent %d leaves no interior", ax, n[ax]);
+  }
+  const int64_t N = int64_t(desc->nx) * desc->ny * desc->nz;
+  // row pitch: the updated cells of a row rounded up to whole 1024-element
+  // groups (the widest vector row a workgroup covers), plus one cache line of
+  // padding that takes the odd step's whole-line stores past the row ends
+  const int ex_x = desc->periodic[0] ? desc->nx : desc->nx - 2 * desc->overlap;
+  const int pitch = std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
+  const int64_t plane = int64_t(pitch) * desc->ny;
+  // tail: one extra row plus the widest vector run past a row end
+  const int64_t tail = 2 * int64_t(pitch) + 1024;
+  const int64_t stride = (flb::kFrontPad + plane * desc->nz + tail + 255) / 256 * 256;
+  if (stride >= (int64_t(1) << 31))
+    return fail(FLB_ERR_ARG, "block of %lld padded cells exceeds 2^31", (long long)stride);
+  if (!(desc->omega > 0.0 && desc->omega < 2.0))
+    return fail(FLB_ERR_ARG, "omega must lie in (0, 2), got %g", desc->omega);
+
+  flb_block b = new flb_block_s();
+  b->desc = *desc;
+  b->d = d;
+  if (d == 2) b->desc.periodic[2] = 0;
+  b->N = N;
+  b->stride = stride;
+  b->pitch = pitch;
+  b->plane = plane;
+  // population layout: directions interleaved per row (default), per plane,
+  // or one array per direction
+  const char* lay = std::getenv("FLB_POP_LAYOUT");
+  const std::string layout = lay ? lay : "row";
+  const int64_t q = desc->q;
+  if (layout == "array") {
+    b->dpop = stride;
+    b->ypop = pitch;
+    b->zpop = plane;
+    b->pop_elems = q * stride;
+  } else if (layout == "plane") {
+    b->dpop = plane;
+    b->ypop = pitch;
+    b->zpop = q * plane;
+    b->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
+  } else {
+    b->dpop = pitch;
+    b->ypop = int(q * pitch);
+    b->zpop = q * plane;
+    b->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
+  }
+  b->elem = desc->storage == FLB_STORE_F64 ? 8 : (desc->storage == FLB_STORE_F32 ? 4 : 2);
+  b->real = desc->storage == FLB_STORE_F64 ? 8 : 4;
+  b->stream = static_cast<hipStream_t>(desc->stream);
+  if (hipGetDevice(&b->device) != hipSuccess) {
+    delete b;
+    return fail(FLB_ERR_HIP, "no HIP device");
+  }
+  if (const char* env = std::getenv("FLB_LAUNCH")) {
+    // "vec,waves_per_workgroup", e.g. "4,4" (vec -1: cell form); invalid settings are ignored
+    int v = 0, w = 0;
+    if (std::sscanf(env, "%d,%d", &v, &w) == 2 && (v == -1 || v == 1 || v == 2 || v == 4) && w >= 1 &&
+        w * flb::kWave <= flb::kMaxThreads) {
+      b->vec = v;
+      b->wpb = w;
+    }
+  }
+  const size_t pop_bytes = size_t(b->pop_elems) * b->elem;
+  if (hipMalloc(&b->d_pops, pop_bytes) != hipSuccess) {
+    delete b;
+    return fail(FLB_ERR_NOMEM, "population allocation of %zu bytes failed", pop_bytes);
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&b->d_flags), size_t(b->stride)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&b->d_red), 4 * sizeof(double)) != hipSuccess) {
+    flb_block_destroy(b);
+    return fail(FLB_ERR_NOMEM, "flag allocation failed");
+  }
+  b->h_flags.assign(size_t(N), 0);
+  if (hipMemsetAsync(b->d_flags, 0, size_t(b->stride), b->stream) != hipSuccess ||
+      hipMemsetAsync(b->d_pops, 0, pop_bytes, b->stream) != hipSuccess) {
+    flb_block_destroy(b);
+    return fail(FLB_ERR_HIP, "initial clear failed");
+  }
+  default_rates(b);
+  *out = b;
+  return FLB_OK;
+}
+
+int flb_block_destroy(flb_block b) {
+  if (!b) return FLB_OK;
+  DeviceGuard g(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  else (void)hipDeviceSynchronize();
+  void* ptrs[] = {b->d_pops, b->d_flags, b->d_rho, b->d_u, b->d_partials, b->d_red, b->d_stage};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete b;
+  return FLB_OK;
+}
+
+int flb_get_info(flb_block b, flb_block_info* info) {
+  CHECK_BLOCK(b);
+  if (!info) return fail(FLB_ERR_ARG, "null info");
+  info->q = b->desc.q;
+  info->storage = b->desc.storage;
+  info->nx = b->desc.nx;
+  info->ny = b->desc.ny;
+  info->nz = b->desc.nz;
+  info->overlap = b->desc.overlap;
+  for (int a = 0; a < 3; ++a) info->periodic[a] = b->desc.periodic[a];
+  info->n_cells = b->N;
+  info->n_interior = int64_t(axis_ext(b, 0)) * axis_ext(b, 1) * axis_ext(b, 2);
+  info->stride = b->stride;
+  info->step = b->step;
+  info->parity = b->parity;
+  info->d_pops = b->d_pops;
+  info->d_flags = b->d_flags;
+  return FLB_OK;
+}
+
+int flb_synchronize(flb_block b) {
+  CHECK_BLOCK(b);
+  DeviceGuard g(b->device);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
+  CHECK_BLOCK(b);
+  if (n < 0 || n > flb::kMaxTasks) return fail(FLB_ERR_ARG, "task count %d outside [0, %d]", n, flb::kMaxTasks);
+  if (n > 0 && !tasks) return fail(FLB_ERR_ARG, "null task table");
+  for (int t = 0; t < n; ++t) {
+    if (tasks[t].dynamics > FLB_DYN_SMAG_MRT)
+      return fail(FLB_ERR_ARG, "task %d: unknown dynamics %d", t, tasks[t].dynamics);
+    if (tasks[t].mask == 0) return fail(FLB_ERR_ARG, "task %d: zero mask never matches", t);
+  }
+  flb_task saved[flb::kMaxTasks];
+  const int saved_n = b->ntasks;
+  std::memcpy(saved, b->tasks, sizeof saved);
+  for (int t = 0; t < n; ++t) b->tasks[t] = tasks[t];
+  b->ntasks = n;
+  const int rc = check_layout(b);
+  if (rc != FLB_OK) {
+    std::memcpy(b->tasks, saved, sizeof saved);
+    b->ntasks = saved_n;
+  }
+  return rc;
+}
+
+int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
+  CHECK_BLOCK(b);
+  if (n < 0 || n > flb::kMaxBc) return fail(FLB_ERR_ARG, "boundary count %d outside [0, %d]", n, flb::kMaxBc);
+  if (n > 0 && !bcs) return fail(FLB_ERR_ARG, "null boundary table");
+  for (int k = 0; k < n; ++k) {
+    if (bcs[k].kind != FLB_HWBC_NORMAL && bcs[k].kind != FLB_HWBC_MOVING)
+      return fail(FLB_ERR_ARG, "boundary %d: unknown kind %d", k, bcs[k].kind);
+    if (bcs[k].void_mask == 0)
+      return fail(FLB_ERR_ARG, "boundary %d: void mask 0 (isFlag needs a non-zero flag)", k);
+  }
+  for (int k = 0; k < n; ++k) b->bcs[k] = bcs[k];
+  b->nbc = n;
+  return FLB_OK;
+}
+
+int flb_set_mrt_rates(flb_block b, const double* rates, int q) {
+  CHECK_BLOCK(b);
+  if (!rates) {
+    default_rates(b);
+    return FLB_OK;
+  }
+  if (q != b->desc.q) return fail(FLB_ERR_ARG, "expected %d rates, got %d", b->desc.q, q);
+  for (int j = 0; j < q; ++j) b->rates[j] = std::isnan(rates[j]) ? -1.0 : rates[j];
+  return FLB_OK;
+}
+
+int flb_set_omega(flb_block b, double omega) {
+  CHECK_BLOCK(b);
+  if (!(omega > 0.0 && omega < 2.0)) return fail(FLB_ERR_ARG, "omega must lie in (0, 2), got %g", omega);
+  b->desc.omega = omega;
+  return FLB_OK;
+}
+
+int flb_upload_flags(flb_block b, const uint8_t* flags) {
+  CHECK_BLOCK(b);
+  if (!flags) return fail(FLB_ERR_ARG, "null flags");
+  DeviceGuard g(b->device);
+  std::vector<uint8_t> saved;
+  saved.swap(b->h_flags);
+  b->h_flags.assign(flags, flags + b->N);
+  const bool was_set = b->flags_set;
+  b->flags_set = true;
+  const int rc = check_layout(b);
+  if (rc != FLB_OK) {
+    b->h_flags.swap(saved);
+    b->flags_set = was_set;
+    return rc;
+  }
+  std::vector<uint8_t> slots(size_t(b->stride));
+  to_slots<uint8_t, uint8_t>(b, b->h_flags.data(), slots.data(), uint8_t(0));
+  HIP_TRY(hipMemcpyAsync(b->d_flags, slots.data(), size_t(b->stride), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_set_field_rho(flb_block b, const double* rho) {
+  CHECK_BLOCK(b);
+  if (!rho) return fail(FLB_ERR_ARG, "null rho");
+  DeviceGuard g(b->device);
+  return upload_field(b, rho, 1, &b->d_rho);
+}
+
+int flb_set_field_u(flb_block b, const double* u) {
+  CHECK_BLOCK(b);
+  if (!u) return fail(FLB_ERR_ARG, "null u");
+  DeviceGuard g(b->device);
+  return upload_field(b, u, b->d, &b->d_u);
+}
+
+static int upload_pops(flb_block b, const void* host, size_t host_real) {
+  if (host_real != b->real)
+    return fail(FLB_ERR_ARG, "population precision does not match storage kind %d", b->desc.storage);
+  DeviceGuard g(b->device);
+  const size_t bytes = size_t(b->N) * size_t(b->desc.q) * b->real;
+  int rc = ensure_stage(b, bytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(b->d_stage, host, bytes, hipMemcpyHostToDevice, b->stream));
+  KArgs a = make_args(b);
+  HIP_TRY(launch_scatter(b, a, b->d_stage));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->parity = 0;
+  b->pending = 0;
+  return FLB_OK;
+}
+
+int flb_upload_pops_f32(flb_block b, const float* pops) {
+  CHECK_BLOCK(b);
+  if (!pops) return fail(FLB_ERR_ARG, "null pops");
+  return upload_pops(b, pops, 4);
+}
+
+int flb_upload_pops_f64(flb_block b, const double* pops) {
+  CHECK_BLOCK(b);
+  if (!pops) return fail(FLB_ERR_ARG, "null pops");
+  return upload_pops(b, pops, 8);
+}
+
+static int download_pops(flb_block b, void* host, size_t host_real) {
+  if (host_real != b->real)
+    return fail(FLB_ERR_ARG, "population precision does not match storage kind %d", b->desc.storage);
+  if (b->pending) return fail(FLB_ERR_STATE, "download between ApplyCellDynamics and Stream");
+  DeviceGuard g(b->device);
+  const size_t bytes = size_t(b->N) * size_t(b->desc.q) * b->real;
+  int rc = ensure_stage(b, bytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(b->d_stage, 0, bytes, b->stream));
+  KArgs a = make_args(b);
+  HIP_TRY(launch_gather(b, a, b->d_stage));
+  HIP_TRY(hipMemcpyAsync(host, b->d_stage, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_download_pops_f32(flb_block b, float* pops) {
+  CHECK_BLOCK(b);
+  if (!pops) return fail(FLB_ERR_ARG, "null pops");
+  return download_pops(b, pops, 4);
+}
+
+int flb_download_pops_f64(flb_block b, double* pops) {
+  CHECK_BLOCK(b);
+  if (!pops) return fail(FLB_ERR_ARG, "null pops");
+  return download_pops(b, pops, 8);
+}
+
+int flb_init_equilibrium(flb_block b, const double* rhou) {
+  CHECK_BLOCK(b);
+  if (!rhou) return fail(FLB_ERR_ARG, "null rhou");
+  DeviceGuard g(b->device);
+  const size_t count = size_t(b->N) * size_t(b->d + 1);
+  int rc = ensure_stage(b, count * b->real);
+  if (rc) return rc;
+  if (b->real == 8) {
+    HIP_TRY(hipMemcpyAsync(b->d_stage, rhou, count * 8, hipMemcpyHostToDevice, b->stream));
+  } else {
+    std::vector<float> tmp(count);
+    for (size_t i = 0; i < count; ++i) tmp[i] = float(rhou[i]);
+    HIP_TRY(hipMemcpyAsync(b->d_stage, tmp.data(), count * 4, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  }
+  KArgs a = make_args(b);
+  HIP_TRY(launch_init_feq(b, a, b->d_stage));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->parity = 0;
+  b->pending = 0;
+  return FLB_OK;
+}
+
+static int launch_one(flb_block b) {
+  if (!b->flags_set) return fail(FLB_ERR_STATE, "flags not uploaded");
+  KArgs a = make_args(b);
+  HIP_TRY(launch_step(b, family_of(b), a));
+  return FLB_OK;
+}
+
+int flb_apply_cell_dynamics(flb_block b) {
+  CHECK_BLOCK(b);
+  if (b->pending) return fail(FLB_ERR_STATE, "ApplyCellDynamics called twice without Stream");
+  DeviceGuard g(b->device);
+  const int rc = launch_one(b);
+  if (rc) return rc;
+  b->pending = 1;
+  return FLB_OK;
+}
+
+int flb_stream(flb_block b) {
+  CHECK_BLOCK(b);
+  if (!b->pending) return fail(FLB_ERR_STATE, "Stream without a preceding ApplyCellDynamics");
+  b->pending = 0;
+  b->parity ^= 1;
+  b->step += 1;
+  return FLB_OK;
+}
+
+int flb_apply_halfway_bc(flb_block b) {
+  CHECK_BLOCK(b);
+  if (b->pending) return fail(FLB_ERR_STATE, "halfway boundaries apply after Stream");
+  return FLB_OK;
+}
+
+int flb_step(flb_block b, int n) {
+  CHECK_BLOCK(b);
+  if (n < 0) return fail(FLB_ERR_ARG, "negative step count");
+  if (b->pending) return fail(FLB_ERR_STATE, "step while a Stream is pending");
+  DeviceGuard g(b->device);
+  for (int s = 0; s < n; ++s) {
+    const int rc = launch_one(b);
+    if (rc) return rc;
+    b->parity ^= 1;
+    b->step += 1;
+  }
+  return FLB_OK;
+}
+
+int flb_moments(flb_block b, uint8_t mask, void* rhou) {
+  CHECK_BLOCK(b);
+  if (!rhou) return fail(FLB_ERR_ARG, "null output");
+  if (b->pending) return fail(FLB_ERR_STATE, "moments between ApplyCellDynamics and Stream");
+  DeviceGuard g(b->device);
+  const size_t bytes = size_t(b->N) * size_t(b->d + 1) * b->real;
+  int rc = ensure_stage(b, bytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(b->d_stage, 0, bytes, b->stream));
+  KArgs a = make_args(b);
+  HIP_TRY(launch_moments(b, a, mask, b->d_stage));
+  HIP_TRY(hipMemcpyAsync(rhou, b->d_stage, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]) {
+  CHECK_BLOCK(b);
+  if (!out) return fail(FLB_ERR_ARG, "null output");
+  if (b->pending) return fail(FLB_ERR_STATE, "reduction between ApplyCellDynamics and Stream");
+  DeviceGuard g(b->device);
+  KArgs a = make_args(b);
+  if (!b->d_partials) {
+    // one double4 per workgroup of the per-cell kernels
+    const size_t nb = flb::cell_blocks(b->N);
+    if (hipMalloc(reinterpret_cast<void**>(&b->d_partials), nb * 4 * sizeof(double)) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "reduction scratch allocation failed");
+  }
+  HIP_TRY(launch_reduce(b, a, mask, b->d_red));
+  HIP_TRY(hipMemcpyAsync(out, b->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_bandwidth_probe(size_t bytes, int reps, double* gbs) {
+  if (!gbs || reps < 1 || bytes < 16) return fail(FLB_ERR_ARG, "bad probe arguments");
+  const size_t n4 = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, n4 * 16) != hipSuccess) return fail(FLB_ERR_NOMEM, "probe allocation failed");
+  if (hipMalloc(&dst, n4 * 16) != hipSuccess) {
+    (void)hipFree(src);
+    return fail(FLB_ERR_NOMEM, "probe allocation failed");
+  }
+  hipEvent_t e0, e1;
+  int rc = FLB_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(FLB_ERR_HIP, "event creation failed");
+  } else if (hipMemset(src, 0, n4 * 16) != hipSuccess ||
+             flb::launch_copy_probe(dst, src, n4, nullptr) != hipSuccess ||
+             hipDeviceSynchronize() != hipSuccess) {
+    rc = fail(FLB_ERR_HIP, "probe warm-up failed");
+  } else {
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) (void)flb::launch_copy_probe(dst, src, n4, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
+      rc = fail(FLB_ERR_HIP, "probe timing failed");
+    } else {
+      *gbs = 2.0 * double(n4 * 16) * reps / (double(ms) * 1e-3) / 1e9;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return rc;
+}
+
+}  // extern "C"

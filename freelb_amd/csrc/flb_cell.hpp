@@ -1,0 +1,391 @@
+// Per-cell lattice-Boltzmann arithmetic for the gfx950 kernels.
+//
+// Every function here works on one cell's q populations held in registers.
+// Expression order follows FreeLB's unrolled specialisations (the *.ur.h
+// files, which the reference builds with -D_UNROLLFOR) term by term, and the
+// kernels are compiled with floating-point contraction off, so an FP32 run
+// rounds exactly as FreeLB's FP32 run does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flb {
+
+// ---------------------------------------------------------------------------
+// Velocity sets in FreeLB's order (lbm/lattice_set.h:168-194 D2Q9,
+// 259-296 D3Q19). Opposite pairs are (2k-1, 2k).
+// ---------------------------------------------------------------------------
+template <int Q>
+struct Lat;
+
+template <>
+struct Lat<19> {
+  static constexpr int D = 3;
+  __host__ __device__ static constexpr int cx(int i) {
+    constexpr int v[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+    return v[i];
+  }
+  __host__ __device__ static constexpr int cy(int i) {
+    constexpr int v[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+    return v[i];
+  }
+  __host__ __device__ static constexpr int cz(int i) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+    return v[i];
+  }
+  // weight class: 0 -> 1/3, 1 -> 1/18, 2 -> 1/36
+  __host__ __device__ static constexpr int wclass(int i) { return i == 0 ? 0 : (i <= 6 ? 1 : 2); }
+  template <class R>
+  __host__ __device__ static constexpr R w(int i) {
+    return wclass(i) == 0 ? R(1) / R(3) : (wclass(i) == 1 ? R(1) / R(18) : R(1) / R(36));
+  }
+};
+
+template <>
+struct Lat<9> {
+  static constexpr int D = 2;
+  __host__ __device__ static constexpr int cx(int i) {
+    constexpr int v[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+    return v[i];
+  }
+  __host__ __device__ static constexpr int cy(int i) {
+    constexpr int v[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+    return v[i];
+  }
+  __host__ __device__ static constexpr int cz(int) { return 0; }
+  __host__ __device__ static constexpr int wclass(int i) { return i == 0 ? 0 : (i <= 4 ? 1 : 2); }
+  template <class R>
+  __host__ __device__ static constexpr R w(int i) {
+    return wclass(i) == 0 ? R(4) / R(9) : (wclass(i) == 1 ? R(1) / R(9) : R(1) / R(36));
+  }
+};
+
+__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+// ---------------------------------------------------------------------------
+// Moments: moment::rhoUImpl (moment.ur.h:461-472 D2Q9, 505-517 D3Q19).
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ void rhou(const R (&c)[19], R& rho, R (&u)[3]) {
+  const R r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] +
+              c[11] + c[12] + c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+  rho = r;
+  u[0] = (c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16]) / r;
+  u[1] = (c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18]) / r;
+  u[2] = (c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18]) / r;
+}
+
+template <class R>
+__device__ __forceinline__ void rhou(const R (&c)[9], R& rho, R (&u)[3]) {
+  const R r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+  rho = r;
+  u[0] = (c[1] - c[2] + c[5] - c[6] + c[7] - c[8]) / r;
+  u[1] = (c[3] - c[4] + c[5] - c[6] - c[7] + c[8]) / r;
+  u[2] = R(0);
+}
+
+// ---------------------------------------------------------------------------
+// equilibrium::SecondOrderImpl (equilibrium.ur.h:65-94 D2Q9, 165-214 D3Q19).
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ void feq(R (&e)[19], R rho, const R (&u)[3]) {
+  const R InvCs2 = R(3);
+  const R InvCs2x = R(0.5) * InvCs2;
+  const R var0 = R(1) - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * InvCs2x;
+  const R rhowk0 = Lat<19>::w<R>(0) * rho;
+  const R rhowk1 = Lat<19>::w<R>(1) * rho;
+  const R rhowk7 = Lat<19>::w<R>(7) * rho;
+  const R cu[9] = {u[0],        u[1],        u[2],        u[0] + u[1], u[0] + u[2],
+                   u[1] + u[2], u[0] - u[1], u[0] - u[2], u[1] - u[2]};
+  e[0] = rhowk0 * var0;
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    const R a = InvCs2 * cu[p];
+    const R v = var0 + a * a * R(0.5);
+    const R rw = p < 3 ? rhowk1 : rhowk7;
+    e[2 * p + 1] = rw * (v + a);
+    e[2 * p + 2] = rw * (v - a);
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void feq(R (&e)[9], R rho, const R (&u)[3]) {
+  const R InvCs2 = R(3);
+  const R InvCs2x = R(0.5) * InvCs2;
+  const R var0 = R(1) - (u[0] * u[0] + u[1] * u[1]) * InvCs2x;
+  const R rhowk0 = Lat<9>::w<R>(0) * rho;
+  const R rhowk1 = Lat<9>::w<R>(1) * rho;
+  const R rhowk5 = Lat<9>::w<R>(5) * rho;
+  const R cu[4] = {u[0], u[1], u[0] + u[1], u[0] - u[1]};
+  e[0] = rhowk0 * var0;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const R a = InvCs2 * cu[p];
+    const R v = var0 + a * a * R(0.5);
+    const R rw = p < 2 ? rhowk1 : rhowk5;
+    e[2 * p + 1] = rw * (v + a);
+    e[2 * p + 2] = rw * (v - a);
+  }
+}
+
+// c_i . u accumulated from zero over the axes (Vector.h:229-236).
+template <int Q, class R>
+__device__ __forceinline__ R cdotu(int i, const R (&u)[3]) {
+  R r = R(0);
+  r += u[0] * R(Lat<Q>::cx(i));
+  r += u[1] * R(Lat<Q>::cy(i));
+  if (Lat<Q>::D == 3) r += u[2] * R(Lat<Q>::cz(i));
+  return r;
+}
+
+// Halfway moving-wall term 2*InvCs2*w_k*RHO*(VELOCITY.c_k)
+// (bounce_back_boundary.h:246-250).
+template <int Q, class R>
+__device__ __forceinline__ R movingwall_term(int k, R rho_f, const R (&u_f)[3]) {
+  return R(6) * Lat<Q>::template w<R>(k) * rho_f * cdotu<Q>(k, u_f);
+}
+
+// ---------------------------------------------------------------------------
+// Collisions.
+// ---------------------------------------------------------------------------
+
+// collision::BGK (collision.h:39-56): f_i <- omega*feq_i + (1-omega)*f_i.
+template <int Q, class R>
+__device__ __forceinline__ void bgk_with(R (&f)[Q], R rho, const R (&u)[3], R omega, R _omega) {
+  R e[Q];
+  feq(e, rho, u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = omega * e[i] + _omega * f[i];
+}
+
+// collision::BounceBack (collision.ur.h:44-93).
+template <int Q, class R>
+__device__ __forceinline__ void bounceback(R (&f)[Q]) {
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const R t = f[i];
+    f[i] = f[i + 1];
+    f[i + 1] = t;
+  }
+}
+
+// collision::BounceBackMovingWall (collision.ur.h:97-187): uc_i = (c_i.u) w_i rhox,
+// where c_i.u sums only the non-zero components (u0, u0+u1, u0-u1, ...).
+template <int Q, class R>
+__device__ __forceinline__ void bounceback_moving(R (&f)[Q], R rho_f, const R (&u)[3]) {
+  const R rhox = R(6) * rho_f;
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const int c[3] = {Lat<Q>::cx(i), Lat<Q>::cy(i), Lat<Q>::cz(i)};
+    R cu = R(0);
+    bool first = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (c[a] == 0) continue;
+      if (first) {
+        cu = c[a] > 0 ? u[a] : -u[a];
+        first = false;
+      } else {
+        cu = c[a] > 0 ? cu + u[a] : cu - u[a];
+      }
+    }
+    const R uc = cu * Lat<Q>::template w<R>(i) * rhox;
+    const R t = f[i];
+    f[i] = f[i + 1] + uc;
+    f[i + 1] = t - uc;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T dsqrt(T v);
+template <>
+__device__ __forceinline__ float dsqrt<float>(float v) {
+  return sqrtf(v);
+}
+template <>
+__device__ __forceinline__ double dsqrt<double>(double v) {
+  return sqrt(v);
+}
+
+// Smagorinsky effective omega (collisionLES.h:36-59): |Pi_neq| from
+// Pi_ab_neq (moment.ur.h:708-718, 750-763) and NormSquare (util.h:418-434);
+// the coefficient is formed in double because std::sqrt(2) is a double.
+template <class R>
+__device__ __forceinline__ R smag_omega(const R (&c)[19], R rho, const R (&u)[3], R omega, R cs,
+                                        R rho_f) {
+  const R cs2 = R(1) / R(3);
+  const R t0 = c[1] + c[2] + c[7] + c[8] + c[9] + c[10] + c[13] + c[14] + c[15] + c[16] -
+               rho * (u[0] * u[0] + cs2);
+  const R t1 = c[7] + c[8] - c[13] - c[14] - rho * (u[0] * u[1]);
+  const R t2 = c[9] + c[10] - c[15] - c[16] - rho * (u[0] * u[2]);
+  const R t3 = c[3] + c[4] + c[7] + c[8] + c[11] + c[12] + c[13] + c[14] + c[17] + c[18] -
+               rho * (u[1] * u[1] + cs2);
+  const R t4 = c[11] + c[12] - c[17] - c[18] - rho * (u[1] * u[2]);
+  const R t5 = c[5] + c[6] + c[9] + c[10] + c[11] + c[12] + c[15] + c[16] + c[17] + c[18] -
+               rho * (u[2] * u[2] + cs2);
+  const R ns = t0 * t0 + t1 * t1 * R(2) + t2 * t2 * R(2) + t3 * t3 + t4 * t4 * R(2) + t5 * t5;
+  const R norm = dsqrt(ns);
+  const R coeff =
+      R(double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951 / double(rho_f));
+  const R rt_mol = R(1) / omega;
+  const R rt_tur = R(0.5) * (dsqrt(rt_mol * rt_mol + coeff * norm) - rt_mol);
+  return R(1) / (rt_mol + rt_tur);
+}
+
+template <class R>
+__device__ __forceinline__ R smag_omega(const R (&c)[9], R rho, const R (&u)[3], R omega, R cs,
+                                        R rho_f) {
+  const R cs2 = R(1) / R(3);
+  const R t0 = c[1] + c[2] + c[5] + c[6] + c[7] + c[8] - rho * (u[0] * u[0] + cs2);
+  const R t1 = c[5] + c[6] - c[7] - c[8] - rho * (u[0] * u[1]);
+  const R t2 = c[3] + c[4] + c[5] + c[6] + c[7] + c[8] - rho * (u[1] * u[1] + cs2);
+  const R ns = t0 * t0 + t1 * t1 * R(2) + t2 * t2;
+  const R norm = dsqrt(ns);
+  const R coeff =
+      R(double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951 / double(rho_f));
+  const R rt_mol = R(1) / omega;
+  const R rt_tur = R(0.5) * (dsqrt(rt_mol * rt_mol + coeff * norm) - rt_mol);
+  return R(1) / (rt_mol + rt_tur);
+}
+
+// ---------------------------------------------------------------------------
+// MRT. D2Q9 as collision::MRT_Feq_RhoU<D2Q9> (collisionMRT.ur.h:45-129) with
+// the data of collisionMRT.h:55-90. D3Q19 on the orthogonal basis of
+// d'Humieres et al. (2002): m = M(f - feq), f <- f - M^T diag(s/|M_j|^2) m.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int m19(int j, int i) {
+  // rows: rho, e, eps, jx, qx, jy, qy, jz, qz, 3pxx, 3pixx, pww, piww, pxy, pyz, pxz, mx, my, mz
+  const int x = Lat<19>::cx(i), y = Lat<19>::cy(i), z = Lat<19>::cz(i);
+  const int c2 = x * x + y * y + z * z;
+  switch (j) {
+    case 0: return 1;
+    case 1: return 19 * c2 - 30;
+    case 2: return (21 * c2 * c2 - 53 * c2 + 24) / 2;
+    case 3: return x;
+    case 4: return (5 * c2 - 9) * x;
+    case 5: return y;
+    case 6: return (5 * c2 - 9) * y;
+    case 7: return z;
+    case 8: return (5 * c2 - 9) * z;
+    case 9: return 3 * x * x - c2;
+    case 10: return (3 * c2 - 5) * (3 * x * x - c2);
+    case 11: return y * y - z * z;
+    case 12: return (3 * c2 - 5) * (y * y - z * z);
+    case 13: return x * y;
+    case 14: return y * z;
+    case 15: return x * z;
+    case 16: return (y * y - z * z) * x;
+    case 17: return (z * z - x * x) * y;
+    default: return (x * x - y * y) * z;
+  }
+}
+
+__host__ __device__ constexpr int m19_norm(int j) {
+  int n = 0;
+  for (int i = 0; i < 19; ++i) n += m19(j, i) * m19(j, i);
+  return n;
+}
+
+// Default D3Q19 rates; a negative entry marks the shear rates (take omega).
+__host__ __device__ constexpr double m19_default_rate(int j) {
+  constexpr double s[19] = {0.0, 1.19, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 1.2, -1.0,
+                            1.4, -1.0, 1.4, -1.0, -1.0, -1.0, 1.98, 1.98, 1.98};
+  return s[j];
+}
+
+// rates[j] < 0 or NaN marks a shear rate, which takes om
+template <class R>
+__device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, const double* rates) {
+  R e[19];
+  feq(e, rho, u);
+  R g[19];
+#pragma unroll
+  for (int i = 0; i < 19; ++i) g[i] = f[i] - e[i];
+  R k[19];
+#pragma unroll
+  for (int j = 0; j < 19; ++j) {
+    const double sr = rates[j];
+    const R s = (sr >= 0.0) ? R(sr) : om;
+    R dm = R(0);
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+      if (m19(j, i) != 0) dm += R(m19(j, i)) * g[i];
+    k[j] = s * dm / R(m19_norm(j));
+  }
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < 19; ++j)
+      if (m19(j, i) != 0) s += R(m19(j, i)) * k[j];
+    f[i] -= s;
+  }
+}
+
+__host__ __device__ constexpr int m9(int i, int j) {
+  constexpr int M[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},      {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+                           {4, -2, -2, -2, -2, 1, 1, 1, 1},  {0, 1, -1, 0, 0, 1, -1, 1, -1},
+                           {0, -2, 2, 0, 0, 1, -1, 1, -1},   {0, 0, 0, 1, -1, 1, -1, -1, 1},
+                           {0, 0, 0, -2, 2, 1, -1, -1, 1},   {0, 1, 1, -1, -1, 0, 0, 0, 0},
+                           {0, 0, 0, 0, 0, 1, 1, -1, -1}};
+  return M[i][j];
+}
+__host__ __device__ constexpr int invm9_n(int i, int j) {
+  constexpr int N[9][9] = {{1, -1, 1, 0, 0, 0, 0, 0, 0},     {1, -1, -1, 1, -1, 0, 0, 1, 0},
+                           {1, -1, -1, -1, 1, 0, 0, 1, 0},   {1, -1, -1, 0, 0, 1, -1, -1, 0},
+                           {1, -1, -1, 0, 0, -1, 1, -1, 0},  {1, 1, 1, 1, 1, 1, 1, 0, 1},
+                           {1, 1, 1, -1, -1, -1, -1, 0, 1},  {1, 1, 1, 1, 1, -1, -1, 0, -1},
+                           {1, 1, 1, -1, -1, 1, 1, 0, -1}};
+  return N[i][j];
+}
+__host__ __device__ constexpr int invm9_d(int i, int j) {
+  constexpr int Dn[9][9] = {{9, 9, 9, 1, 1, 1, 1, 1, 1},     {9, 36, 18, 6, 6, 1, 1, 4, 1},
+                            {9, 36, 18, 6, 6, 1, 1, 4, 1},   {9, 36, 18, 1, 1, 6, 6, 4, 1},
+                            {9, 36, 18, 1, 1, 6, 6, 4, 1},   {9, 18, 36, 6, 12, 6, 12, 1, 4},
+                            {9, 18, 36, 6, 12, 6, 12, 1, 4}, {9, 18, 36, 6, 12, 6, 12, 1, 4},
+                            {9, 18, 36, 6, 12, 6, 12, 1, 4}};
+  return Dn[i][j];
+}
+
+template <class R>
+__device__ __forceinline__ void mrt(R (&f)[9], R rho, const R (&u)[3], R om, const double* rates) {
+  R rt[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const double sr = rates[j];
+    rt[j] = (sr >= 0.0) ? R(sr) : om;
+  }
+  R m[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) s += R(m9(i, j)) * f[j];
+    m[i] = s;
+  }
+  R dm[9];
+  const R ux2 = u[0] * u[0];
+  const R uy2 = u[1] * u[1];
+  const R rhoux = rho * u[0];
+  const R rhouy = rho * u[1];
+  const R cse0 = R(3) * rho * (ux2 + uy2);
+  dm[0] = m[0] - rho;
+  dm[1] = m[1] + R(2) * rho - cse0;
+  dm[2] = m[2] - R(9) * rho * ux2 * uy2 + cse0 - rho;
+  dm[3] = m[3] - rhoux;
+  dm[4] = m[4] - rhoux * (R(3) * uy2 - R(1));
+  dm[5] = m[5] - rhouy;
+  dm[6] = m[6] - rhouy * (R(3) * ux2 - R(1));
+  dm[7] = m[7] - rho * (ux2 - uy2);
+  dm[8] = m[8] - rho * u[0] * u[1];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) s += ((R(invm9_n(i, j)) / R(invm9_d(i, j))) * rt[j]) * dm[j];
+    f[i] -= s;
+  }
+}
+
+}  // namespace flb

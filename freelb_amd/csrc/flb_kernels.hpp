@@ -1,0 +1,908 @@
+// Lattice kernels for gfx950: the fused AA-pattern collide-and-stream step
+// and its read-out companions (templates; instantiated per storage kind in
+// flb_inst_*.hip).
+//
+// AA (esoteric) in-place streaming, one array per direction (SoA):
+//   even step  read  f_i <- A[i][x]              write A[opp i][x]   <- f*_i
+//   odd step   read  f_i <- A[opp i][x - c_i]    write A[i][x + c_i] <- f*_i
+// After an odd step the arrays hold the natural layout (A[i][x] = post-stream
+// f_i(x)); after an even step A[opp i][x] holds the post-collision f*_i(x),
+// i.e. the streamed state is implicit. Each cell update reads and writes its
+// q populations exactly once: 2*q*sizeof(element) bytes per cell.
+//
+// Halfway bounce-back (FreeLB's bounceback::normal / movingwall, applied after
+// Stream from getPrevious) needs no extra pass in this pattern. At an odd step
+// a link whose upstream neighbour x - c_i is a wall reads the cell's own
+// post-collision f*_{opp i}, which the even step left in A[i][x]; a link whose
+// downstream neighbour x + c_i is a wall additionally writes f*_i into
+// A[opp i][x], where the next even step reads f_{opp i}. The ordinary write of
+// such a link lands in the wall cell's slot, which no halfway cell reads, and
+// A[opp i][x] has no other writer, because its ordinary writer would be the
+// (not updated) wall cell.
+//
+// Vector width. Each thread updates VEC consecutive cells of one row, so the
+// even step moves whole VEC-element vectors per direction (rows start
+// 256-byte aligned in the slot layout). At the odd step the directions with
+// c_x = +-1 are one element off the vector grid: a thread loads its aligned
+// vector and takes the missing element from the neighbouring lane with a
+// cross-lane shuffle (only the first / last lane of a wavefront load it from
+// memory), and writes are assembled the same way, so every direction still
+// moves aligned vectors. A wavefront never spans two rows.
+//
+// Workgroups are remapped so that the workgroups the hardware places on one
+// XCD (consecutive IDs go round-robin over the 8 XCDs) cover one contiguous
+// slab of the block: the y/z neighbours an odd step reads then mostly sit in
+// that XCD's own L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "flb_cell.hpp"
+#include "flb_internal.hpp"
+#include "flb_storage.hpp"
+
+namespace flb {
+namespace kern {
+
+enum : int {
+  DYN_NONE = 0,
+  DYN_BGK = 1,
+  DYN_BGK_FIELD = 2,
+  DYN_BB = 3,
+  DYN_BB_MW = 4,
+  DYN_MRT = 5,
+  DYN_SMAG_BGK = 6,
+  DYN_SMAG_MRT = 7
+};
+enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1 };
+
+// Logical workgroup index: the hardware hands consecutive workgroups to the
+// 8 XCDs in turn, so workgroup b runs on XCD b % 8 as its (b / 8)-th one.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb >> 3, rem = nb & 7u, xcd = b & 7u, slot = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
+template <class S>
+__device__ __forceinline__ typename S::E* arr(const KArgs& a, int i) {
+  return static_cast<typename S::E*>(a.pops) + int64_t(i) * a.dpop;
+}
+
+// Row / plane strides of the population slots (POP) or of the per-cell
+// arrays -- flags and fields (!POP).
+template <bool POP>
+__device__ __forceinline__ int ystr(const KArgs& a) {
+  return POP ? a.ypop : a.pitch;
+}
+template <bool POP>
+__device__ __forceinline__ int zstr(const KArgs& a) {
+  return POP ? int(a.zpop) : int(a.plane);
+}
+
+// Slot offset of the y/z part of c_i for a cell in row (y, z), wrapping
+// periodic axes (uniform over a wavefront, which covers one row).
+template <int Q, bool PER, bool POP>
+__device__ __forceinline__ int yz_off(const KArgs& a, int i, int y, int z) {
+  const int cy = Lat<Q>::cy(i), cz = Lat<Q>::cz(i);
+  int o = cy * ystr<POP>(a) + cz * zstr<POP>(a);
+  if (PER) {
+    if (a.py && cy != 0) {
+      const int yn = y + cy;
+      if (yn < 0) o += a.ny * ystr<POP>(a);
+      else if (yn >= a.ny) o -= a.ny * ystr<POP>(a);
+    }
+    if (Lat<Q>::D == 3 && a.pz && cz != 0) {
+      const int zn = z + cz;
+      if (zn < 0) o += a.nz * zstr<POP>(a);
+      else if (zn >= a.nz) o -= a.nz * zstr<POP>(a);
+    }
+  }
+  return o;
+}
+
+// Full slot offset of the neighbour x + c_i.
+template <int Q, bool PER, bool POP>
+__device__ __forceinline__ int nb_off(const KArgs& a, int i, int x, int y, int z) {
+  const int cx = Lat<Q>::cx(i);
+  int o = cx + yz_off<Q, PER, POP>(a, i, y, z);
+  if (PER && a.px && cx != 0) {
+    const int xn = x + cx;
+    if (xn < 0) o += a.nx;
+    else if (xn >= a.nx) o -= a.nx;
+  }
+  return o;
+}
+
+// First task of the table matching the flag (TaskSelector, utils/tmp.h:283-309).
+__device__ __forceinline__ int match_task(const KArgs& a, uint8_t fl, float& cs) {
+  for (int t = 0; t < a.ntasks; ++t) {
+    if (fl & a.tmask[t]) {
+      cs = a.tcs[t];
+      return a.tdyn[t];
+    }
+  }
+  return DYN_NONE;
+}
+
+// Halfway wall links of a cell (bit k: neighbour x + c_k is a wall), split by
+// kind; managers apply in order so a later match overrides an earlier one
+// (BlockFixedBoundary::addtoBd, basic_boundary.h:198-212).
+template <int Q, bool PER>
+__device__ __forceinline__ void wall_links(const KArgs& a, int x, int y, int z, int slot,
+                                           uint8_t fl, unsigned& ln, unsigned& lm) {
+  ln = 0u;
+  lm = 0u;
+  for (int k = 0; k < a.nbc; ++k) {
+    if (!(fl & a.bcmask[k])) continue;
+    const uint8_t vm = a.bcvoid[k];
+    unsigned v = 0u;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (a.flags[slot + nb_off<Q, PER, false>(a, i, x, y, z)] & vm) v |= 1u << i;
+    }
+    if (a.bckind[k] == HWBC_MOVING) {
+      lm |= v;
+      ln &= ~v;
+    } else {
+      ln |= v;
+      lm &= ~v;
+    }
+  }
+}
+
+template <int Q, class R>
+__device__ __forceinline__ void load_fields(const KArgs& a, int slot, R& rho_f, R (&u_f)[3]) {
+  rho_f = a.rho_f ? static_cast<const R*>(a.rho_f)[slot] : R(1);
+  const R* u = static_cast<const R*>(a.u_f);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u_f[d] = (u && d < Lat<Q>::D) ? u[d * a.fstride + slot] : R(0);
+}
+
+// One cell's collision. The RHO / VELOCITY fields of the cell (slot) are read
+// here, by the dynamics that use them, so they hold no registers elsewhere.
+template <int Q, class R, int FAM>
+__device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
+                                        R om, R _om) {
+  R rho, u[3], rho_f, u_f[3];
+  switch (dyn) {
+    case DYN_BGK:
+      rhou(f, rho, u);
+      bgk_with<Q>(f, rho, u, om, _om);
+      break;
+    case DYN_BGK_FIELD:
+      load_fields<Q>(a, slot, rho_f, u_f);
+      bgk_with<Q>(f, rho_f, u_f, om, _om);
+      break;
+    case DYN_BB:
+      bounceback<Q>(f);
+      break;
+    case DYN_BB_MW:
+      load_fields<Q>(a, slot, rho_f, u_f);
+      bounceback_moving<Q>(f, rho_f, u_f);
+      break;
+    default:
+      if constexpr (FAM == FAM_FULL) {
+        rhou(f, rho, u);
+        R eo = om;
+        if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
+          load_fields<Q>(a, slot, rho_f, u_f);
+          eo = smag_omega(f, rho, u, om, R(cs), rho_f);
+        }
+        if (dyn == DYN_SMAG_BGK) {
+          bgk_with<Q>(f, rho, u, eo, R(1) - eo);
+        } else {
+          mrt(f, rho, u, eo, a.rates);
+        }
+      }
+      break;
+  }
+}
+
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1): lane l receives
+// lane l-1's (shr) or lane l+1's (shl) value; the lane with no source gets 0.
+__device__ __forceinline__ float wave_shr(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_shl(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double wave_shr(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(uint64_t(b) >> 32)), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+}
+__device__ __forceinline__ double wave_shl(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(uint64_t(b) >> 32)), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+}
+__device__ __forceinline__ unsigned wave_shr(unsigned v) {
+  return unsigned(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ unsigned wave_shl(unsigned v) {
+  return unsigned(__builtin_amdgcn_update_dpp(0, int(v), 0x130, 0xf, 0xf, false));
+}
+
+template <int VEC>
+__device__ __forceinline__ void load_flags(const uint8_t* p, uint8_t (&fl)[VEC]) {
+  if constexpr (VEC == 1) {
+    fl[0] = p[0];
+  } else if constexpr (VEC == 2) {
+    const uint16_t w = *reinterpret_cast<const uint16_t*>(p);
+    fl[0] = uint8_t(w);
+    fl[1] = uint8_t(w >> 8);
+  } else {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) fl[j] = uint8_t(w >> (8 * j));
+  }
+}
+
+// VEC elements of one direction at p (aligned), converted to the compute type.
+template <int Q, class S, int VEC>
+__device__ __forceinline__ void load_row(const typename S::E* p, int i, typename S::R (&v)[VEC]) {
+  if constexpr (VEC == 1) {
+    v[0] = S::template from<Q>(p[0], i);
+  } else {
+    const auto w = load_vec<typename S::E, VEC>(p);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) v[j] = S::template from<Q>(w[j], i);
+  }
+}
+
+// Store the elements of v whose bit is set in `mask` (full mask: one vector store).
+template <int Q, class S, int VEC>
+__device__ __forceinline__ void store_row(typename S::E* p, int i, const typename S::R (&v)[VEC],
+                                          unsigned mask) {
+  using E = typename S::E;
+  constexpr unsigned full = (1u << VEC) - 1u;
+  if constexpr (VEC == 1) {
+    if (mask & 1u) p[0] = S::template to<Q>(v[0], i);
+  } else {
+    if (mask == full) {
+      vec_t<E, VEC> w;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) w[j] = S::template to<Q>(v[j], i);
+      store_vec<E, VEC>(p, w);
+    } else if (mask) {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j)
+        if ((mask >> j) & 1u) p[j] = S::template to<Q>(v[j], i);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The step kernel, cell form: one cell per thread, any axis periodic.
+// ---------------------------------------------------------------------------
+template <int Q, class S, int PAR, int FAM, bool PER>
+__global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
+  using R = typename S::R;
+  using E = typename S::E;
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
+  const int row = gw / a.wpr;
+  if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
+  const int wv = gw - row * a.wpr;
+  const int y = a.ys + row % a.ey;
+  const int z = a.zs + row / a.ey;
+  const int c0 = wv * kWave + lane;
+  if (c0 >= a.ex) return;
+  const int x = a.xs + c0;
+  const int s0 = c0 + a.pitch * y + int(a.plane) * z;
+  E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
+  const uint8_t fl = a.flags[s0];
+  float cs = 0.0f;
+  const int dyn = match_task(a, fl, cs);
+  if (dyn == DYN_NONE) return;
+  unsigned ln = 0u, lm = 0u;
+  if (PAR == 1 && a.nbc) wall_links<Q, PER>(a, x, y, z, s0, fl, ln, lm);
+  const unsigned lk = ln | lm;
+  R f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (PAR == 0) {
+      f[i] = S::template from<Q>((P + int64_t(i) * a.dpop)[c0], i);
+    } else {
+      // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
+      const int o = opp(i);
+      const bool wall = i != 0 && ((lk >> o) & 1u);
+      const E* p = wall ? (P + int64_t(i) * a.dpop) + c0
+                        : (P + int64_t(o) * a.dpop) + c0 + nb_off<Q, PER, true>(a, o, x, y, z);
+      f[i] = S::template from<Q>(*p, i);
+    }
+  }
+  if (PAR == 1 && lm) {
+    R rho_f, u_f[3];
+    load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+    for (int i = 1; i < Q; ++i)
+      if ((lm >> opp(i)) & 1u) f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
+  }
+  const R om = R(a.omega);
+  collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
+  R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
+  if (PAR == 1 && lm) load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (PAR == 0) {
+      (P + int64_t(opp(i)) * a.dpop)[c0] = S::template to<Q>(f[i], i);
+    } else {
+      // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
+      const bool wall = i != 0 && ((lk >> i) & 1u);
+      R v = f[i];
+      if (i != 0 && ((lm >> i) & 1u)) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
+      E* p = wall ? (P + int64_t(opp(i)) * a.dpop) + c0
+                  : (P + int64_t(i) * a.dpop) + c0 + nb_off<Q, PER, true>(a, i, x, y, z);
+      *p = S::template to<Q>(v, i);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The step kernel, row form: one workgroup per row, VEC cells per thread, no
+// periodic axis. Every store is an aligned vector and, at the odd step, the
+// stores of each direction cover whole 128-byte lines: a cell whose source
+// lies outside the row (the overlap layer, the row padding) contributes an
+// arbitrary value, written only into slots that no cell ever reads (ghost
+// slots of directions pointing out of the block, row padding) or that the same
+// thread overwrites right after with its halfway value. Partial-line writes
+// would otherwise cost a read-modify-write in the memory system.
+// ---------------------------------------------------------------------------
+template <int Q, class S, int PAR, int FAM, int VEC>
+__global__ __launch_bounds__(kMaxThreads) void k_step_row(KArgs a) {
+  using R = typename S::R;
+  using E = typename S::E;
+  constexpr unsigned full = (1u << VEC) - 1u;
+  constexpr int L = 128 / int(sizeof(E));  // elements of one cache line
+  constexpr int kMaxW = kMaxThreads / kWave;
+  __shared__ R xfirst[kMaxW][Q];  // each wavefront's lane-0 cell 0 values
+  __shared__ R xlast[kMaxW][Q];   // each wavefront's lane-63 cell VEC-1 values
+  __shared__ unsigned xmask[kMaxW][2];
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int W = a.wpr;  // wavefronts per row (= per workgroup)
+  const int row = int(xcd_block(blockIdx.x, unsigned(a.nblocks)));
+  const int y = a.ys + row % a.ey;
+  const int z = a.zs + row / a.ey;
+  const int c0 = (wv * kWave + lane) * VEC;  // x - xs of this thread's first cell
+  const int s0 = c0 + a.pitch * y + int(a.plane) * z;
+  E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
+
+  // --- dynamics ---------------------------------------------------------------------
+  uint8_t fl[VEC];
+  load_flags<VEC>(a.flags + s0, fl);
+  int dyn[VEC];
+  float cs[VEC];
+  unsigned live = 0u, beyond = 0u;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) {
+    cs[j] = 0.0f;
+    dyn[j] = DYN_NONE;
+    if (c0 + j < a.ex) dyn[j] = match_task(a, fl[j], cs[j]);
+    else beyond |= 1u << j;
+    if (dyn[j] != DYN_NONE) live |= 1u << j;
+  }
+  // slots this thread may write: live cells, and cells past the row end
+  // (their slots are the ghost cell and padding, never read)
+  const unsigned wmask = live | beyond;
+
+  unsigned ln[VEC], lm[VEC];
+  unsigned anylink = 0u;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) {
+    ln[j] = 0u;
+    lm[j] = 0u;
+    if (PAR == 1 && a.nbc && ((live >> j) & 1u))
+      wall_links<Q, false>(a, a.xs + c0 + j, y, z, s0 + j, fl[j], ln[j], lm[j]);
+    anylink |= ln[j] | lm[j];
+  }
+
+  // --- read ---------------------------------------------------------------------------
+  R f[VEC][Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    R v[VEC];
+    if (PAR == 0) {
+      load_row<Q, S, VEC>((P + int64_t(i) * a.dpop) + c0, i, v);
+    } else {
+      const int o = opp(i);
+      const int cx = Lat<Q>::cx(o);  // source x + c_o = x - c_i
+      const E* base = (P + int64_t(o) * a.dpop) + c0 + yz_off<Q, false, true>(a, o, y, z);
+      R w[VEC];
+      load_row<Q, S, VEC>(base, i, w);
+      if (cx == 0) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) v[j] = w[j];
+      } else if (cx > 0) {
+        R nxt = wave_shl(w[0]);
+        if (lane == kWave - 1) nxt = S::template from<Q>(base[VEC], i);
+#pragma unroll
+        for (int j = 0; j < VEC - 1; ++j) v[j] = w[j + 1];
+        v[VEC - 1] = nxt;
+      } else {
+        R prv = wave_shr(w[VEC - 1]);
+        if (lane == 0) prv = S::template from<Q>(base[-1], i);
+        v[0] = prv;
+#pragma unroll
+        for (int j = 1; j < VEC; ++j) v[j] = w[j - 1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) f[j][i] = v[j];
+  }
+  if (PAR == 1 && anylink) {
+    // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const unsigned lk = ln[j] | lm[j];
+      if (!lk) continue;
+      R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
+      if (lm[j]) load_fields<Q>(a, s0 + j, rho_f, u_f);
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        if ((lk >> opp(i)) & 1u) {
+          R v = S::template from<Q>((P + int64_t(i) * a.dpop)[c0 + j], i);
+          if ((lm[j] >> opp(i)) & 1u) v = v + movingwall_term<Q>(i, rho_f, u_f);
+          f[j][i] = v;
+        }
+      }
+    }
+  }
+
+  // --- collide --------------------------------------------------------------------------
+  const R om = R(a.omega);
+  const R _om = R(1) - om;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j)
+    if ((live >> j) & 1u) collide<Q, R, FAM>(a, s0 + j, dyn[j], cs[j], f[j], om, _om);
+
+  // --- write ------------------------------------------------------------------------------
+  if (PAR == 0) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      R v[VEC];
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[j] = f[j][i];
+      store_row<Q, S, VEC>((P + int64_t(opp(i)) * a.dpop) + c0, i, v, wmask);
+    }
+    return;
+  }
+  // odd step: the values crossing a wavefront boundary go through LDS
+  if (W > 1) {
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) xfirst[wv][i] = f[0][i];
+      xmask[wv][0] = wmask & 1u;
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) xlast[wv][i] = f[VEC - 1][i];
+      xmask[wv][1] = (wmask >> (VEC - 1)) & 1u;
+    }
+    __syncthreads();
+  }
+  const bool row_tail = (wv == W - 1) && (a.ex == W * kWave * VEC);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int cx = Lat<Q>::cx(i);  // destination x + c_i
+    E* base = (P + int64_t(i) * a.dpop) + c0 + yz_off<Q, false, true>(a, i, y, z);
+    R w[VEC];
+    unsigned m;
+    if (cx == 0) {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) w[j] = f[j][i];
+      m = wmask;
+    } else if (cx > 0) {
+      // slot j receives cell j-1; slot 0 the previous lane's last cell
+      R prv = wave_shr(f[VEC - 1][i]);
+      unsigned pm = (wave_shr(wmask) >> (VEC - 1)) & 1u;
+      if (lane == 0) {
+        if (wv > 0) {
+          prv = xlast[wv - 1][i];
+          pm = xmask[wv - 1][1];
+        } else {
+          pm = 1u;  // source is the overlap cell: slot x = xs is never read unfixed
+        }
+      }
+      w[0] = prv;
+#pragma unroll
+      for (int j = 1; j < VEC; ++j) w[j] = f[j - 1][i];
+      m = pm | ((wmask << 1) & full);
+      if (row_tail) {
+        // the row's last cell moves into the next line: write that whole line
+        const R last = __shfl(f[VEC - 1][i], kWave - 1, kWave);
+        if (lane < L / VEC) {
+          R t[VEC];
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) t[j] = (lane == 0 && j == 0) ? last : R(0);
+          store_row<Q, S, VEC>(base - c0 + a.ex + lane * VEC, i, t, full);
+        }
+      }
+    } else {
+      // slot j receives cell j+1; the last slot the next lane's first cell
+      R nxt = wave_shl(f[0][i]);
+      unsigned nm = wave_shl(wmask) & 1u;
+      if (lane == kWave - 1) {
+        if (wv < W - 1) {
+          nxt = xfirst[wv + 1][i];
+          nm = xmask[wv + 1][0];
+        } else {
+          nm = 1u;  // past the row end: ghost / padding slot
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < VEC - 1; ++j) w[j] = f[j + 1][i];
+      w[VEC - 1] = nxt;
+      m = (wmask >> 1) | (nm << (VEC - 1));
+      if (wv == 0) {
+        // the row's first cell moves to the overlap cell x = xs - 1, the last
+        // slot of the line before the row: write that whole line
+        const R first = __shfl(f[0][i], 0, kWave);
+        if (lane < L / VEC) {
+          R t[VEC];
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) t[j] = (lane == L / VEC - 1 && j == VEC - 1) ? first : R(0);
+          store_row<Q, S, VEC>(base - c0 - L + lane * VEC, i, t, full);
+        }
+      }
+    }
+    store_row<Q, S, VEC>(base, i, w, m);
+  }
+  if (anylink) {
+    // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const unsigned lk = ln[j] | lm[j];
+      if (!lk) continue;
+      R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
+      if (lm[j]) load_fields<Q>(a, s0 + j, rho_f, u_f);
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        if ((lk >> i) & 1u) {
+          R v = f[j][i];
+          if ((lm[j] >> i) & 1u) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
+          (P + int64_t(opp(i)) * a.dpop)[c0 + j] = S::template to<Q>(v, i);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell companions (one thread per host cell, dense host index).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool host_cell(const KArgs& a, int64_t id, int& x, int& y, int& z) {
+  x = int(id % a.nx);
+  const int64_t r = id / a.nx;
+  y = int(r % a.ny);
+  z = int(r / a.ny);
+  return true;
+}
+
+// Slot of cell (x, y, z) in the per-cell arrays (flags, fields) ...
+__device__ __forceinline__ int slot_of(const KArgs& a, int x, int y, int z) {
+  return (x - a.xs) + a.pitch * y + int(a.plane) * z;
+}
+// ... and in the population arrays (64-bit: interleaved planes of a large
+// block pass 2^31 slots).
+__device__ __forceinline__ int64_t pslot_of(const KArgs& a, int x, int y, int z) {
+  return int64_t(x - a.xs) + int64_t(a.ypop) * y + a.zpop * z;
+}
+
+__device__ __forceinline__ bool updated(const KArgs& a, int x, int y, int z) {
+  return x >= a.xs && x < a.xs + a.ex && y >= a.ys && y < a.ys + a.ey && z >= a.zs &&
+         z < a.zs + a.ez;
+}
+
+// Logical (post-stream) populations of one updated cell for the given parity
+// (slot: flags / fields slot, pslot: population slot of the cell).
+template <int Q, class S, int PAR>
+__device__ __forceinline__ void read_logical(const KArgs& a, int x, int y, int z, int slot,
+                                             int64_t pslot, typename S::R (&f)[Q]) {
+  using R = typename S::R;
+  if (PAR == 0) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = S::template from<Q>(arr<S>(a, i)[pslot], i);
+    return;
+  }
+  unsigned ln = 0u, lm = 0u;
+  R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
+  if (a.nbc) {
+    wall_links<Q, true>(a, x, y, z, slot, a.flags[slot], ln, lm);
+    if (lm) load_fields<Q>(a, slot, rho_f, u_f);
+  }
+  const unsigned lk = ln | lm;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int o = opp(i);
+    if (i != 0 && ((lk >> o) & 1u)) {
+      R v = S::template from<Q>(arr<S>(a, i)[pslot], i);
+      if ((lm >> o) & 1u) v = v + movingwall_term<Q>(i, rho_f, u_f);
+      f[i] = v;
+    } else {
+      f[i] = S::template from<Q>(arr<S>(a, o)[pslot + nb_off<Q, true, true>(a, o, x, y, z)], i);
+    }
+  }
+}
+
+template <int Q, class S, int PAR>
+__global__ __launch_bounds__(kMaxThreads) void k_gather(KArgs a, typename S::R* out) {
+  using R = typename S::R;
+  const int64_t N = int64_t(a.nx) * a.ny * a.nz;
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= N) return;
+  int x, y, z;
+  host_cell(a, id, x, y, z);
+  R f[Q];
+  if (updated(a, x, y, z)) {
+    read_logical<Q, S, PAR>(a, x, y, z, slot_of(a, x, y, z), pslot_of(a, x, y, z), f);
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = R(0);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) out[i * N + id] = f[i];
+}
+
+template <int Q, class S, int PAR>
+__global__ __launch_bounds__(kMaxThreads) void k_moments(KArgs a, uint8_t mask,
+                                                         typename S::R* out) {
+  using R = typename S::R;
+  const int64_t N = int64_t(a.nx) * a.ny * a.nz;
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= N) return;
+  int x, y, z;
+  host_cell(a, id, x, y, z);
+  R rho = R(0), u[3] = {R(0), R(0), R(0)};
+  if (updated(a, x, y, z)) {
+    const int slot = slot_of(a, x, y, z);
+    if (a.flags[slot] & mask) {
+      R f[Q];
+      read_logical<Q, S, PAR>(a, x, y, z, slot, pslot_of(a, x, y, z), f);
+      rhou(f, rho, u);
+    }
+  }
+  out[id] = rho;
+#pragma unroll
+  for (int d = 0; d < Lat<Q>::D; ++d) out[(d + 1) * N + id] = u[d];
+}
+
+template <int Q, class S, int PAR>
+__global__ __launch_bounds__(kMaxThreads) void k_reduce(KArgs a, uint8_t mask, double* partials) {
+  using R = typename S::R;
+  __shared__ double sh[4][kMaxThreads];
+  const int tid = threadIdx.x;
+  const int64_t N = int64_t(a.nx) * a.ny * a.nz;
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  if (id < N) {
+    int x, y, z;
+    host_cell(a, id, x, y, z);
+    if (updated(a, x, y, z)) {
+      const int slot = slot_of(a, x, y, z);
+      if (a.flags[slot] & mask) {
+        R f[Q];
+        read_logical<Q, S, PAR>(a, x, y, z, slot, pslot_of(a, x, y, z), f);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          const double v = double(f[i]);
+          m[0] += v;
+          m[1] += double(Lat<Q>::cx(i)) * v;
+          m[2] += double(Lat<Q>::cy(i)) * v;
+          m[3] += double(Lat<Q>::cz(i)) * v;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[c][tid] = m[c];
+  __syncthreads();
+  for (int s = kMaxThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh[c][tid] += sh[c][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) partials[int64_t(blockIdx.x) * 4 + c] = sh[c][0];
+  }
+}
+
+// Sums the per-workgroup partials in a fixed order (deterministic result).
+// (A template only so each storage kind's translation unit owns its copy.)
+template <class S>
+__global__ __launch_bounds__(kMaxThreads) void k_reduce_final(const double* partials, int n,
+                                                             double* out) {
+  __shared__ double sh[4][kMaxThreads];
+  const int tid = threadIdx.x;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = tid; b < n; b += kMaxThreads) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m[c] += partials[int64_t(b) * 4 + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[c][tid] = m[c];
+  __syncthreads();
+  for (int s = kMaxThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh[c][tid] += sh[c][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] = sh[c][0];
+  }
+}
+
+// Natural-layout store of every host cell (overlap layer included).
+template <int Q, class S>
+__global__ __launch_bounds__(kMaxThreads) void k_scatter(KArgs a, const typename S::R* in) {
+  const int64_t N = int64_t(a.nx) * a.ny * a.nz;
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= N) return;
+  int x, y, z;
+  host_cell(a, id, x, y, z);
+  const int64_t slot = pslot_of(a, x, y, z);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) arr<S>(a, i)[slot] = S::template to<Q>(in[i * N + id], i);
+}
+
+template <int Q, class S>
+__global__ __launch_bounds__(kMaxThreads) void k_init_feq(KArgs a, const typename S::R* rhou_in) {
+  using R = typename S::R;
+  const int64_t N = int64_t(a.nx) * a.ny * a.nz;
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= N) return;
+  int x, y, z;
+  host_cell(a, id, x, y, z);
+  const int64_t slot = pslot_of(a, x, y, z);
+  R u[3] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int d = 0; d < Lat<Q>::D; ++d) u[d] = rhou_in[(d + 1) * N + id];
+  R e[Q];
+  feq(e, rhou_in[id], u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) arr<S>(a, i)[slot] = S::template to<Q>(e[i], i);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers for one storage kind.
+// ---------------------------------------------------------------------------
+template <int Q, class S, int PAR>
+hipError_t step_q(int family, const KArgs& a, hipStream_t s) {
+  const dim3 grid(a.nblocks), block(kWave * a.wpb);
+  const bool per = a.px || a.py || a.pz;
+  if (per || a.vec == 0) {
+    if (family == FAM_FULL) {
+      if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, false>), grid, block, 0, s, a);
+    } else {
+      if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, false>), grid, block, 0, s, a);
+    }
+  } else if (family == FAM_FULL) {
+    if (a.vec == 1) hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 1>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 2>), grid, block, 0, s, a);
+  } else if (a.vec == 1) {
+    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 1>), grid, block, 0, s, a);
+  } else if (a.vec == 2) {
+    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 2>), grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 4>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_step(int q, int parity, int family, const KArgs& a, hipStream_t s) {
+  if (q == 19) return parity ? step_q<19, S, 1>(family, a, s) : step_q<19, S, 0>(family, a, s);
+  if (q == 9) return parity ? step_q<9, S, 1>(family, a, s) : step_q<9, S, 0>(family, a, s);
+  return hipErrorInvalidValue;
+}
+
+template <class S>
+hipError_t launch_gather(int q, int parity, const KArgs& a, void* out, hipStream_t s) {
+  const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
+  using R = typename S::R;
+  R* o = static_cast<R*>(out);
+  if (q == 19) {
+    if (parity) hipLaunchKernelGGL((k_gather<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+    else hipLaunchKernelGGL((k_gather<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+  } else {
+    if (parity) hipLaunchKernelGGL((k_gather<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+    else hipLaunchKernelGGL((k_gather<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+  }
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_scatter(int q, const KArgs& a, const void* in, hipStream_t s) {
+  const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
+  using R = typename S::R;
+  const R* p = static_cast<const R*>(in);
+  if (q == 19) hipLaunchKernelGGL((k_scatter<19, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  else hipLaunchKernelGGL((k_scatter<9, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_init_feq(int q, const KArgs& a, const void* rhou, hipStream_t s) {
+  const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
+  using R = typename S::R;
+  const R* p = static_cast<const R*>(rhou);
+  if (q == 19) hipLaunchKernelGGL((k_init_feq<19, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  else hipLaunchKernelGGL((k_init_feq<9, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_moments(int q, int parity, const KArgs& a, uint8_t mask, void* out,
+                          hipStream_t s) {
+  const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
+  using R = typename S::R;
+  R* o = static_cast<R*>(out);
+  if (q == 19) {
+    if (parity) hipLaunchKernelGGL((k_moments<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+    else hipLaunchKernelGGL((k_moments<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+  } else {
+    if (parity) hipLaunchKernelGGL((k_moments<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+    else hipLaunchKernelGGL((k_moments<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+  }
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_reduce(int q, int parity, const KArgs& a, uint8_t mask, double* partials,
+                         double* out, hipStream_t s) {
+  const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
+  if (q == 19) {
+    if (parity) hipLaunchKernelGGL((k_reduce<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+    else hipLaunchKernelGGL((k_reduce<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+  } else {
+    if (parity) hipLaunchKernelGGL((k_reduce<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+    else hipLaunchKernelGGL((k_reduce<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_final<S>, dim3(1), dim3(kMaxThreads), 0, s, partials, int(nb), out);
+  return hipGetLastError();
+}
+
+}  // namespace kern
+}  // namespace flb
+
+// Defines the launchers of flb_internal.hpp for one storage kind.
+#define FLB_DEFINE_LAUNCHERS(SUFFIX, STORE)                                                     \
+  namespace flb {                                                                              \
+  hipError_t launch_step_##SUFFIX(int q, int parity, int family, const KArgs& a,              \
+                                  hipStream_t s) {                                            \
+    return kern::launch_step<STORE>(q, parity, family, a, s);                                 \
+  }                                                                                            \
+  hipError_t launch_gather_##SUFFIX(int q, int parity, const KArgs& a, void* out,             \
+                                    hipStream_t s) {                                          \
+    return kern::launch_gather<STORE>(q, parity, a, out, s);                                  \
+  }                                                                                            \
+  hipError_t launch_scatter_##SUFFIX(int q, const KArgs& a, const void* in, hipStream_t s) {  \
+    return kern::launch_scatter<STORE>(q, a, in, s);                                          \
+  }                                                                                            \
+  hipError_t launch_init_feq_##SUFFIX(int q, const KArgs& a, const void* rhou, hipStream_t s) { \
+    return kern::launch_init_feq<STORE>(q, a, rhou, s);                                       \
+  }                                                                                            \
+  hipError_t launch_moments_##SUFFIX(int q, int parity, const KArgs& a, uint8_t mask,         \
+                                     void* out, hipStream_t s) {                              \
+    return kern::launch_moments<STORE>(q, parity, a, mask, out, s);                           \
+  }                                                                                            \
+  hipError_t launch_reduce_##SUFFIX(int q, int parity, const KArgs& a, uint8_t mask,          \
+                                    double* partials, double* out, hipStream_t s) {           \
+    return kern::launch_reduce<STORE>(q, parity, a, mask, partials, out, s);                  \
+  }                                                                                            \
+  }

@@ -1,0 +1,203 @@
+"""Host-side mirror of FreeLB's block-lattice interface over the C ABI.
+
+``BlockLattice`` is one FreeLB ``BlockLattice`` resident on the GPU. Its
+methods keep the reference's call shape and names:
+
+    lat.ApplyCellDynamics()   # BlockLattice::ApplyCellDynamics<TaskSelector>(flags)
+    lat.Stream()              # BlockLattice::Stream
+    lat.ApplyBoundaries()     # BlockBoundaryManager::Apply (fused; order check only)
+
+(data_struct/block_lattice.hh:64-92, boundary/basic_boundary.h:345-347). The
+dynamics of a cell is chosen by a first-match task table, as FreeLB's
+``TaskSelector<uint8_t, CELL, Key_TypePair<mask, Dynamics>...>`` does
+(utils/tmp.h:283-309); halfway bounce-back managers are registered with
+``set_halfway_bc`` (boundary/boundary.h:75-129).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass(frozen=True)
+class Key_TypePair:
+    """FreeLB's tmp::Key_TypePair<mask, Dynamics> (utils/tmp.h:194-202)."""
+    mask: int
+    dynamics: int
+    smagorinsky: float = 0.0
+
+
+@dataclass(frozen=True)
+class HalfwayBB:
+    """One BBLikeFixedBlockBdManager<bounceback::normal|movingwall> (boundary/boundary.h:75-129)."""
+    cell_mask: int
+    void_mask: int = 1
+    moving: bool = False
+
+
+def weights(q: int, dtype=np.float64) -> np.ndarray:
+    """Lattice weights as FreeLB forms them, T(1)/T(18) etc. (lbm/lattice_set.h)."""
+    one = dtype(1)
+    if q == 19:
+        return np.array([one / dtype(3)] + [one / dtype(18)] * 6 + [one / dtype(36)] * 12, dtype=dtype)
+    return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
+
+
+class BlockLattice:
+    def __init__(self, q: int, dims: Sequence[int], omega: float, storage: str = "f32",
+                 overlap: int = 1, periodic: Sequence[bool] = (False, False, False), stream=None):
+        L = abi.lib()
+        self.q = int(q)
+        self.d = 3 if self.q == 19 else 2
+        nx, ny = int(dims[0]), int(dims[1])
+        nz = int(dims[2]) if len(dims) > 2 else 1
+        self.dims = (nx, ny, nz)
+        self.N = nx * ny * nz
+        self.storage = storage
+        desc = abi.BlockDesc()
+        desc.q = self.q
+        desc.storage = abi.STORAGE[storage]
+        desc.nx, desc.ny, desc.nz = nx, ny, nz
+        desc.overlap = int(overlap)
+        for a in range(3):
+            desc.periodic[a] = 1 if (a < len(periodic) and periodic[a]) else 0
+        desc.omega = float(omega)
+        desc.stream = stream
+        h = C.c_void_p()
+        abi.check(L.flb_block_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self._real = np.float64 if storage == "f64" else np.float32
+
+    # ---- lifecycle -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.check(abi.lib().flb_block_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> abi.BlockInfo:
+        inf = abi.BlockInfo()
+        abi.check(abi.lib().flb_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def synchronize(self) -> None:
+        abi.check(abi.lib().flb_synchronize(self._h))
+
+    # ---- set-up ------------------------------------------------------------
+    def set_tasks(self, tasks: Iterable) -> None:
+        tasks = [t if isinstance(t, Key_TypePair) else Key_TypePair(*t) for t in tasks]
+        arr = (abi.Task * max(1, len(tasks)))()
+        for k, t in enumerate(tasks):
+            arr[k].mask, arr[k].dynamics, arr[k].smagorinsky = t.mask, t.dynamics, t.smagorinsky
+        abi.check(abi.lib().flb_set_tasks(self._h, arr, len(tasks)))
+
+    def set_halfway_bc(self, bcs: Iterable) -> None:
+        bcs = [b if isinstance(b, HalfwayBB) else HalfwayBB(*b) for b in bcs]
+        arr = (abi.HWBC * max(1, len(bcs)))()
+        for k, b in enumerate(bcs):
+            arr[k].cell_mask, arr[k].void_mask = b.cell_mask, b.void_mask
+            arr[k].kind = abi.HWBC_MOVING if b.moving else abi.HWBC_NORMAL
+        abi.check(abi.lib().flb_set_halfway_bc(self._h, arr, len(bcs)))
+
+    def set_mrt_rates(self, rates=None) -> None:
+        if rates is None:
+            abi.check(abi.lib().flb_set_mrt_rates(self._h, None, self.q))
+            return
+        r = np.ascontiguousarray(rates, dtype=np.float64)
+        abi.check(abi.lib().flb_set_mrt_rates(self._h, abi.ptr(r, C.c_double), r.size))
+
+    def set_omega(self, omega: float) -> None:
+        abi.check(abi.lib().flb_set_omega(self._h, float(omega)))
+
+    def upload_flags(self, flags: np.ndarray) -> None:
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        if f.size != self.N:
+            raise ValueError(f"flags has {f.size} cells, block has {self.N}")
+        abi.check(abi.lib().flb_upload_flags(self._h, abi.ptr(f, C.c_uint8)))
+
+    def set_field_rho(self, rho: np.ndarray) -> None:
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        if r.size != self.N:
+            raise ValueError("rho field size mismatch")
+        abi.check(abi.lib().flb_set_field_rho(self._h, abi.ptr(r, C.c_double)))
+
+    def set_field_u(self, u: np.ndarray) -> None:
+        v = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if v.size != self.d * self.N:
+            raise ValueError("velocity field size mismatch (expects d*N SoA)")
+        abi.check(abi.lib().flb_set_field_u(self._h, abi.ptr(v, C.c_double)))
+
+    # ---- populations -------------------------------------------------------
+    def upload_pops(self, pops: np.ndarray) -> None:
+        p = np.ascontiguousarray(pops, dtype=self._real).reshape(-1)
+        if p.size != self.q * self.N:
+            raise ValueError("population array size mismatch (expects q*N SoA)")
+        if self._real is np.float64:
+            abi.check(abi.lib().flb_upload_pops_f64(self._h, abi.ptr(p, C.c_double)))
+        else:
+            abi.check(abi.lib().flb_upload_pops_f32(self._h, abi.ptr(p, C.c_float)))
+
+    def init_uniform(self, rho0: float = 1.0) -> None:
+        """f_i = rho0 * w_i everywhere (BlockLatticeManager ctor, block_lattice.hh:676-681)."""
+        w = weights(self.q, self._real)
+        pops = np.repeat((self._real(rho0) * w)[:, None], self.N, axis=1)
+        self.upload_pops(pops)
+
+    def init_equilibrium(self, rho: np.ndarray, u: np.ndarray) -> None:
+        rhou = np.concatenate([np.asarray(rho, dtype=np.float64).reshape(1, -1),
+                               np.asarray(u, dtype=np.float64).reshape(self.d, -1)], axis=0)
+        rhou = np.ascontiguousarray(rhou)
+        abi.check(abi.lib().flb_init_equilibrium(self._h, abi.ptr(rhou, C.c_double)))
+
+    def pops(self) -> np.ndarray:
+        out = np.empty((self.q, self.N), dtype=self._real)
+        if self._real is np.float64:
+            abi.check(abi.lib().flb_download_pops_f64(self._h, abi.ptr(out, C.c_double)))
+        else:
+            abi.check(abi.lib().flb_download_pops_f32(self._h, abi.ptr(out, C.c_float)))
+        return out
+
+    # ---- the hot path -------------------------------------------------------
+    def ApplyCellDynamics(self) -> None:
+        abi.check(abi.lib().flb_apply_cell_dynamics(self._h))
+
+    def Stream(self) -> None:
+        abi.check(abi.lib().flb_stream(self._h))
+
+    def ApplyBoundaries(self) -> None:
+        abi.check(abi.lib().flb_apply_halfway_bc(self._h))
+
+    def step(self, n: int = 1) -> None:
+        abi.check(abi.lib().flb_step(self._h, int(n)))
+
+    # ---- read-out -------------------------------------------------------------
+    def moments(self, mask: int = 0xFE) -> np.ndarray:
+        """rho, u of cells with flag & mask (moment::rhoU<CELL, true>); [d+1][N]."""
+        out = np.empty((self.d + 1, self.N), dtype=self._real)
+        abi.check(abi.lib().flb_moments(self._h, mask & 0xFF, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def mass_momentum(self, mask: int = 0xFE) -> np.ndarray:
+        out = np.zeros(4, dtype=np.float64)
+        abi.check(abi.lib().flb_reduce_mass_momentum(self._h, mask & 0xFF, abi.ptr(out, C.c_double)))
+        return out

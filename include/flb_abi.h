@@ -1,0 +1,196 @@
+/*
+ * flb_abi.h -- C ABI of the MI355X (gfx950) lattice-update engine.
+ *
+ * One flb_block is one FreeLB BlockLattice living on one GPU: SoA population
+ * storage, a uint8 flag field, a first-match task table (FreeLB's
+ * TaskSelector<flag, Key_TypePair<mask, Dynamics>...>) and a list of halfway
+ * bounce-back managers (BBLikeFixedBlockBdManager). The update is the fused
+ * collide-and-stream step of the AA (esoteric) in-place pattern, with the
+ * halfway boundary fix-up folded into the same kernel.
+ *
+ * Conventions
+ *   - Every function returns FLB_OK (0) or an FLB_ERR_* code; the message of
+ *     the last failure on the calling thread is flb_last_error().
+ *   - Host arrays are plain C arrays: populations are SoA [q][N], fields are
+ *     SoA [d][N], with N = nx*ny*nz and cell index id = x + nx*(y + ny*z)
+ *     (FreeLB's BasicBlock::getIndex order).
+ *   - Work is enqueued on the block's stream; functions that return data to
+ *     the host synchronise that stream. Calls on one block are not
+ *     re-entrant; different blocks may be driven from different threads.
+ *
+ * Reference interfaces replaced (paths relative to the FreeLB source tree):
+ *   flb_apply_cell_dynamics  BlockLattice::ApplyCellDynamics<TaskSelector>(flagarr)
+ *                              data_struct/block_lattice.hh:72-92 (CPU),
+ *                              BlockLatticeManager::CuDevApplyCellDynamics
+ *                              data_struct/block_lattice.hh:919-936 (CUDA)
+ *   flb_stream               BlockLattice::Stream / CuDevStream
+ *                              data_struct/block_lattice.hh:64-70, 915-917
+ *   flb_apply_halfway_bc     BlockBoundaryManager::Apply -> BBLikeFixedBlockBdManager::Apply
+ *                              boundary/basic_boundary.h:345-347, boundary/boundary.h:121-128
+ *   flb_set_tasks            TaskSelector / Key_TypePair  utils/tmp.h:194-202, 283-309
+ *   flb_set_halfway_bc       BBLikeFixedBlockBdManager ctor + BlockFixedBoundary::Setup
+ *                              boundary/boundary.h:94-111, boundary/basic_boundary.h:198-264
+ *   flb_block_create         BlockLattice ctor (Omega, 1-Omega, 1-Omega/2)
+ *                              data_struct/block_lattice.hh:24-35; POP field storage
+ *                              data_struct/field.h:1419-1575, utils/alias.h:296-308
+ *   flb_upload_flags         BlockFieldManager<FLAG>::copyToDevice  data_struct/field.h:883-896
+ *   flb_set_field_rho/u      RHO / VELOCITY fields read by moment::useFieldrhoU,
+ *                              BounceBackMovingWall and bounceback::movingwall
+ *                              lbm/moment.h:71-90, lbm/collision.ur.h:130-187,
+ *                              boundary/bounce_back_boundary.h:240-256
+ *   flb_moments              moment::rhoU<CELL, true> via TaskSelectorRhoU
+ *                              lbm/moment.h:381-398, examples/cavity3d/cavity3d.cpp:192-194
+ *   flb_init_equilibrium     BlockLatticeManager ctor pop init / equilibrium::SecondOrder
+ *                              data_struct/block_lattice.hh:676-681, lbm/equilibrium.h:156-178
+ */
+#ifndef FLB_ABI_H
+#define FLB_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLB_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  FLB_OK = 0,
+  FLB_ERR_ARG = 1,         /* bad argument (null pointer, bad size, unknown enum) */
+  FLB_ERR_STATE = 2,       /* call out of order (e.g. ApplyCellDynamics twice without Stream) */
+  FLB_ERR_HIP = 3,         /* a HIP runtime call failed */
+  FLB_ERR_UNSUPPORTED = 4, /* combination not built (e.g. MRT on a storage kind without it) */
+  FLB_ERR_LAYOUT = 5,      /* a cell on the outer layer of a non-periodic axis matches a task */
+  FLB_ERR_NOMEM = 6        /* device allocation failed */
+};
+
+/* ---- enums ------------------------------------------------------------- */
+enum { FLB_D2Q9 = 9, FLB_D3Q19 = 19 };
+
+/* Population storage. The shifted forms store f_i - w_i (rho0 = 1) and do
+ * all arithmetic in FP32; FLB_STORE_F64 stores and computes in FP64. */
+enum { FLB_STORE_F32 = 0, FLB_STORE_F16S = 1, FLB_STORE_BF16S = 2, FLB_STORE_F64 = 3 };
+
+/* Cell dynamics (the `Dynamics` of a Key_TypePair). */
+enum {
+  FLB_DYN_NONE = 0,          /* no-op: the cell is not updated (void / solid) */
+  FLB_DYN_BGK = 1,           /* collision::BGK<moment::rhoU, equilibrium::SecondOrder>      lbm/collision.h:32-59 */
+  FLB_DYN_BGK_FIELD = 2,     /* collision::BGK<moment::useFieldrhoU, SecondOrder>           lbm/moment.h:71-90 */
+  FLB_DYN_BOUNCEBACK = 3,    /* collision::BounceBack (fullway wall as collision)           lbm/collision.ur.h:62-93 */
+  FLB_DYN_BOUNCEBACK_MW = 4, /* collision::BounceBackMovingWall                             lbm/collision.ur.h:130-187 */
+  FLB_DYN_MRT = 5,           /* collision::MRT_Feq_RhoU (D2Q9 as lbm/collisionMRT.ur.h:45-129;
+                                D3Q19 on the d'Humieres 2002 basis, no reference form) */
+  FLB_DYN_SMAG_BGK = 6,      /* collision::SmagorinskyBGK                                   lbm/collisionLES.h:29-67 */
+  FLB_DYN_SMAG_MRT = 7       /* MRT whose shear rates take the Smagorinsky omega (no reference form) */
+};
+
+/* Halfway bounce-back kinds: bounceback::normal / bounceback::movingwall
+ * (boundary/bounce_back_boundary.h:169-181, 240-256). */
+enum { FLB_HWBC_NORMAL = 0, FLB_HWBC_MOVING = 1 };
+
+/* ---- descriptors ------------------------------------------------------- */
+typedef struct flb_block_s* flb_block;
+
+typedef struct {
+  int q;            /* FLB_D2Q9 or FLB_D3Q19 */
+  int storage;      /* FLB_STORE_* */
+  int nx, ny, nz;   /* extents including the overlap layer; nz = 1 for D2Q9 */
+  int overlap;      /* overlap (ghost) width on non-periodic axes, >= 1 */
+  int periodic[3];  /* non-zero: the axis wraps (no overlap layer on it) */
+  double omega;     /* 1/tau of the block (BlockLattice::Omega) */
+  void* stream;     /* hipStream_t to enqueue on; NULL = the null stream */
+} flb_block_desc;
+
+/* One Key_TypePair<mask, Dynamics>: the first entry whose (flag & mask) != 0 runs. */
+typedef struct {
+  uint8_t mask;
+  uint8_t dynamics;  /* FLB_DYN_* */
+  uint8_t pad[2];
+  float smagorinsky; /* C_s for the Smagorinsky kinds (the SMAGORINSKY data field) */
+} flb_task;
+
+/* One halfway manager: boundary cells have (flag & cell_mask) != 0; a link to
+ * a neighbour with (flag & void_mask) != 0 is a wall link. Entries apply in
+ * order, so where two match the same cell the later one's value stands. */
+typedef struct {
+  uint8_t cell_mask;
+  uint8_t void_mask;
+  uint8_t kind;      /* FLB_HWBC_* */
+  uint8_t pad;
+} flb_hwbc;
+
+typedef struct {
+  int q, storage, nx, ny, nz, overlap;
+  int periodic[3];
+  int64_t n_cells;        /* nx*ny*nz */
+  int64_t n_interior;     /* cells the kernel visits (overlap layer excluded) */
+  int64_t stride;         /* elements between two population arrays */
+  int64_t step;           /* completed time steps */
+  int parity;             /* 0: natural layout, 1: AA-swapped layout */
+  void* d_pops;           /* device base of the population arrays */
+  const void* d_flags;    /* device flag field */
+} flb_block_info;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+const char* flb_last_error(void);
+int flb_abi_version(void);
+int flb_block_create(const flb_block_desc* desc, flb_block* out);
+int flb_block_destroy(flb_block b);
+int flb_get_info(flb_block b, flb_block_info* info);
+int flb_synchronize(flb_block b);
+
+/* ---- set-up ------------------------------------------------------------- */
+int flb_set_tasks(flb_block b, const flb_task* tasks, int n);      /* n <= 8 */
+int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n);   /* n <= 4 */
+/* MRT relaxation rates, q values; NaN marks the shear-viscosity rates that
+ * take omega. NULL restores the defaults. */
+int flb_set_mrt_rates(flb_block b, const double* rates, int q);
+int flb_set_omega(flb_block b, double omega);
+int flb_upload_flags(flb_block b, const uint8_t* flags);           /* N bytes */
+int flb_set_field_rho(flb_block b, const double* rho);             /* N values  */
+int flb_set_field_u(flb_block b, const double* u);                 /* d*N values, SoA */
+
+/* ---- populations --------------------------------------------------------- */
+/* Natural-layout (post-stream) populations [q][N]. _f32 serves the F32, F16S
+ * and BF16S storage kinds, _f64 serves F64. Upload resets the layout to
+ * natural; download resolves the AA layout of the current parity. Values of
+ * cells outside the visited interior download as 0. */
+int flb_upload_pops_f32(flb_block b, const float* pops);
+int flb_upload_pops_f64(flb_block b, const double* pops);
+int flb_download_pops_f32(flb_block b, float* pops);
+int flb_download_pops_f64(flb_block b, double* pops);
+/* f_i = feq_i(rho, u) (equilibrium::SecondOrder) from host rhou[d+1][N]. */
+int flb_init_equilibrium(flb_block b, const double* rhou);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* Collide + stream + halfway fix-up of one time step, fused (AA pattern). */
+int flb_apply_cell_dynamics(flb_block b);
+/* Completes the step begun by flb_apply_cell_dynamics (AA parity flip, O(1)). */
+int flb_stream(flb_block b);
+/* Halfway managers are fused into flb_apply_cell_dynamics; this checks call
+ * order only and keeps the reference's three-call step shape. */
+int flb_apply_halfway_bc(flb_block b);
+/* n complete steps (apply + stream) back to back. */
+int flb_step(flb_block b, int n);
+
+/* ---- read-out ------------------------------------------------------------ */
+/* rho and u of cells with (flag & mask) != 0, 0 elsewhere, into host
+ * rhou[d+1][N] (float for F32/F16S/BF16S storage, double for F64). */
+int flb_moments(flb_block b, uint8_t mask, void* rhou);
+/* FP64 totals over cells with (flag & mask) != 0: out = {mass, Px, Py, Pz}. */
+int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+/* Device copy bandwidth: a float4 copy kernel over `bytes` (rounded down to a
+ * multiple of 16) between two fresh buffers, `reps` timed launches; *gbs gets
+ * (read + written bytes) / mean launch time. No reference counterpart: it
+ * calibrates the roofline figures the benchmark reports. */
+int flb_bandwidth_probe(size_t bytes, int reps, double* gbs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLB_ABI_H */

@@ -79,6 +79,13 @@ class Lattice:
         self.N = self.nx * self.ny * self.nz
         self.flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
         assert self.flags.size == self.N
+        # python-side copies, so a test can hand the same case to the GPU path
+        self.tasks = [tuple(t) for t in tasks]
+        self.bcs = [tuple(b) for b in bcs]
+        self.omega = float(omega)
+        self.rho_field = None if rho_field is None else np.asarray(rho_field, dtype=np.float64)
+        self.u_field = None if u_field is None else np.asarray(u_field, dtype=np.float64)
+        self.mrt_rates = None if mrt_rates is None else np.asarray(mrt_rates, dtype=np.float64)
         self._keep = [self.flags]
         L = _Lattice()
         L.q, L.nx, L.ny, L.nz, L.overlap = q, self.nx, self.ny, self.nz, overlap
@@ -152,12 +159,14 @@ def weights(q, dtype=np.float32):
     return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
 
 
-def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float32):
+def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float32, cs=0.17):
     """The reference cavity driver as an oracle Lattice plus its initial pops.
 
     mode 'fw'      : benchmarks/cavity3d/cavity3d.cpp:170-182 (wall cells swap).
     mode 'hw'      : every non-void cell BGK<rhoU>; halfway normal / moving wall.
     mode 'hwfield' : examples/couette2d/couette2d.cpp:183-216 (walls BGK<useFieldrhoU>).
+    mode 'mrt'     : fullway walls, fluid MRT_Feq_RhoU (D2Q9 in FreeLB).
+    mode 'smag'    : fullway walls, fluid SmagorinskyBGK with C_s = cs.
     """
     q = 19 if Nk > 0 else 9
     flags = cavity_flags(Ni, Nj, Nk)
@@ -177,6 +186,13 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
     elif mode == "hwfield":
         tasks = [(AABB, DYN_BGK), (BB | BBMW, DYN_BGK_FIELD)]
         bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+    elif mode == "mrt":
+        tasks = [(AABB, DYN_MRT), (BB, DYN_BOUNCEBACK), (BBMW, DYN_BOUNCEBACK_MW)]
+        bcs = []
+    elif mode == "smag":
+        tasks = [(AABB, DYN_SMAG_BGK, float(np.float32(cs))), (BB, DYN_BOUNCEBACK),
+                 (BBMW, DYN_BOUNCEBACK_MW)]
+        bcs = []
     else:
         raise ValueError(mode)
     if dtype == np.float64:

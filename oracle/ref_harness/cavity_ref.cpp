@@ -8,7 +8,7 @@
 // the HIP kernels can be compared against it.
 //
 // Usage:
-//   cavity_ref <d3q19|d2q9> <fw|hw|hwfield> Ni Nj Nk steps RT KineVisc Uwall out.bin
+//   cavity_ref <d3q19|d2q9> <fw|hw|hwfield|mrt|smag> Ni Nj Nk steps RT KineVisc Uwall out.bin [Cs]
 // (Nk is ignored for d2q9.)
 //
 // Modes
@@ -20,6 +20,10 @@
 //             (boundary.h:75-129).
 //   hwfield : like hw but wall cells collide with BGK<useFieldrhoU>
 //             (examples/couette2d/couette2d.cpp:183-184).
+//   mrt     : fw walls, fluid cells collide with MRT_Feq_RhoU (D2Q9 only:
+//             lbm/collisionMRT.ur.h:45-129).
+//   smag    : fw walls, fluid cells collide with SmagorinskyBGK<rhoU>
+//             (lbm/collisionLES.h:29-67), SMAGORINSKY data field = Cs.
 //
 // Output layout (little endian):
 //   char[8]  "FLBREF01"
@@ -34,6 +38,7 @@
 #include "freelb.hh"
 
 #include <cstdio>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -53,7 +58,7 @@ constexpr std::uint8_t BBMovingWallFlag = std::uint8_t(8);
 struct Params {
   std::string lattice, mode;
   int Ni, Nj, Nk, steps;
-  T RT, visc, uwall;
+  T RT, visc, uwall, cs = T(0);
   std::string out;
 };
 
@@ -109,12 +114,14 @@ void dump(const Params& p, BLM& lat, FM& flagfm, T omega, T ulat, int nz) {
   std::fclose(f);
 }
 
-template <typename LatSet, int D>
+template <typename LatSet, int D, bool SMAG>
 int run(const Params& p) {
   BaseConverter<T> BaseConv(LatSet::cs2);
   BaseConv.ConvertFromRT(T(1), p.RT, T(0.001), T(p.Ni), p.uwall, p.visc);
 
-  using FIELDS = TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>>;
+  using FIELDS = std::conditional_t<SMAG,
+      TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>, SMAGORINSKY<T>>,
+      TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>>>;
   using CELL = Cell<T, LatSet, FIELDS>;
 
   Vector<T, D> U_Wall{};
@@ -129,7 +136,14 @@ int run(const Params& p) {
       if (util::isFlag(field.get(id), BouncebackFlag)) field.SetField(id, BBMovingWallFlag);
     });
 
-    ValuePack InitValues(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{});
+    auto make_init = [&]() {
+      if constexpr (SMAG) {
+        return ValuePack(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{}, p.cs);
+      } else {
+        return ValuePack(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{});
+      }
+    };
+    auto InitValues = make_init();
     BlockLatticeManager<T, LatSet, FIELDS> NSLattice(Geo, InitValues, BaseConv);
     NSLattice.template getField<VELOCITY<T, LatSet::d>>().forEach(
       toplid, FlagFM, BBMovingWallFlag,
@@ -146,7 +160,29 @@ int run(const Params& p) {
     using BGKrhoU = collision::BGK<moment::rhoU<CELL>, equilibrium::SecondOrder<CELL>>;
     using BGKfield = collision::BGK<moment::useFieldrhoU<CELL>, equilibrium::SecondOrder<CELL>>;
 
-    if (p.mode == "fw") {
+    if (p.mode == "mrt" || p.mode == "smag") {
+      using BBTask = tmp::Key_TypePair<BouncebackFlag, collision::BounceBack<CELL>>;
+      using BBMVTask = tmp::Key_TypePair<BBMovingWallFlag, collision::BounceBackMovingWall<CELL>>;
+      if constexpr (SMAG) {
+        using Bulk = tmp::Key_TypePair<AABBFlag,
+            collision::SmagorinskyBGK<moment::rhoU<CELL>, equilibrium::SecondOrder<CELL>>>;
+        using NSTask = TaskSelector<std::uint8_t, CELL, Bulk, BBTask, BBMVTask>;
+        for (int s = 0; s < p.steps; ++s) {
+          NSLattice.template ApplyCellDynamics<NSTask>(FlagFM);
+          NSLattice.Stream();
+        }
+      } else if constexpr (D == 2) {
+        using Bulk = tmp::Key_TypePair<AABBFlag, collision::MRT_Feq_RhoU<CELL>>;
+        using NSTask = TaskSelector<std::uint8_t, CELL, Bulk, BBTask, BBMVTask>;
+        for (int s = 0; s < p.steps; ++s) {
+          NSLattice.template ApplyCellDynamics<NSTask>(FlagFM);
+          NSLattice.Stream();
+        }
+      } else {
+        std::fprintf(stderr, "FreeLB has no D3Q19 MRT\n");
+        std::exit(2);
+      }
+    } else if (p.mode == "fw") {
       using BulkTask = tmp::Key_TypePair<AABBFlag, BGKrhoU>;
       using BBTask = tmp::Key_TypePair<BouncebackFlag, collision::BounceBack<CELL>>;
       using BBMVTask = tmp::Key_TypePair<BBMovingWallFlag, collision::BounceBackMovingWall<CELL>>;
@@ -199,9 +235,9 @@ int run(const Params& p) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 11) {
+  if (argc != 11 && argc != 12) {
     std::fprintf(stderr,
-                 "usage: %s <d3q19|d2q9> <fw|hw|hwfield> Ni Nj Nk steps RT KineVisc Uwall out\n",
+                 "usage: %s <d3q19|d2q9> <fw|hw|hwfield|mrt|smag> Ni Nj Nk steps RT KineVisc Uwall out [Cs]\n",
                  argv[0]);
     return 2;
   }
@@ -218,11 +254,13 @@ int main(int argc, char** argv) {
   p.visc = std::strtof(argv[8], nullptr);
   p.uwall = std::strtof(argv[9], nullptr);
   p.out = argv[10];
+  if (argc == 12) p.cs = std::strtof(argv[11], nullptr);
 #ifdef _OPENMP
   Thread_Num = omp_get_max_threads();
 #endif
-  if (p.lattice == "d3q19") return run<D3Q19<T>, 3>(p);
-  if (p.lattice == "d2q9") return run<D2Q9<T>, 2>(p);
+  const bool smag = p.mode == "smag";
+  if (p.lattice == "d3q19") return smag ? run<D3Q19<T>, 3, true>(p) : run<D3Q19<T>, 3, false>(p);
+  if (p.lattice == "d2q9") return smag ? run<D2Q9<T>, 2, true>(p) : run<D2Q9<T>, 2, false>(p);
   std::fprintf(stderr, "unknown lattice %s\n", p.lattice.c_str());
   return 2;
 }

@@ -1,0 +1,99 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the header
+declares, and its argument checks fail cleanly before any device work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from freelb_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "flb_abi.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(flb_\w+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    lib = C.CDLL(abi.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared in flb_abi.h but not exported: {missing}"
+    assert set(names) == set(abi.EXPORTS), "abi.EXPORTS out of step with the header"
+
+
+def test_abi_version():
+    assert abi.lib().flb_abi_version() == 1
+
+
+def _desc(**kw):
+    d = abi.BlockDesc()
+    d.q, d.storage, d.nx, d.ny, d.nz, d.overlap, d.omega = 19, 0, 10, 10, 10, 1, 1.5
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("q", 15, "q must be"),
+    ("storage", 9, "storage kind"),
+    ("nx", 0, "extents"),
+    ("omega", 2.5, "omega"),
+    ("overlap", 0, "overlap"),
+])
+def test_create_rejects_bad_descriptor(field, value, msg):
+    L = abi.lib()
+    h = C.c_void_p()
+    rc = L.flb_block_create(C.byref(_desc(**{field: value})), C.byref(h))
+    assert rc == abi.FLB_ERR_ARG
+    assert msg in L.flb_last_error().decode()
+    assert not h.value
+
+
+def test_create_rejects_d2q9_with_depth():
+    L = abi.lib()
+    h = C.c_void_p()
+    assert L.flb_block_create(C.byref(_desc(q=9, nz=4)), C.byref(h)) == abi.FLB_ERR_ARG
+
+
+def test_null_arguments():
+    L = abi.lib()
+    assert L.flb_block_create(None, None) == abi.FLB_ERR_ARG
+    for fn in ("flb_block_destroy",):
+        assert getattr(L, fn)(None) == abi.FLB_OK  # destroying nothing is a no-op
+    for fn in ("flb_apply_cell_dynamics", "flb_stream", "flb_synchronize", "flb_apply_halfway_bc"):
+        assert getattr(L, fn)(None) == abi.FLB_ERR_ARG
+    assert L.flb_step(None, 1) == abi.FLB_ERR_ARG
+    assert L.flb_set_tasks(None, None, 0) == abi.FLB_ERR_ARG
+    assert "null block" in L.flb_last_error().decode()
+    g = C.c_double()
+    assert L.flb_bandwidth_probe(0, 1, C.byref(g)) == abi.FLB_ERR_ARG
+
+
+def test_no_gpu_fails_loudly():
+    """Without a device the create call reports a HIP error (there is no CPU path)."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    from freelb_amd import BlockLattice, FLBError
+    with pytest.raises(FLBError) as e:
+        BlockLattice(19, (8, 8, 8), omega=1.5)
+    assert e.value.code == abi.FLB_ERR_HIP
+
+
+def test_host_mirror_shapes():
+    from freelb_amd import HalfwayBB, Key_TypePair, weights
+    assert Key_TypePair(2, abi.DYN_BGK).smagorinsky == 0.0
+    assert HalfwayBB(4).void_mask == 1 and not HalfwayBB(4).moving
+    w = weights(19)
+    assert abs(w.sum() - 1.0) < 1e-15 and w.shape == (19,)
+    assert abs(weights(9).sum() - 1.0) < 1e-15

@@ -1,0 +1,238 @@
+"""Parity of the HIP lattice update (through the C ABI) with the oracle.
+
+The oracle restates FreeLB's algorithm (PS streaming, post-stream halfway
+bounce-back) and is itself pinned bit for bit to FreeLB by
+tests/test_oracle.py, so agreement here is agreement with the reference.
+
+Bar: FP32 and FP64 storage reproduce the oracle bit for bit on every cell
+whose populations the reference fully determines -- fluid (AABB) cells in the
+fullway mode, every non-void cell in the halfway modes. The published
+tolerance for FP32 (max relative error 1e-5 on rho and u after 1000 steps at
+100^3) is asserted as well, with u's denominator floored at 1e-3*max|u|.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_case import gpu_lattice, interior_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def _select(case, mode):
+    nonvoid = case.flags != O.VOID
+    if mode == "fw":
+        return case.flags == O.AABB
+    return nonvoid & interior_mask(case)
+
+
+def _run_both(mode, Ni, Nj, Nk, steps, storage="f32", **kw):
+    dtype = np.float64 if storage == "f64" else np.float32
+    case, p0 = O.cavity_case(mode, Ni, Nj, Nk, dtype=dtype, **kw)
+    print(f"[case] mode={mode} dims={Ni}x{Nj}x{Nk} steps={steps} storage={storage}", flush=True)
+    lat = gpu_lattice(case, storage=storage, pops=p0)
+    lat.step(steps)
+    g = lat.pops()
+    gm = lat.moments(0xFF & ~O.VOID)
+    lat.close()
+    ref = case.run(p0.copy(), steps)
+    rm = case.rhou(ref)
+    return case, g, ref, gm, rm
+
+
+@pytest.mark.parametrize("mode", ["fw", "hw", "hwfield"])
+@pytest.mark.parametrize("steps", [39, 40])
+def test_d3q19_f32_bitwise_small(mode, steps):
+    case, g, ref, gm, rm = _run_both(mode, 8, 7, 6, steps)
+    sel = _select(case, mode)
+    assert np.array_equal(g[:, sel], ref[:, sel]), (
+        f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}")
+    assert np.array_equal(gm[:, sel], rm[:, sel])
+
+
+@pytest.mark.parametrize("mode", ["fw", "hw"])
+@pytest.mark.parametrize("dims", [(301, 5, 4), (257, 3, 3), (130, 7, 2)])
+def test_d3q19_f32_bitwise_long_rows(mode, dims):
+    """Rows longer than one wavefront of cells and lengths that are not a
+    multiple of the vector width: exercises the cross-lane edges."""
+    case, g, ref, gm, rm = _run_both(mode, *dims, 23)
+    sel = _select(case, mode)
+    assert np.array_equal(g[:, sel], ref[:, sel]), (
+        f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}")
+
+
+@pytest.mark.parametrize("mode", ["fw", "hw", "hwfield"])
+@pytest.mark.parametrize("steps", [99, 100])
+def test_d2q9_f32_bitwise_small(mode, steps):
+    case, g, ref, gm, rm = _run_both(mode, 16, 13, 0, steps)
+    sel = _select(case, mode)
+    assert np.array_equal(g[:, sel], ref[:, sel]), (
+        f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}")
+    assert np.array_equal(gm[:, sel], rm[:, sel])
+
+
+@pytest.mark.parametrize("mode", ["fw", "hw"])
+def test_d3q19_f64_matches_oracle(mode):
+    case, g, ref, gm, rm = _run_both(mode, 12, 10, 9, 61, storage="f64")
+    sel = _select(case, mode)
+    assert np.array_equal(g[:, sel], ref[:, sel]), (
+        f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}")
+
+
+def _max_rel(a, b, floor):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+@pytest.mark.parametrize("mode", ["hw", "fw"])
+def test_config0_cavity_100_1000_steps(mode):
+    """100^3 lid-driven cavity, tau = 0.53, u_lid = 0.1, 1000 steps (FP32)."""
+    case, g, ref, gm, rm = _run_both(mode, 100, 100, 100, 1000)
+    sel = _select(case, mode)
+    rho_err = _max_rel(gm[0, sel], rm[0, sel], 1e-30)
+    umax = float(np.abs(rm[1:, sel]).max())
+    u_err = _max_rel(gm[1:, sel], rm[1:, sel], 1e-3 * umax)
+    nbit = int(np.count_nonzero(g[:, sel] != ref[:, sel]))
+    print(f"[config0 {mode}] max rel err rho {rho_err:.3e} u {u_err:.3e}; "
+          f"populations differing bitwise: {nbit}", flush=True)
+    assert rho_err <= 1e-5 and u_err <= 1e-5
+    assert nbit == 0
+
+
+@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2)])
+def test_shifted_half_storage_tracks_f64(storage, tol):
+    """Shifted FP16 / BF16 storage against the FP64 oracle of the same case."""
+    case64, p64 = O.cavity_case("hw", 24, 24, 24, dtype=np.float64)
+    ref = case64.run(p64.copy(), 200)
+    rm = case64.rhou(ref)
+    case32, p32 = O.cavity_case("hw", 24, 24, 24)
+    lat = gpu_lattice(case32, storage=storage, pops=p32)
+    lat.step(200)
+    gm = lat.moments(0xFF & ~O.VOID)
+    lat.close()
+    sel = _select(case32, "hw")
+    du = gm[1:, sel] - rm[1:, sel]
+    l2 = float(np.sqrt((du ** 2).sum() / (rm[1:, sel] ** 2).sum()))
+    print(f"[{storage}] L2 rel velocity error vs FP64 after 200 steps: {l2:.3e}", flush=True)
+    assert l2 < tol
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_mass_conserved_closed_box(storage):
+    """Halfway bounce-back on every wall, no lid. In FP64 storage the total mass
+    (FP64 device reduction) is conserved to round-off. In FP32 the weights
+    T(1)/T(18), T(1)/T(36), T(1)/T(3) do not sum to exactly 1, so every BGK
+    relaxation nudges the mass by O(1e-8) per cell -- the reference drifts the
+    same way -- and the device total must equal the oracle's total instead."""
+    dtype = np.float64 if storage == "f64" else np.float32
+    case, _ = O.cavity_case("hw", 20, 18, 16, dtype=dtype)
+    case.bcs = [(O.BB | O.BBMW, O.VOID, O.HWBC_NORMAL)]
+    case.L.nbc = 1
+    case.L.bcs[0].cell_mask, case.L.bcs[0].void_mask, case.L.bcs[0].kind = O.BB | O.BBMW, O.VOID, 0
+    rng = np.random.default_rng(7)
+    rhou = np.concatenate([1.0 + 1e-3 * rng.standard_normal((1, case.N)),
+                           1e-3 * rng.standard_normal((3, case.N))]).astype(dtype)
+    lat = gpu_lattice(case, storage=storage)
+    lat.init_equilibrium(rhou[0], rhou[1:])
+    m0 = lat.mass_momentum()
+    lat.step(301)
+    m1 = lat.mass_momentum()
+    lat.close()
+    print(f"[mass {storage}] before {m0[0]!r} after {m1[0]!r}", flush=True)
+    if storage == "f64":
+        assert abs(m1[0] - m0[0]) / m0[0] < 1e-12
+    else:
+        p = case.init_feq(rhou)
+        p = case.run(p, 301)
+        sel = case.flags != O.VOID
+        ref_mass = float(p[:, sel].astype(np.float64).sum())
+        assert abs(m1[0] - ref_mass) / ref_mass < 1e-12
+
+
+def test_call_order_and_layout_errors():
+    from freelb_amd import FLBError, Key_TypePair
+    from freelb_amd import abi
+    case, p0 = O.cavity_case("fw", 6, 6, 6)
+    lat = gpu_lattice(case, pops=p0)
+    lat.ApplyCellDynamics()
+    with pytest.raises(FLBError) as e:
+        lat.ApplyCellDynamics()
+    assert e.value.code == abi.FLB_ERR_STATE
+    lat.Stream()
+    with pytest.raises(FLBError) as e:
+        lat.Stream()
+    assert e.value.code == abi.FLB_ERR_STATE
+    lat.ApplyBoundaries()
+    # a task matching the void overlap layer is rejected (the kernels do not visit it)
+    with pytest.raises(FLBError) as e:
+        lat.set_tasks([Key_TypePair(O.VOID | O.AABB, abi.DYN_BGK)])
+    assert e.value.code == abi.FLB_ERR_LAYOUT
+    lat.close()
+
+
+def test_two_call_shape_equals_step():
+    """ApplyCellDynamics + Stream + ApplyBoundaries per step == flb_step(n)."""
+    case, p0 = O.cavity_case("hw", 10, 9, 8)
+    a = gpu_lattice(case, pops=p0)
+    b = gpu_lattice(case, pops=p0)
+    for _ in range(7):
+        a.ApplyCellDynamics()
+        a.Stream()
+        a.ApplyBoundaries()
+    b.step(7)
+    assert np.array_equal(a.pops(), b.pops())
+    a.close()
+    b.close()
+
+
+# ---- against FreeLB's own outputs (tests/golden, written by FreeLB) ----------
+import glob  # noqa: E402
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _fixtures():
+    return sorted(p for p in glob.glob(os.path.join(GOLDEN, "cavity_d*_[0-9]*x*x*_[0-9]*.npz"))
+                  if not p.endswith("_midplane.npz"))
+
+
+@pytest.mark.parametrize("path", _fixtures(), ids=lambda p: os.path.basename(p))
+def test_gpu_matches_freelb_fixture(path):
+    base = os.path.basename(path)[len("cavity_"):-len(".npz")]
+    lat, mode, dims, steps = base.split("_")
+    ni, nj, nk = (int(v) for v in dims.split("x"))
+    steps = int(steps)
+    fx = np.load(path)
+    case, p0 = O.cavity_case(mode, ni, nj, nk if lat == "d3q19" else 0)
+    print(f"[fixture] {base}", flush=True)
+    g = gpu_lattice(case, pops=p0)
+    g.step(steps)
+    pops = g.pops()
+    g.close()
+    sel = case.flags == O.AABB if mode in ("fw", "mrt", "smag") else _select(case, "hw")
+    assert np.array_equal(pops[:, sel], fx["pops"][:, sel]), (
+        f"max |diff| {np.abs(pops[:, sel] - fx['pops'][:, sel]).max()}")
+
+
+@pytest.mark.parametrize("mode", ["hw", "fw"])
+def test_gpu_config0_against_freelb(mode):
+    """100^3 cavity, 1000 steps, FP32: the GPU's rho/u equal FreeLB's bit for bit
+    (hash over every non-void cell in the halfway mode, where FreeLB determines
+    all of them; mid-plane slice of the fluid cells in the fullway mode)."""
+    meta = json.load(open(os.path.join(GOLDEN, "cavity_large.json")))[f"d3q19_{mode}_100x100x100_1000"]
+    case, p0 = O.cavity_case(mode, 100, 100, 100)
+    g = gpu_lattice(case, pops=p0)
+    g.step(1000)
+    rhou = g.moments(0xFF & ~O.VOID)
+    g.close()
+    nonvoid = case.flags != O.VOID
+    if mode == "hw":
+        assert hashlib.sha256(rhou[:, nonvoid].tobytes()).hexdigest() == meta["sha256_rhou_nonvoid"]
+    sl = np.load(os.path.join(GOLDEN, f"cavity_d3q19_{mode}_100x100x100_1000_midplane.npz"))
+    y = int(sl["y"])
+    mine = rhou.reshape(4, 102, 102, 102)[:, :, y, :]
+    fl = case.flags.reshape(102, 102, 102)[:, y, :]
+    fluid = fl == O.AABB
+    assert np.array_equal(mine[:, fluid], sl["rhou_y"][:, fluid])
