@@ -16,6 +16,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -113,6 +114,19 @@ def time_storage(storage, steps, warmup, timer):
             "finite": finite}
 
 
+def profiled_traffic(storage):
+    """HBM bytes per launch of the step kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
+    copy kernel of known size in the same runs). None when absent."""
+    path = os.path.join(ROOT, "profiles", "k_step_cell_d3q19_cavity256.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(rf"^traffic_bytes_per_launch {storage} = (\d+)", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(n=N_EDGE, steps=20):
     """FreeLB's own benchmarks/cavity3d driver (built from the reference sources
     into oracle/_ref by oracle/Makefile) on the host cores, bounded sample."""
@@ -196,7 +210,7 @@ def main():
                    "storage": "f32", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(f32["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(f32["frac"], 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic("f32")},
     }
     for s, r in runs.items():
         if s == "f32":

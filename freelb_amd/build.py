@@ -58,6 +58,26 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     return LIB
 
 
+EXAMPLES = {"cavity3d": "cavity3d.cpp"}
+
+
+def build_examples(verbose: bool = False) -> list:
+    """C++ drivers over include/flb/block_lattice.hpp, built with the host compiler."""
+    root = os.path.dirname(HERE)
+    out = []
+    for name, src in EXAMPLES.items():
+        exe = os.path.join(HERE, "examples", name)
+        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra",
+               "-I" + os.path.join(root, "include"), os.path.join(HERE, "examples", src),
+               "-L" + HERE, "-lflb_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_examples(verbose="-v" in sys.argv)
     print(LIB)

@@ -169,10 +169,12 @@ int family_of(const flb_block_s* b);
 
 // Step kernel form and cells per thread along x. 0 selects the cell form
 // (one cell per thread, any periodic axis); 1, 2, 4 the row form, where one
-// workgroup of at most 4 wavefronts covers a whole row.
+// workgroup of at most 4 wavefronts covers a whole row. The row form is only
+// taken without halfway boundaries: its whole-line odd-step stores of a row
+// can land on a wall-redirect slot that another row writes in the same launch.
 int vec_of(const flb_block_s* b) {
   const bool per = b->desc.periodic[0] || b->desc.periodic[1] || b->desc.periodic[2];
-  if (per) return 0;
+  if (per || b->nbc > 0) return 0;
   int v = b->vec;
   if (v <= 0) return 0;
   if (family_of(b) == flb::FAM_FULL && v > 2) v = 2;

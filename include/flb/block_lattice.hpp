@@ -1,0 +1,294 @@
+// flb/block_lattice.hpp -- C++ host adapter over the C ABI (include/flb_abi.h).
+//
+// It keeps the call shape of FreeLB's block-lattice interface so that a
+// FreeLB driver's main loop ports by changing types, not structure:
+//
+//   FreeLB (benchmarks/cavity3d/cavity3d.cpp:155-222)     this adapter
+//   -----------------------------------------------       ------------------------------------
+//   using BulkTask = tmp::Key_TypePair<AABB,               using BulkTask = tmp::Key_TypePair<AABB,
+//     collision::BGK<moment::rhoU<CELL>,                     collision::BGK<moment::rhoU<CELL>,
+//                    equilibrium::SecondOrder<CELL>>>;                      equilibrium::SecondOrder<CELL>>>;
+//   using NSTask = TaskSelector<uint8_t, CELL, ...>;       using NSTask = TaskSelector<uint8_t, CELL, ...>;
+//   NSLattice.ApplyCellDynamics<NSTask>(FlagFM);           NSLattice.ApplyCellDynamics<NSTask>(flags);
+//   NSLattice.Stream();                                    NSLattice.Stream();
+//   BM.Apply();                                            BM.Apply();
+//
+// Types mirror FreeLB's names (lbm/collision.h, lbm/moment.h,
+// lbm/equilibrium.h, boundary/bounce_back_boundary.h, utils/tmp.h,
+// lbm/lattice_set.h); each dynamics type maps to one FLB_DYN_* kind, and a
+// TaskSelector to the first-match task table the GPU kernel evaluates per
+// cell. Everything runs on the GPU: the adapter only builds tables and calls
+// the C ABI. Errors throw flb::Error carrying the ABI's code and message.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../flb_abi.h"
+
+namespace flb {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(int rc) {
+  if (rc != FLB_OK) throw Error(rc, std::string("flb: ") + flb_last_error());
+}
+
+// ---- lattice sets (lbm/lattice_set.h:168-194, 259-296) -----------------------
+template <typename T>
+struct D3Q19 {
+  using FloatType = T;
+  static constexpr unsigned int d = 3;
+  static constexpr unsigned int q = 19;
+};
+template <typename T>
+struct D2Q9 {
+  using FloatType = T;
+  static constexpr unsigned int d = 2;
+  static constexpr unsigned int q = 9;
+};
+
+// The CELL of FreeLB's dynamics templates; only its lattice set matters here.
+template <typename T, typename LatSet>
+struct Cell {
+  using FloatType = T;
+  using LatticeSet = LatSet;
+};
+
+// ---- moments / equilibria / collisions (tags) ----------------------------------
+namespace moment {
+template <typename CELL, bool WriteToField = false>
+struct rhoU {};  // lbm/moment.h:381-398
+template <typename CELL>
+struct useFieldrhoU {};  // lbm/moment.h:71-90
+}  // namespace moment
+
+namespace equilibrium {
+template <typename CELL>
+struct SecondOrder {};  // lbm/equilibrium.h:79-117
+}  // namespace equilibrium
+
+namespace collision {
+template <typename MomentaScheme, typename EquilibriumScheme>
+struct BGK;  // lbm/collision.h:32-59
+template <typename CELL, typename EQ>
+struct BGK<moment::rhoU<CELL>, EQ> {
+  static constexpr uint8_t kind = FLB_DYN_BGK;
+};
+template <typename CELL, typename EQ>
+struct BGK<moment::useFieldrhoU<CELL>, EQ> {
+  static constexpr uint8_t kind = FLB_DYN_BGK_FIELD;
+};
+template <typename CELL>
+struct BounceBack {  // lbm/collision.ur.h:62-93
+  static constexpr uint8_t kind = FLB_DYN_BOUNCEBACK;
+};
+template <typename CELL>
+struct BounceBackMovingWall {  // lbm/collision.ur.h:130-187
+  static constexpr uint8_t kind = FLB_DYN_BOUNCEBACK_MW;
+};
+template <typename CELL, bool WriteToField = false>
+struct MRT_Feq_RhoU {  // lbm/collisionMRT.h:132-190
+  static constexpr uint8_t kind = FLB_DYN_MRT;
+};
+template <typename MomentaScheme, typename EquilibriumScheme, bool WriteToField = false>
+struct SmagorinskyBGK {  // lbm/collisionLES.h:29-67
+  static constexpr uint8_t kind = FLB_DYN_SMAG_BGK;
+};
+// MRT relaxation whose shear rates take the Smagorinsky omega (no FreeLB form).
+template <typename CELL>
+struct SmagorinskyMRT {
+  static constexpr uint8_t kind = FLB_DYN_SMAG_MRT;
+};
+}  // namespace collision
+
+// ---- task selection (utils/tmp.h:194-202, 283-309) -------------------------------
+namespace tmp {
+template <auto KeyValue, typename Type>
+struct Key_TypePair {
+  static constexpr auto key = KeyValue;
+  using type = Type;
+};
+}  // namespace tmp
+
+template <typename FlagType, typename CELL, typename... Tasks>
+struct TaskSelector {
+  static std::vector<flb_task> table(float smagorinsky) {
+    std::vector<flb_task> t;
+    (t.push_back(flb_task{static_cast<uint8_t>(Tasks::key), Tasks::type::kind, {0, 0}, smagorinsky}),
+     ...);
+    return t;
+  }
+};
+
+// ---- halfway bounce-back kinds (boundary/bounce_back_boundary.h:169-256) -----------
+namespace bounceback {
+template <typename CELL>
+struct normal {
+  static constexpr uint8_t kind = FLB_HWBC_NORMAL;
+};
+template <typename CELL>
+struct movingwall {
+  static constexpr uint8_t kind = FLB_HWBC_MOVING;
+};
+}  // namespace bounceback
+
+enum class Storage { F32 = FLB_STORE_F32, F16S = FLB_STORE_F16S, BF16S = FLB_STORE_BF16S, F64 = FLB_STORE_F64 };
+
+// ---- one block on the GPU ----------------------------------------------------------
+// T is the host type of populations and fields (float for F32 / F16S / BF16S
+// storage, double for F64).
+template <typename T, typename LatSet>
+class BlockLattice {
+ public:
+  using FloatType = T;
+  using LatticeSet = LatSet;
+
+  // nx, ny, nz include the overlap layer (FreeLB's block extent); omega = 1/tau.
+  BlockLattice(int nx, int ny, int nz, double omega, Storage storage = Storage::F32,
+               int overlap = 1, void* stream = nullptr, const int periodic[3] = nullptr) {
+    flb_block_desc d{};
+    d.q = int(LatSet::q);
+    d.storage = int(storage);
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = LatSet::d == 3 ? nz : 1;
+    d.overlap = overlap;
+    for (int a = 0; a < 3; ++a) d.periodic[a] = periodic ? periodic[a] : 0;
+    d.omega = omega;
+    d.stream = stream;
+    check(flb_block_create(&d, &h_));
+    n_ = size_t(nx) * size_t(ny) * size_t(d.nz);
+  }
+  ~BlockLattice() { flb_block_destroy(h_); }
+  BlockLattice(const BlockLattice&) = delete;
+  BlockLattice& operator=(const BlockLattice&) = delete;
+
+  flb_block handle() const { return h_; }
+  size_t size() const { return n_; }
+
+  // The flag field (FreeLB's BlockFieldManager<FLAG>). Like FreeLB's
+  // FlagFM.copyToDevice(), an upload is explicit: ApplyCellDynamics uploads a
+  // flag field the first time it sees that object and reuses the device copy
+  // afterwards; call setFlags again after changing its contents.
+  void setFlags(const std::vector<uint8_t>& flags) {
+    if (flags.size() != n_) throw Error(FLB_ERR_ARG, "flb: flag field size mismatch");
+    check(flb_upload_flags(h_, flags.data()));
+    flags_src_ = &flags;
+  }
+  // RHO / VELOCITY fields (VELOCITY as d*N structure of arrays).
+  void setRhoField(const std::vector<double>& rho) { check(flb_set_field_rho(h_, rho.data())); }
+  void setVelocityField(const std::vector<double>& u) { check(flb_set_field_u(h_, u.data())); }
+  void setSmagorinsky(float cs) { smagorinsky_ = cs; }
+
+  // f_i = rho0 * w_i everywhere (BlockLatticeManager ctor, block_lattice.hh:676-681).
+  void initUniform(double rho0 = 1.0) {
+    std::vector<double> rhou(n_ * (LatSet::d + 1), 0.0);
+    for (size_t k = 0; k < n_; ++k) rhou[k] = rho0;
+    check(flb_init_equilibrium(h_, rhou.data()));
+  }
+  void initEquilibrium(const std::vector<double>& rhou) { check(flb_init_equilibrium(h_, rhou.data())); }
+
+  std::vector<T> populations() const {
+    std::vector<T> p(n_ * LatSet::q);
+    if constexpr (sizeof(T) == 8) check(flb_download_pops_f64(h_, p.data()));
+    else check(flb_download_pops_f32(h_, p.data()));
+    return p;
+  }
+  void setPopulations(const std::vector<T>& p) {
+    if constexpr (sizeof(T) == 8) check(flb_upload_pops_f64(h_, p.data()));
+    else check(flb_upload_pops_f32(h_, p.data()));
+  }
+
+  // BlockLattice::ApplyCellDynamics<TaskSelector>(flagarr) (block_lattice.hh:72-92).
+  template <typename Selector>
+  void ApplyCellDynamics(const std::vector<uint8_t>& flags) {
+    if (flags_src_ != &flags) setFlags(flags);
+    const std::vector<flb_task> t = Selector::table(smagorinsky_);
+    if (t.size() != tasks_.size() || !same_tasks(t)) {
+      check(flb_set_tasks(h_, t.data(), int(t.size())));
+      tasks_ = t;
+    }
+    check(flb_apply_cell_dynamics(h_));
+  }
+  // BlockLattice::Stream (block_lattice.hh:64-70): completes the fused step.
+  void Stream() { check(flb_stream(h_)); }
+
+  // rho, u of cells with flag & mask (moment::rhoU<CELL, true>): (d+1)*N SoA.
+  std::vector<T> moments(uint8_t mask) const {
+    std::vector<T> out(n_ * (LatSet::d + 1));
+    check(flb_moments(h_, mask, out.data()));
+    return out;
+  }
+
+  // called by the halfway managers
+  void addHalfway(const flb_hwbc& bc) {
+    bcs_.push_back(bc);
+    check(flb_set_halfway_bc(h_, bcs_.data(), int(bcs_.size())));
+  }
+
+ private:
+  bool same_tasks(const std::vector<flb_task>& t) const {
+    for (size_t k = 0; k < t.size(); ++k)
+      if (t[k].mask != tasks_[k].mask || t[k].dynamics != tasks_[k].dynamics ||
+          t[k].smagorinsky != tasks_[k].smagorinsky)
+        return false;
+    return true;
+  }
+
+  flb_block h_ = nullptr;
+  size_t n_ = 0;
+  const std::vector<uint8_t>* flags_src_ = nullptr;
+  std::vector<flb_task> tasks_;
+  std::vector<flb_hwbc> bcs_;
+  float smagorinsky_ = 0.0f;
+};
+
+// ---- halfway managers (boundary/boundary.h:75-129, basic_boundary.h:331-348) -------
+class AbstractBlockBoundary {
+ public:
+  virtual ~AbstractBlockBoundary() = default;
+  virtual void Apply() = 0;
+};
+
+// BBLikeFixedBlockBdManager<bounceback::normal|movingwall<CELL>, ...>: registers the
+// manager's boundary cells (flag & cellflag) and wall flag (voidflag) with the
+// lattice; the fix-up itself runs inside the fused update kernel.
+template <typename CELLDYNAMICS, typename LATTICE>
+class BBLikeFixedBlockBdManager final : public AbstractBlockBoundary {
+ public:
+  BBLikeFixedBlockBdManager(std::string name, LATTICE& lat, std::uint8_t cellflag,
+                            std::uint8_t voidflag = std::uint8_t(1))
+      : name_(std::move(name)), lat_(lat) {
+    lat_.addHalfway(flb_hwbc{cellflag, voidflag, CELLDYNAMICS::kind, 0});
+  }
+  void Apply() override { check(flb_apply_halfway_bc(lat_.handle())); }
+  const std::string& name() const { return name_; }
+
+ private:
+  std::string name_;
+  LATTICE& lat_;
+};
+
+class BlockBoundaryManager {
+ public:
+  template <typename... Args>
+  explicit BlockBoundaryManager(Args*... args) : bds_{args...} {}
+  void Apply() {
+    for (AbstractBlockBoundary* b : bds_) b->Apply();
+  }
+
+ private:
+  std::vector<AbstractBlockBoundary*> bds_;
+};
+
+}  // namespace flb
