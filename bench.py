@@ -60,34 +60,10 @@ class HipTimer:
         return float(ms.value)
 
 
-def cavity_flags(n):
-    """Flag layout of the reference cavity driver: void shell 1, fluid 2,
-    walls 4, lid 8 (benchmarks/cavity3d/cavity3d.cpp:121-142)."""
-    m = n + 2
-    f = np.full((m, m, m), 1, dtype=np.uint8)
-    f[1:-1, 1:-1, 1:-1] = 4
-    f[2:-2, 2:-2, 2:-2] = 2
-    f[n, 2:-2, 2:-2] = 8  # top layer z = n, x/y in [2, n-1]
-    return f.reshape(-1)
-
-
-def make_block(storage, n=N_EDGE):
-    from freelb_amd import BlockLattice, HalfwayBB, Key_TypePair
-    from freelb_amd import abi
-    omega = float(np.float32(1.0) / np.float32(0.53))
-    ulid = 0.1
-    lat = BlockLattice(19, (n + 2, n + 2, n + 2), omega=omega, storage=storage)
-    flags = cavity_flags(n)
-    lat.upload_flags(flags)
-    lat.set_tasks([Key_TypePair(2 | 4 | 8, abi.DYN_BGK)])
-    lat.set_halfway_bc([HalfwayBB(4, 1, False), HalfwayBB(8, 1, True)])
-    N = flags.size
-    u = np.zeros((3, N))
-    u[0, flags == 8] = ulid
-    lat.set_field_rho(np.ones(N))
-    lat.set_field_u(u)
-    lat.init_uniform(1.0)
-    return lat
+def make_block(storage, n=N_EDGE, collision="bgk"):
+    from freelb_amd.cases import cavity_block
+    return cavity_block(n, storage=storage, collision=collision,
+                        tau=0.53 if collision == "bgk" else 0.5005)
 
 
 def time_storage(storage, steps, warmup, timer):
@@ -112,6 +88,115 @@ def time_storage(storage, steps, warmup, timer):
     return {"storage": storage, "ms_per_step": ms_step, "mlups": mlups, "wall_s": wall,
             "achieved_gbs": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": bytes_per,
             "finite": finite}
+
+
+def _vorticity(rhou, n):
+    """Central-difference curl of u over the cavity cells whose six
+    neighbours are cavity cells; rhou is [4][(n+2)^3]."""
+    u = [np.asarray(rhou[1 + d], dtype=np.float64).reshape(n + 2, n + 2, n + 2) for d in range(3)]
+    ux, uy, uz = u  # indexed [z, y, x]
+    c = slice(2, n)
+
+    def d(f, axis):
+        sl_p = [c, c, c]
+        sl_m = [c, c, c]
+        sl_p[axis] = slice(3, n + 1)
+        sl_m[axis] = slice(1, n - 1)
+        return 0.5 * (f[tuple(sl_p)] - f[tuple(sl_m)])
+    # axis 0 = z, 1 = y, 2 = x
+    return np.stack([d(uz, 1) - d(uy, 0), d(ux, 0) - d(uz, 2), d(uy, 2) - d(ux, 1)])
+
+
+def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
+                    storages=("f32", "f16s", "bf16s"), seed=2024, u0_amp=1e-3, timer=None):
+    """Errors of FP32 / FP16-shift / BF16-shift storage against an FP64 run of
+    the same cavity (D3Q19, halfway walls, lid 0.1): rho and u from
+    flb_moments, vorticity by central differences. Start: rho = 1, u i.i.d.
+    U(-u0_amp, u0_amp) per component (numpy default_rng(seed)), f = feq."""
+    from freelb_amd.cases import VOID, cavity_block, cavity_flags
+    live = cavity_flags(n) != VOID
+    out = {"cells": n ** 3, "steps": steps, "seed": seed, "u0_amp": u0_amp, "runs": {}}
+    for coll in collisions:
+        tau = 0.53 if coll == "bgk" else 0.5005
+        ref = None
+        for s in ("f64",) + tuple(storages):
+            lat = cavity_block(n, storage=s, collision=coll, tau=tau, u0_amp=u0_amp, seed=seed)
+            try:
+                t0 = time.perf_counter()
+                lat.step(steps)
+                lat.synchronize()
+                dt = time.perf_counter() - t0
+                m = lat.moments()
+            finally:
+                lat.close()
+            rec = {"mlups": round(n ** 3 * steps / dt / 1e6, 1),
+                   "finite": bool(np.isfinite(m[:, live]).all())}
+            if s == "f64":
+                ref = m
+                vref = _vorticity(ref, n)
+                rec["u_max"] = float(np.abs(ref[1:, live]).max())
+            else:
+                du = (m[1:, live].astype(np.float64) - ref[1:, live])
+                rec["u_l2_rel"] = float(np.sqrt((du ** 2).sum() / (ref[1:, live] ** 2).sum()))
+                rec["rho_max_abs"] = float(np.abs(m[0, live].astype(np.float64) - ref[0, live]).max())
+                dv = _vorticity(m, n) - vref
+                rec["vort_l2_rel"] = float(np.sqrt((dv ** 2).sum() / (vref ** 2).sum()))
+            out["runs"][f"{coll}/{s}"] = rec
+            print(f"# precision {coll}/{s}: {rec}", file=sys.stderr, flush=True)
+    return out
+
+
+def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", loop_steps=20):
+    """512 independent D3Q19 BGK halfway cavities, edges drawn per axis from
+    U{16..96}, tau from U[0.51, 0.9], lid speed from U[0.02, 0.1] (numpy
+    default_rng(seed)), stepped together: one launch per step over the block
+    table (autotuned shape), recorded as a hipGraph of graph_steps steps. The
+    same blocks stepped with one launch per block and step (FreeLB's
+    per-block manager loop) are timed beside it."""
+    from freelb_amd import BlockLatticeManager
+    from freelb_amd.cases import cavity_block
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(16, 97, size=(nblocks, 3))
+    taus = rng.uniform(0.51, 0.9, size=nblocks)
+    lids = rng.uniform(0.02, 0.1, size=nblocks)
+    t0 = time.perf_counter()
+    blocks = [cavity_block(tuple(int(v) for v in edges[k]), storage=storage, tau=float(taus[k]),
+                           u_lid=float(lids[k])) for k in range(nblocks)]
+    setup_s = time.perf_counter() - t0
+    cells = int(np.prod(edges, axis=1).sum())
+    try:
+        # one launch per block and step
+        for b in blocks:
+            b.step(2)
+        timer.sync()
+        timer.start()
+        for _ in range(loop_steps // 2):
+            for b in blocks:
+                b.step(2)
+        loop_ms = timer.stop() / loop_steps
+        with BlockLatticeManager(blocks) as m:
+            tuned = m.autotune(10)
+            m.capture(graph_steps)
+            m.replay()
+            m.synchronize()
+            timer.start()
+            m.replay()
+            graph_ms = timer.stop() / graph_steps
+            m.synchronize()
+            mass = [b.mass_momentum()[0] for b in blocks[:4]]
+    finally:
+        for b in blocks:
+            b.close()
+    bytes_per = 152 if storage == "f32" else 76
+    return {"workload": f"{nblocks} cavities, edges U{{16..96}}^3 (seed {seed}), tau U[0.51,0.9], "
+                        f"lid U[0.02,0.1], hipGraph of {graph_steps} steps",
+            "blocks": nblocks, "cells": cells, "storage": storage,
+            "mlups": round(cells / (graph_ms / 1e3) / 1e6, 1),
+            "ms_per_step": round(graph_ms, 5),
+            "achieved_gbs": round(cells * bytes_per / (graph_ms / 1e3) / 1e9, 1),
+            "per_block_launch_mlups": round(cells / (loop_ms / 1e3) / 1e6, 1),
+            "launch": tuned, "setup_s": round(setup_s, 1),
+            "finite": bool(np.isfinite(mass).all())}
 
 
 def profiled_traffic(storage):
@@ -178,12 +263,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f32,f16s,bf16s")
     ap.add_argument("--no-probe", action="store_true", help="skip the device copy-bandwidth probe")
+    ap.add_argument("--precision", action="store_true",
+                    help="run the 256^3 x 2000-step FP32/FP16/BF16 vs FP64 error study instead")
+    ap.add_argument("--precision-steps", type=int, default=2000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-block batched run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each GPU would run its own share of blocks; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    if args.precision:
+        print(json.dumps({"precision_study": precision_study(steps=args.precision_steps)}), flush=True)
+        return
     timer = HipTimer()
     runs = {}
     for s in args.storages.split(","):
@@ -226,6 +318,9 @@ def main():
         abi.check(abi.lib().flb_bandwidth_probe(C.c_size_t(1 << 30), 20, C.byref(gbs)))
         line["device_copy_gbs"] = round(gbs.value, 1)
         line["roofline"]["frac_of_copy"] = round(f32["achieved_gbs"] / gbs.value, 4)
+    if not args.no_batch:
+        line["batch"] = batch_bench(timer)
+        print(f"# batch: {line['batch']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

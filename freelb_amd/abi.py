@@ -30,7 +30,9 @@ EXPORTS = (
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
     "flb_upload_pops_f64", "flb_download_pops_f32", "flb_download_pops_f64", "flb_init_equilibrium",
     "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
-    "flb_reduce_mass_momentum", "flb_bandwidth_probe",
+    "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
+    "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
+    "flb_batch_synchronize", "flb_batch_autotune", "flb_bandwidth_probe",
 )
 
 
@@ -56,6 +58,10 @@ class BlockInfo(C.Structure):
                 ("n_cells", C.c_int64), ("n_interior", C.c_int64), ("stride", C.c_int64),
                 ("step", C.c_int64), ("parity", C.c_int), ("d_pops", C.c_void_p),
                 ("d_flags", C.c_void_p)]
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [("waves_per_group", C.c_int), ("grid", C.c_int)]
 
 
 class FLBError(RuntimeError):
@@ -104,6 +110,15 @@ def lib() -> C.CDLL:
         "flb_step": [P, C.c_int],
         "flb_moments": [P, C.c_uint8, P],
         "flb_reduce_mass_momentum": [P, C.c_uint8, C.POINTER(C.c_double)],
+        "flb_batch_create": [C.POINTER(P), C.c_int, P, C.POINTER(P)],
+        "flb_batch_destroy": [P],
+        "flb_batch_info": [P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(LaunchCfg)],
+        "flb_batch_set_launch": [P, C.POINTER(LaunchCfg)],
+        "flb_batch_step": [P, C.c_int],
+        "flb_batch_capture": [P, C.c_int],
+        "flb_batch_replay": [P],
+        "flb_batch_synchronize": [P],
+        "flb_batch_autotune": [P, C.c_int, C.POINTER(LaunchCfg), C.POINTER(C.c_double)],
         "flb_bandwidth_probe": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():

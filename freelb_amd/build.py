@@ -9,8 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflb_hip.so")
 SOURCES = ["flb_inst_f32.hip", "flb_inst_f16s.hip", "flb_inst_bf16s.hip", "flb_inst_f64.hip",
-           "flb_probe.hip", "flb_abi.hip"]
-HEADERS = ["flb_cell.hpp", "flb_storage.hpp", "flb_internal.hpp", "flb_kernels.hpp"]
+           "flb_probe.hip", "flb_abi.hip", "flb_batch.hip"]
+HEADERS = ["flb_cell.hpp", "flb_storage.hpp", "flb_internal.hpp", "flb_kernels.hpp", "flb_host.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLB_OFFLOAD_ARCH", "gfx950")
@@ -32,13 +32,16 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
-    """Compile every source to an object in parallel, then link the library."""
+def build(verbose: bool = False, jobs: int = 0, extra_flags=(), lib: str = LIB,
+          objdir: str = "") -> str:
+    """Compile every source to an object in parallel, then link the library.
+    `extra_flags` (e.g. -D switches), `lib` and `objdir` build a variant
+    library beside the default one."""
     from concurrent.futures import ThreadPoolExecutor
 
-    objdir = os.path.join(HERE, "_build")
+    objdir = objdir or os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    compile_flags = [f for f in FLAGS if f != "-shared"]
+    compile_flags = [f for f in FLAGS if f != "-shared"] + list(extra_flags)
 
     def compile_one(src):
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
@@ -51,11 +54,11 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
-    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB, *objs]
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
-    return LIB
+    return lib
 
 
 EXAMPLES = {"cavity3d": "cavity3d.cpp"}

@@ -1,0 +1,75 @@
+"""Lid-driven cavity set-up as FreeLB's cavity drivers build it.
+
+Flags follow benchmarks/cavity3d/cavity3d.cpp:121-142 with the AABB rule of
+geometry/basic_geometry.hh:38-48: a void shell of one cell (1), walls on the
+outer layer of the n^3 cavity (4), the lid on the top layer without its edges
+(8), fluid inside (2). In the halfway mode every non-void cell collides and
+the walls sit at the half link to the void shell (bounceback::normal for 4,
+bounceback::movingwall for 8, boundary/bounce_back_boundary.h:169-256); the
+lid's RHO / VELOCITY fields hold rho = 1, u = (u_lid, 0, 0).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .lattice import BlockLattice, HalfwayBB, Key_TypePair
+
+VOID, FLUID, WALL, LID = 1, 2, 4, 8
+
+
+def cavity_flags(n: int, ny: int | None = None, nz: int | None = None) -> np.ndarray:
+    """Flags of an n x ny x nz cavity plus the void shell, FreeLB index order."""
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    f = np.full((nz + 2, ny + 2, n + 2), VOID, dtype=np.uint8)
+    f[1:-1, 1:-1, 1:-1] = WALL
+    f[2:-2, 2:-2, 2:-2] = FLUID
+    f[nz, 2:-2, 2:-2] = LID
+    return f.reshape(-1)
+
+
+def cavity_block(n, storage: str = "f32", collision: str = "bgk", tau: float = 0.53,
+                 u_lid: float = 0.1, cs: float = 0.17, u0_amp: float = 0.0, seed: int = 0,
+                 mrt_rates=None, stream=None) -> BlockLattice:
+    """A D3Q19 halfway cavity of n^3 (or n = (nx, ny, nz)) cells on the GPU, at equilibrium with
+    rho = 1 and u = 0, or u drawn i.i.d. from U(-u0_amp, u0_amp) per
+    component on every non-void cell (numpy default_rng(seed)).
+
+    collision: "bgk" (collision::BGK) or "mrt_les" (MRT whose shear rates
+    take the Smagorinsky omega with constant cs)."""
+    omega = float(np.float32(1.0) / np.float32(tau))
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else (int(v) for v in n)
+    lat = BlockLattice(19, (nx + 2, ny + 2, nz + 2), omega=omega, storage=storage, stream=stream)
+    try:
+        flags = cavity_flags(nx, ny, nz)
+        lat.upload_flags(flags)
+        mask = FLUID | WALL | LID
+        if collision == "bgk":
+            lat.set_tasks([Key_TypePair(mask, abi.DYN_BGK)])
+        elif collision == "mrt":
+            lat.set_tasks([Key_TypePair(mask, abi.DYN_MRT)])
+            lat.set_mrt_rates(mrt_rates)
+        elif collision == "mrt_les":
+            lat.set_tasks([Key_TypePair(mask, abi.DYN_SMAG_MRT, cs)])
+            lat.set_mrt_rates(mrt_rates)
+        else:
+            raise ValueError(f"unknown collision {collision!r}")
+        lat.set_halfway_bc([HalfwayBB(WALL, VOID, False), HalfwayBB(LID, VOID, True)])
+        N = flags.size
+        u = np.zeros((3, N))
+        u[0, flags == LID] = u_lid
+        lat.set_field_rho(np.ones(N))
+        lat.set_field_u(u)
+        if u0_amp:
+            rng = np.random.default_rng(seed)
+            u0 = np.zeros((3, N))
+            live = flags != VOID
+            u0[:, live] = rng.uniform(-u0_amp, u0_amp, size=(3, int(live.sum())))
+            lat.init_equilibrium(np.ones(N), u0)
+        else:
+            lat.init_uniform(1.0)
+    except Exception:
+        lat.close()
+        raise
+    return lat

@@ -15,46 +15,11 @@
 #include "../../include/flb_abi.h"
 #include "flb_cell.hpp"
 #include "flb_internal.hpp"
+#include "flb_host.hpp"
 
 using flb::KArgs;
 
-struct flb_block_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  flb_block_desc desc{};
-  int d = 3;
-  int64_t N = 0;        // host cells nx*ny*nz
-  int64_t stride = 0;   // slots between two per-cell arrays (pads included)
-  int pitch = 0;        // slots per row
-  int64_t plane = 0;    // slots per plane
-  int64_t dpop = 0, zpop = 0;  // population strides (see flb_internal.hpp)
-  int ypop = 0;
-  int64_t pop_elems = 0;       // elements allocated for the populations
-  size_t elem = 4;   // bytes per stored population
-  size_t real = 4;   // bytes of the compute type
-  void* d_pops = nullptr;
-  uint8_t* d_flags = nullptr;
-  void* d_rho = nullptr;
-  void* d_u = nullptr;
-  double* d_partials = nullptr;
-  double* d_red = nullptr;
-  void* d_stage = nullptr;
-  size_t stage_bytes = 0;
-  std::vector<uint8_t> h_flags;
-  bool flags_set = false;
-  flb_task tasks[flb::kMaxTasks]{};
-  int ntasks = 0;
-  flb_hwbc bcs[flb::kMaxBc]{};
-  int nbc = 0;
-  double rates[19]{};
-  int64_t step = 0;
-  int parity = 0;
-  int pending = 0;  // 1: cell dynamics applied, stream not yet called
-  int vec = -1;  // step kernel form: -1 cell form, 1/2/4 row form with that many cells per thread
-  int wpb = 4;   // wavefronts per workgroup
-};
-
-namespace {
+namespace flbh {
 
 thread_local std::string g_err;
 
@@ -67,27 +32,6 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
-
-#define HIP_TRY(expr)                                                                 \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(FLB_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
-  } while (0)
-
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
-      (void)hipSetDevice(dev);
-    } else {
-      prev = -1;
-    }
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
 
 bool storage_ok(int s) {
   return s == FLB_STORE_F32 || s == FLB_STORE_F16S || s == FLB_STORE_BF16S || s == FLB_STORE_F64;
@@ -337,10 +281,9 @@ int upload_field(flb_block b, const double* host, int ncomp, void** dst) {
   return FLB_OK;
 }
 
-#define CHECK_BLOCK(b) \
-  if (!(b)) return fail(FLB_ERR_ARG, "null block handle")
+}  // namespace flbh
 
-}  // namespace
+using namespace flbh;
 
 extern "C" {
 
@@ -486,6 +429,7 @@ int flb_synchronize(flb_block b) {
 
 int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
   CHECK_BLOCK(b);
+  b->version++;
   if (n < 0 || n > flb::kMaxTasks) return fail(FLB_ERR_ARG, "task count %d outside [0, %d]", n, flb::kMaxTasks);
   if (n > 0 && !tasks) return fail(FLB_ERR_ARG, "null task table");
   for (int t = 0; t < n; ++t) {
@@ -508,6 +452,7 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
 
 int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
   CHECK_BLOCK(b);
+  b->version++;
   if (n < 0 || n > flb::kMaxBc) return fail(FLB_ERR_ARG, "boundary count %d outside [0, %d]", n, flb::kMaxBc);
   if (n > 0 && !bcs) return fail(FLB_ERR_ARG, "null boundary table");
   for (int k = 0; k < n; ++k) {
@@ -523,6 +468,7 @@ int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
 
 int flb_set_mrt_rates(flb_block b, const double* rates, int q) {
   CHECK_BLOCK(b);
+  b->version++;
   if (!rates) {
     default_rates(b);
     return FLB_OK;
@@ -534,6 +480,7 @@ int flb_set_mrt_rates(flb_block b, const double* rates, int q) {
 
 int flb_set_omega(flb_block b, double omega) {
   CHECK_BLOCK(b);
+  b->version++;
   if (!(omega > 0.0 && omega < 2.0)) return fail(FLB_ERR_ARG, "omega must lie in (0, 2), got %g", omega);
   b->desc.omega = omega;
   return FLB_OK;
@@ -541,6 +488,7 @@ int flb_set_omega(flb_block b, double omega) {
 
 int flb_upload_flags(flb_block b, const uint8_t* flags) {
   CHECK_BLOCK(b);
+  b->version++;
   if (!flags) return fail(FLB_ERR_ARG, "null flags");
   DeviceGuard g(b->device);
   std::vector<uint8_t> saved;
@@ -563,6 +511,7 @@ int flb_upload_flags(flb_block b, const uint8_t* flags) {
 
 int flb_set_field_rho(flb_block b, const double* rho) {
   CHECK_BLOCK(b);
+  b->version++;
   if (!rho) return fail(FLB_ERR_ARG, "null rho");
   DeviceGuard g(b->device);
   return upload_field(b, rho, 1, &b->d_rho);
@@ -570,6 +519,7 @@ int flb_set_field_rho(flb_block b, const double* rho) {
 
 int flb_set_field_u(flb_block b, const double* u) {
   CHECK_BLOCK(b);
+  b->version++;
   if (!u) return fail(FLB_ERR_ARG, "null u");
   DeviceGuard g(b->device);
   return upload_field(b, u, b->d, &b->d_u);

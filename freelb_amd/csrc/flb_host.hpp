@@ -1,0 +1,85 @@
+// Host-side internals shared by the C-ABI translation units: the block
+// object behind flb_block and the helpers that build kernel arguments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/flb_abi.h"
+#include "flb_internal.hpp"
+
+struct flb_block_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  flb_block_desc desc{};
+  int d = 3;
+  int64_t N = 0;        // host cells nx*ny*nz
+  int64_t stride = 0;   // slots between two per-cell arrays (pads included)
+  int pitch = 0;        // slots per row
+  int64_t plane = 0;    // slots per plane
+  int64_t dpop = 0, zpop = 0;  // population strides (see flb_internal.hpp)
+  int ypop = 0;
+  int64_t pop_elems = 0;       // elements allocated for the populations
+  size_t elem = 4;   // bytes per stored population
+  size_t real = 4;   // bytes of the compute type
+  void* d_pops = nullptr;
+  uint8_t* d_flags = nullptr;
+  void* d_rho = nullptr;
+  void* d_u = nullptr;
+  double* d_partials = nullptr;
+  double* d_red = nullptr;
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  std::vector<uint8_t> h_flags;
+  bool flags_set = false;
+  flb_task tasks[flb::kMaxTasks]{};
+  int ntasks = 0;
+  flb_hwbc bcs[flb::kMaxBc]{};
+  int nbc = 0;
+  double rates[19]{};
+  int64_t step = 0;
+  int parity = 0;
+  int pending = 0;  // 1: cell dynamics applied, stream not yet called
+  int vec = -1;  // step kernel form: -1 cell form, 1/2/4 row form with that many cells per thread
+  int wpb = 4;   // wavefronts per workgroup
+  uint64_t version = 0;  // bumped by every change a batch table must see
+};
+
+namespace flbh {
+
+using flb::KArgs;
+
+// Records the message for flb_last_error() (thread-local) and returns code.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(FLB_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+      (void)hipSetDevice(dev);
+    } else {
+      prev = -1;
+    }
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+KArgs make_args(const flb_block_s* b);
+int family_of(const flb_block_s* b);
+
+}  // namespace flbh
+
+#define CHECK_BLOCK(b) \
+  if (!(b)) return fail(FLB_ERR_ARG, "null block handle")

@@ -275,16 +275,42 @@ __device__ __forceinline__ void store_row(typename S::E* p, int i, const typenam
   }
 }
 
+// Population loads and stores of the step kernels. Each population is read
+// and written once per step, so the non-temporal forms (FLB_NT_LOAD /
+// FLB_NT_STORE) can keep them from displacing the neighbour lines that the
+// odd step reuses from L2.
+#ifndef FLB_NT_LOAD
+#define FLB_NT_LOAD 0
+#endif
+#ifndef FLB_NT_STORE
+#define FLB_NT_STORE 0
+#endif
+template <class E>
+__device__ __forceinline__ E ld_pop(const E* p) {
+#if FLB_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <class E>
+__device__ __forceinline__ void st_pop(E* p, E v) {
+#if FLB_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // The step kernel, cell form: one cell per thread, any axis periodic.
 // ---------------------------------------------------------------------------
+// Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
+// up to 64 cells; lane is the lane id.
 template <int Q, class S, int PAR, int FAM, bool PER>
-__global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
+__device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
   using R = typename S::R;
   using E = typename S::E;
-  const int lane = int(threadIdx.x) & (kWave - 1);
-  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-  const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
   const int row = gw / a.wpr;
   if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
   const int wv = gw - row * a.wpr;
@@ -306,14 +332,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      f[i] = S::template from<Q>((P + int64_t(i) * a.dpop)[c0], i);
+      f[i] = S::template from<Q>(ld_pop((P + int64_t(i) * a.dpop) + c0), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
       const int o = opp(i);
       const bool wall = i != 0 && ((lk >> o) & 1u);
       const E* p = wall ? (P + int64_t(i) * a.dpop) + c0
                         : (P + int64_t(o) * a.dpop) + c0 + nb_off<Q, PER, true>(a, o, x, y, z);
-      f[i] = S::template from<Q>(*p, i);
+      f[i] = S::template from<Q>(ld_pop(p), i);
     }
   }
   if (PAR == 1 && lm) {
@@ -330,7 +356,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      (P + int64_t(opp(i)) * a.dpop)[c0] = S::template to<Q>(f[i], i);
+      st_pop((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
@@ -338,8 +364,47 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
       if (i != 0 && ((lm >> i) & 1u)) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
       E* p = wall ? (P + int64_t(opp(i)) * a.dpop) + c0
                   : (P + int64_t(i) * a.dpop) + c0 + nb_off<Q, PER, true>(a, i, x, y, z);
-      *p = S::template to<Q>(v, i);
+      st_pop(p, S::template to<Q>(v, i));
     }
+  }
+}
+
+template <int Q, class S, int PAR, int FAM, bool PER>
+__global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
+  cell_update<Q, S, PAR, FAM, PER>(a, gw, lane);
+}
+
+// ---------------------------------------------------------------------------
+// The batched step: many blocks of one lattice and storage kind in one launch.
+// tab[e] holds block e's kernel arguments and wave0[e] the number of
+// wavefronts (64-cell row segments) of blocks 0..e-1; wave0[nb] is the total.
+// Wavefronts are taken wpb at a time ("groups"); workgroup b walks groups
+// xcd_block(b) + k * gridDim.x, so a grid smaller than the group count runs
+// persistently and a grid of one workgroup per group runs each once. Every
+// wavefront finds its block by binary search over wave0 (uniform, scalar).
+// ---------------------------------------------------------------------------
+template <int Q, class S, int PAR, int FAM, bool PER>
+__global__ __launch_bounds__(kMaxThreads) void k_step_batch(const KArgs* __restrict__ tab,
+                                                            const int64_t* __restrict__ wave0,
+                                                            int nb, int64_t groups, int wpb) {
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  if (wib >= wpb) return;
+  const int64_t total = wave0[nb];
+  for (int64_t g = xcd_block(blockIdx.x, gridDim.x); g < groups; g += gridDim.x) {
+    const int64_t w = g * wpb + wib;
+    if (w >= total) break;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wave0[mid] <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    cell_update<Q, S, PAR, FAM, PER>(tab[lo], int(w - wave0[lo]), lane);
   }
 }
 
@@ -802,6 +867,33 @@ hipError_t step_q(int family, const KArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int Q, class S, int PAR>
+hipError_t batch_q(int family, bool per, const KArgs* tab, const int64_t* wave0, int nb,
+                   int64_t groups, int wpb, int grid, hipStream_t s) {
+  const dim3 g(grid), b(kWave * wpb);
+  if (family == FAM_FULL) {
+    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, true>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, false>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+  } else {
+    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, true>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, false>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+  }
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_batch(int q, int parity, int family, bool per, const KArgs* tab,
+                        const int64_t* wave0, int nb, int64_t groups, int wpb, int grid,
+                        hipStream_t s) {
+  if (q == 19)
+    return parity ? batch_q<19, S, 1>(family, per, tab, wave0, nb, groups, wpb, grid, s)
+                  : batch_q<19, S, 0>(family, per, tab, wave0, nb, groups, wpb, grid, s);
+  if (q == 9)
+    return parity ? batch_q<9, S, 1>(family, per, tab, wave0, nb, groups, wpb, grid, s)
+                  : batch_q<9, S, 0>(family, per, tab, wave0, nb, groups, wpb, grid, s);
+  return hipErrorInvalidValue;
+}
+
 template <class S>
 hipError_t launch_step(int q, int parity, int family, const KArgs& a, hipStream_t s) {
   if (q == 19) return parity ? step_q<19, S, 1>(family, a, s) : step_q<19, S, 0>(family, a, s);
@@ -904,5 +996,11 @@ hipError_t launch_reduce(int q, int parity, const KArgs& a, uint8_t mask, double
   hipError_t launch_reduce_##SUFFIX(int q, int parity, const KArgs& a, uint8_t mask,          \
                                     double* partials, double* out, hipStream_t s) {           \
     return kern::launch_reduce<STORE>(q, parity, a, mask, partials, out, s);                  \
+  }                                                                                            \
+  hipError_t launch_batch_##SUFFIX(int q, int parity, int family, bool per, const KArgs* tab, \
+                                   const int64_t* wave0, int nb, int64_t groups, int wpb,     \
+                                   int grid, hipStream_t s) {                                 \
+    return kern::launch_batch<STORE>(q, parity, family, per, tab, wave0, nb, groups, wpb,     \
+                                     grid, s);                                                \
   }                                                                                            \
   }

@@ -201,3 +201,62 @@ class BlockLattice:
         out = np.zeros(4, dtype=np.float64)
         abi.check(abi.lib().flb_reduce_mass_momentum(self._h, mask & 0xFF, abi.ptr(out, C.c_double)))
         return out
+
+
+class BlockLatticeManager:
+    """Many BlockLattice blocks stepped together, one launch per time step
+    (FreeLB's BlockLatticeManager::ApplyCellDynamics / Stream over its blocks,
+    data_struct/block_lattice.hh:744-778). The blocks keep their own set-up
+    and read-out and must stay open while the manager lives."""
+
+    def __init__(self, blocks: Sequence[BlockLattice], stream=None):
+        L = abi.lib()
+        self.blocks = list(blocks)
+        arr = (C.c_void_p * len(self.blocks))(*[b.handle for b in self.blocks])
+        h = C.c_void_p()
+        abi.check(L.flb_batch_create(arr, len(self.blocks), stream, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.lib().flb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self):
+        n, cells, cfg = C.c_int(), C.c_int64(), abi.LaunchCfg()
+        abi.check(abi.lib().flb_batch_info(self._h, C.byref(n), C.byref(cells), C.byref(cfg)))
+        return {"blocks": n.value, "cells": cells.value,
+                "waves_per_group": cfg.waves_per_group, "grid": cfg.grid}
+
+    def set_launch(self, waves_per_group: int, grid: int = 0) -> None:
+        cfg = abi.LaunchCfg(int(waves_per_group), int(grid))
+        abi.check(abi.lib().flb_batch_set_launch(self._h, C.byref(cfg)))
+
+    def step(self, n: int = 1) -> None:
+        abi.check(abi.lib().flb_batch_step(self._h, int(n)))
+
+    def capture(self, n: int) -> None:
+        abi.check(abi.lib().flb_batch_capture(self._h, int(n)))
+
+    def replay(self) -> None:
+        abi.check(abi.lib().flb_batch_replay(self._h))
+
+    def synchronize(self) -> None:
+        abi.check(abi.lib().flb_batch_synchronize(self._h))
+
+    def autotune(self, steps: int = 10):
+        cfg, ms = abi.LaunchCfg(), C.c_double()
+        abi.check(abi.lib().flb_batch_autotune(self._h, int(steps), C.byref(cfg), C.byref(ms)))
+        return {"waves_per_group": cfg.waves_per_group, "grid": cfg.grid, "ms_per_step": ms.value}

@@ -182,6 +182,42 @@ int flb_moments(flb_block b, uint8_t mask, void* rhou);
 /* FP64 totals over cells with (flag & mask) != 0: out = {mass, Px, Py, Pz}. */
 int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]);
 
+/* ---- batches ----------------------------------------------------------------
+ * Many blocks of one lattice and storage kind stepped by one launch per time
+ * step: a device table of the blocks' kernel arguments, walked by a grid of
+ * workgroups that is either one workgroup per group of wavefronts or a
+ * persistent grid; steps can be recorded once as a hipGraph and replayed.
+ * Replaces the per-block launch loop of
+ *   BlockLatticeManager::ApplyCellDynamics / Stream  data_struct/block_lattice.hh:744-778
+ *   BlockLatticeManager::CuDevApplyCellDynamics       data_struct/block_lattice.hh:919-936
+ * The blocks stay usable on their own (set-up, read-out) and must outlive the
+ * batch; settings changed on a block after batch creation are picked up at
+ * the next batch call. All blocks must be at the same parity with no Stream
+ * pending. Work on the batch is ordered after earlier work on the blocks'
+ * streams and before later work on them. */
+typedef struct flb_batch_s* flb_batch;
+typedef struct {
+  int waves_per_group;  /* wavefronts per workgroup, 1..4 */
+  int grid;             /* workgroups; 0 = one per group of wavefronts (no persistence) */
+} flb_launch_cfg;
+int flb_batch_create(const flb_block* blocks, int n, void* stream, flb_batch* out);
+int flb_batch_destroy(flb_batch t);
+int flb_batch_info(flb_batch t, int* n_blocks, int64_t* n_cells, flb_launch_cfg* cfg);
+int flb_batch_set_launch(flb_batch t, const flb_launch_cfg* cfg);
+/* n steps of every block, one launch each (enqueued, not synchronised). */
+int flb_batch_step(flb_batch t, int n);
+/* Records n steps from the current parity as a hipGraph (nothing runs). */
+int flb_batch_capture(flb_batch t, int n);
+/* Launches the recorded graph: n more steps. The blocks must be at the
+ * parity of the capture (an even n replays any number of times). */
+int flb_batch_replay(flb_batch t);
+int flb_batch_synchronize(flb_batch t);
+/* Times `steps` (even) launches of each candidate shape (1, 2, 4 wavefronts
+ * per workgroup; one workgroup per group, or 1, 2, 4, 8 x 4 wavefronts per
+ * compute unit, persistent) and keeps the fastest. Populations are restored
+ * afterwards, so the blocks' state is unchanged. */
+int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* best_ms_per_step);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* Device copy bandwidth: a float4 copy kernel over `bytes` (rounded down to a
  * multiple of 16) between two fresh buffers, `reps` timed launches; *gbs gets

@@ -97,3 +97,17 @@ def test_host_mirror_shapes():
     w = weights(19)
     assert abs(w.sum() - 1.0) < 1e-15 and w.shape == (19,)
     assert abs(weights(9).sum() - 1.0) < 1e-15
+
+
+def test_batch_null_arguments():
+    L = abi.lib()
+    h = C.c_void_p()
+    assert L.flb_batch_create(None, 0, None, C.byref(h)) == abi.FLB_ERR_ARG
+    assert L.flb_batch_destroy(None) == abi.FLB_OK
+    for fn in ("flb_batch_replay", "flb_batch_synchronize"):
+        assert getattr(L, fn)(None) == abi.FLB_ERR_ARG
+    assert L.flb_batch_step(None, 1) == abi.FLB_ERR_ARG
+    assert L.flb_batch_capture(None, 2) == abi.FLB_ERR_ARG
+    assert L.flb_batch_set_launch(None, None) == abi.FLB_ERR_ARG
+    assert L.flb_batch_autotune(None, 2, None, None) == abi.FLB_ERR_ARG
+    assert "null batch" in L.flb_last_error().decode()
