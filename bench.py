@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="run the 256^3 x 2000-step FP32/FP16/BF16 vs FP64 error study instead")
     ap.add_argument("--precision-steps", type=int, default=2000)
+    ap.add_argument("--collisions", default="bgk,mrt_les", help="collisions of the precision study")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-block batched run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -274,7 +275,10 @@ def main():
         sys.exit(2)
 
     if args.precision:
-        print(json.dumps({"precision_study": precision_study(steps=args.precision_steps)}), flush=True)
+        st = tuple(s for s in args.storages.split(",") if s != "f64")
+        res = precision_study(steps=args.precision_steps, collisions=tuple(args.collisions.split(",")),
+                              storages=st)
+        print(json.dumps({"precision_study": res}), flush=True)
         return
     timer = HipTimer()
     runs = {}

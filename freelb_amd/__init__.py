@@ -7,6 +7,7 @@ kernels' sources (csrc/), the build recipe and a thin host-side mirror of
 FreeLB's BlockLattice interface.
 """
 from .abi import FLBError  # noqa: F401
+from .adjoint import Adjoint  # noqa: F401
 from .lattice import BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair, weights  # noqa: F401
 
-__all__ = ["BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]
+__all__ = ["Adjoint", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]

@@ -32,7 +32,10 @@ EXPORTS = (
     "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
     "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
     "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
-    "flb_batch_synchronize", "flb_batch_autotune", "flb_bandwidth_probe",
+    "flb_batch_synchronize", "flb_batch_autotune", "flb_adjoint_create", "flb_adjoint_destroy",
+    "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
+    "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
+    "flb_adjoint_timing", "flb_bandwidth_probe",
 )
 
 
@@ -62,6 +65,12 @@ class BlockInfo(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("waves_per_group", C.c_int), ("grid", C.c_int)]
+
+
+class AdjointDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("periodic", C.c_int * 3),
+                ("storage", C.c_int), ("omega", C.c_double), ("force", C.c_double * 3),
+                ("brinkman_alpha", C.c_double), ("outlet_x", C.c_int), ("stream", C.c_void_p)]
 
 
 class FLBError(RuntimeError):
@@ -119,6 +128,16 @@ def lib() -> C.CDLL:
         "flb_batch_replay": [P],
         "flb_batch_synchronize": [P],
         "flb_batch_autotune": [P, C.c_int, C.POINTER(LaunchCfg), C.POINTER(C.c_double)],
+        "flb_adjoint_create": [C.POINTER(AdjointDesc), C.POINTER(P)],
+        "flb_adjoint_destroy": [P],
+        "flb_adjoint_set_solid": [P, C.POINTER(C.c_uint8)],
+        "flb_adjoint_set_fraction": [P, C.POINTER(C.c_double)],
+        "flb_adjoint_set_fraction_at": [P, C.c_int64, C.c_double],
+        "flb_adjoint_init_equilibrium": [P, C.POINTER(C.c_double)],
+        "flb_adjoint_forward": [P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "flb_adjoint_backward": [P, C.POINTER(C.c_double)],
+        "flb_adjoint_state": [P, C.POINTER(C.c_double)],
+        "flb_adjoint_timing": [P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "flb_bandwidth_probe": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():

@@ -73,3 +73,37 @@ def cavity_block(n, storage: str = "f32", collision: str = "bgk", tau: float = 0
         lat.close()
         raise
     return lat
+
+
+def solid_fraction(dims, seed: int, sigma: float = 4.0) -> np.ndarray:
+    """Brinkman solid fraction s in (0, 1) per cell, FreeLB index order: a
+    seeded N(0,1) field (numpy default_rng(seed)), a periodic Gaussian filter
+    of width sigma cells (in Fourier space), rescaled to zero mean and unit
+    variance, then the logistic sigmoid."""
+    nx, ny, nz = (int(v) for v in dims)
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((nz, ny, nx))
+    kz, ky, kx = np.meshgrid(np.fft.fftfreq(nz), np.fft.fftfreq(ny), np.fft.fftfreq(nx), indexing="ij")
+    filt = np.exp(-2.0 * (np.pi * sigma) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    sm = np.real(np.fft.ifftn(np.fft.fftn(g) * filt))
+    sm = (sm - sm.mean()) / sm.std()
+    return (1.0 / (1.0 + np.exp(-sm))).reshape(-1)
+
+
+def channel_adjoint(dims=(192, 96, 96), storage: str = "f32", tau: float = 0.6, g: float = 1e-6,
+                    alpha: float = 0.1, seed: int = 7, sigma: float = 4.0):
+    """The differentiable channel: periodic in x, halfway walls beyond y and z,
+    body force g along x, Brinkman drag alpha*s with s from solid_fraction,
+    objective on the last x plane, at rest (rho = 1, u = 0) at t = 0."""
+    from .adjoint import Adjoint
+    adj = Adjoint(dims, omega=1.0 / tau, force=(g, 0.0, 0.0), brinkman_alpha=alpha,
+                  outlet_x=int(dims[0]) - 1, periodic=(True, False, False), storage=storage)
+    try:
+        N = int(np.prod(dims))
+        s = solid_fraction(dims, seed, sigma)
+        adj.set_fraction(s)
+        adj.init_equilibrium(np.ones(N), np.zeros((3, N)))
+    except Exception:
+        adj.close()
+        raise
+    return adj, s

@@ -1,0 +1,661 @@
+// Differentiable D3Q19 lattice update: forward BGK with a body force and a
+// Brinkman drag, the hand-written discrete adjoint of that step, and the
+// checkpoint / recompute driver, behind the C ABI (include/flb_abi.h).
+//
+// Model (one time step, every fluid cell x):
+//   rho = sum_i f_i,  j = sum_i c_i f_i,  D = 1 + alpha*s(x)/2
+//   u   = (j + G/2) / (rho*D)                 (FreeLB's forcerhoU, lbm/moment.h:401-438,
+//                                             with the implicit drag folded in)
+//   F   = G - alpha*s(x)*rho*u                 (body force density G, Brinkman drag)
+//   F_i = w_i [3 (c_i - u) + 9 (c_i . u) c_i] . F        (ForcePopImpl, lbm/force.h:29-36)
+//   f*_i = omega feq_i(rho, u) + (1 - omega) f_i + (1 - omega/2) F_i
+//                                             (BGKForce, lbm/collision.h:61-90)
+//   f_i(x + c_i, t+1) = f*_i(x), or f_{opp i}(x, t+1) = f*_i(x) when x + c_i is
+//   solid or outside a non-periodic axis (halfway bounce-back,
+//   boundary/bounce_back_boundary.h:169-181).
+// Objective: J = (1/T) sum_{t=1..T} sum_{x on plane x = outlet} sum_i c_ix f_i(x, t),
+// summed in FP64.
+//
+// Adjoint. With lambda^t the adjoint of f^t and mu^t = P^T lambda^{t+1} the
+// adjoint of the post-collision populations (mu_i(x) = lambda_i(x + c_i), or
+// lambda_{opp i}(x) across a wall link), one backward step is per cell
+//   lambda_j = dJ/df_j + (1 - omega) mu_j + A_rho + A_u . (c_j/(rho D) - u/rho)
+//   dJ/ds   += A_u . (-alpha u / (2 D)) - (1 - omega/2) alpha rho (B . u)
+// with the weighted moments of mu, M = sum w_i mu_i, V = sum w_i mu_i c_i,
+// T = sum w_i mu_i c_i c_i^T, and
+//   B     = 3 V - 3 M u + 9 T u          (= sum mu_i dF_i/dF = sum mu_i dfeq_i/du / rho)
+//   A_rho = omega [M (1 - 1.5 u.u) + 3 V.u + 4.5 u.T.u] - (1 - omega/2) alpha s (B . u)
+//   A_u   = omega rho B + (1 - omega/2) [-3 M F + 9 T F - alpha s rho B].
+// No reference implementation exists (FreeLB has no adjoint); the derivation
+// is pinned by FP64 finite differences and by autodiff of an FP64 CPU model.
+//
+// Layout: one dense array per direction, [19][N], N = nx*ny*nz,
+// id = x + nx*(y + ny*z) (FreeLB's index order); states are kept whole for
+// checkpoints, so no AA parity is involved.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/flb_abi.h"
+#include "flb_cell.hpp"
+#include "flb_host.hpp"
+
+namespace flb {
+namespace adj {
+
+constexpr int Q = 19;
+constexpr int kThreads = 256;
+
+template <class R>
+struct AdjArgs {
+  int nx, ny, nz;
+  int px, py, pz;
+  int64_t N;
+  const uint8_t* solid;  // nonzero: not updated, links into it bounce back
+  const R* s;            // Brinkman solid fraction per cell
+  R omega;
+  R G[3];
+  R alpha;
+  int outlet;            // x of the objective plane
+  R invT;                // 1 / T
+};
+
+// Neighbour id of (x, y, z) along c_i, or -1 for a wall link.
+template <class R>
+__device__ __forceinline__ int64_t neighbour(const AdjArgs<R>& a, int x, int y, int z, int i) {
+  int xn = x + Lat<Q>::cx(i), yn = y + Lat<Q>::cy(i), zn = z + Lat<Q>::cz(i);
+  if (xn < 0 || xn >= a.nx) {
+    if (!a.px) return -1;
+    xn = xn < 0 ? xn + a.nx : xn - a.nx;
+  }
+  if (yn < 0 || yn >= a.ny) {
+    if (!a.py) return -1;
+    yn = yn < 0 ? yn + a.ny : yn - a.ny;
+  }
+  if (zn < 0 || zn >= a.nz) {
+    if (!a.pz) return -1;
+    zn = zn < 0 ? zn + a.nz : zn - a.nz;
+  }
+  const int64_t id = int64_t(xn) + int64_t(a.nx) * (yn + int64_t(a.ny) * zn);
+  return a.solid[id] ? -1 : id;
+}
+
+// rho, u, F and D of a cell (the forward model above).
+template <class R>
+__device__ __forceinline__ void force_state(const R (&f)[Q], R s, const AdjArgs<R>& a, R& rho,
+                                            R (&u)[3], R (&F)[3], R& D) {
+  rho = R(0);
+  R j[3] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += f[i];
+    j[0] += R(Lat<Q>::cx(i)) * f[i];
+    j[1] += R(Lat<Q>::cy(i)) * f[i];
+    j[2] += R(Lat<Q>::cz(i)) * f[i];
+  }
+  D = R(1) + a.alpha * s * R(0.5);
+  const R inv = R(1) / (rho * D);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u[d] = (j[d] + a.G[d] * R(0.5)) * inv;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] = a.G[d] - a.alpha * s * rho * u[d];
+}
+
+template <class R>
+__device__ __forceinline__ R cdot(int i, const R (&v)[3]) {
+  return R(Lat<Q>::cx(i)) * v[0] + R(Lat<Q>::cy(i)) * v[1] + R(Lat<Q>::cz(i)) * v[2];
+}
+
+__device__ __forceinline__ void cell_xyz(int64_t id, int nx, int ny, int& x, int& y, int& z) {
+  const int64_t yz = id / nx;
+  x = int(id - yz * nx);
+  y = int(yz % ny);
+  z = int(yz / ny);
+}
+
+// ---- forward step: state src -> dst (push, halfway bounce-back) -------------
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
+                                                  R* __restrict__ dst) {
+  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (id >= a.N || a.solid[id]) return;
+  int x, y, z;
+  cell_xyz(id, a.nx, a.ny, x, y, z);
+  R f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = src[int64_t(i) * a.N + id];
+  R rho, u[3], F[3], D;
+  force_state(f, a.s[id], a, rho, u, F, D);
+  R e[Q];
+  feq(e, rho, u);
+  const R om = a.omega, _om = R(1) - a.omega, fom = R(1) - a.omega * R(0.5);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const R cu = cdot(i, u);
+    const R t0 = R(3) * (R(Lat<Q>::cx(i)) - u[0]) + R(9) * cu * R(Lat<Q>::cx(i));
+    const R t1 = R(3) * (R(Lat<Q>::cy(i)) - u[1]) + R(9) * cu * R(Lat<Q>::cy(i));
+    const R t2 = R(3) * (R(Lat<Q>::cz(i)) - u[2]) + R(9) * cu * R(Lat<Q>::cz(i));
+    const R Fi = Lat<Q>::template w<R>(i) * (F[0] * t0 + F[1] * t1 + F[2] * t2);
+    const R v = om * e[i] + _om * f[i] + fom * Fi;
+    const int64_t nb = i == 0 ? id : neighbour(a, x, y, z, i);
+    if (nb < 0) dst[int64_t(opp(i)) * a.N + id] = v;
+    else dst[int64_t(i) * a.N + nb] = v;
+  }
+}
+
+// ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restrict__ f,
+                                                  double* __restrict__ out) {
+  __shared__ double part[kThreads];
+  double acc = 0.0;
+  const int64_t plane = int64_t(a.ny) * a.nz;
+  for (int64_t k = threadIdx.x; k < plane; k += kThreads) {
+    const int64_t id = a.outlet + int64_t(a.nx) * k;
+    if (a.solid[id]) continue;
+    double m = 0.0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i)
+      if (Lat<Q>::cx(i) != 0) m += double(Lat<Q>::cx(i)) * double(f[int64_t(i) * a.N + id]);
+    acc += m;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// ---- equilibrium initial state ----------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_init(AdjArgs<R> a, const R* __restrict__ rhou,
+                                                   R* __restrict__ f) {
+  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (id >= a.N) return;
+  R e[Q];
+  const R u[3] = {rhou[a.N + id], rhou[2 * a.N + id], rhou[3 * a.N + id]};
+  feq(e, rhou[id], u);
+  const bool sol = a.solid[id] != 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[int64_t(i) * a.N + id] = sol ? R(0) : e[i];
+}
+
+// ---- adjoint seed: lambda^T = dJ/df^T -----------------------------------------
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_seed(AdjArgs<R> a, R* __restrict__ lam) {
+  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (id >= a.N) return;
+  const bool on = !a.solid[id] && int(id % a.nx) == a.outlet;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) lam[int64_t(i) * a.N + id] = on ? R(Lat<Q>::cx(i)) * a.invT : R(0);
+}
+
+// ---- one backward step: lambda^{t+1}, f^t -> lambda^t, grad += ds ----------------
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restrict__ fst,
+                                                  const R* __restrict__ lam1, R* __restrict__ lam0,
+                                                  R* __restrict__ grad, int with_obj) {
+  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (id >= a.N || a.solid[id]) return;
+  int x, y, z;
+  cell_xyz(id, a.nx, a.ny, x, y, z);
+  R f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = fst[int64_t(i) * a.N + id];
+  const R s = a.s[id];
+  R rho, u[3], F[3], D;
+  force_state(f, s, a, rho, u, F, D);
+  // mu = P^T lambda^{t+1}, kept in f's registers' place
+  R mu[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int64_t nb = i == 0 ? id : neighbour(a, x, y, z, i);
+    mu[i] = nb < 0 ? lam1[int64_t(opp(i)) * a.N + id] : lam1[int64_t(i) * a.N + nb];
+  }
+  R M = R(0), V[3] = {R(0), R(0), R(0)}, Txx = R(0), Tyy = R(0), Tzz = R(0), Txy = R(0), Txz = R(0),
+    Tyz = R(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const R wm = Lat<Q>::template w<R>(i) * mu[i];
+    const R cx = R(Lat<Q>::cx(i)), cy = R(Lat<Q>::cy(i)), cz = R(Lat<Q>::cz(i));
+    M += wm;
+    V[0] += wm * cx;
+    V[1] += wm * cy;
+    V[2] += wm * cz;
+    Txx += wm * cx * cx;
+    Tyy += wm * cy * cy;
+    Tzz += wm * cz * cz;
+    Txy += wm * cx * cy;
+    Txz += wm * cx * cz;
+    Tyz += wm * cy * cz;
+  }
+  const R Tu[3] = {Txx * u[0] + Txy * u[1] + Txz * u[2], Txy * u[0] + Tyy * u[1] + Tyz * u[2],
+                   Txz * u[0] + Tyz * u[1] + Tzz * u[2]};
+  const R TF[3] = {Txx * F[0] + Txy * F[1] + Txz * F[2], Txy * F[0] + Tyy * F[1] + Tyz * F[2],
+                   Txz * F[0] + Tyz * F[1] + Tzz * F[2]};
+  R B[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) B[d] = R(3) * V[d] - R(3) * M * u[d] + R(9) * Tu[d];
+  const R om = a.omega, fom = R(1) - a.omega * R(0.5);
+  const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const R Vu = V[0] * u[0] + V[1] * u[1] + V[2] * u[2];
+  const R uTu = u[0] * Tu[0] + u[1] * Tu[1] + u[2] * Tu[2];
+  const R Bu = B[0] * u[0] + B[1] * u[1] + B[2] * u[2];
+  const R as = a.alpha * s;
+  const R Arho = om * (M * (R(1) - R(1.5) * uu) + R(3) * Vu + R(4.5) * uTu) - fom * as * Bu;
+  R Au[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    Au[d] = om * rho * B[d] + fom * (R(-3) * M * F[d] + R(9) * TF[d] - as * rho * B[d]);
+  const R Auu = Au[0] * u[0] + Au[1] * u[1] + Au[2] * u[2];
+  grad[id] += -Auu * a.alpha / (R(2) * D) - fom * a.alpha * rho * Bu;
+  const R irD = R(1) / (rho * D);
+  const R base = Arho - Auu / rho;
+  const bool obj = with_obj && x == a.outlet;
+  const R _om = R(1) - om;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    R l = _om * mu[j] + base + (R(Lat<Q>::cx(j)) * Au[0] + R(Lat<Q>::cy(j)) * Au[1] +
+                                R(Lat<Q>::cz(j)) * Au[2]) * irD;
+    if (obj) l += R(Lat<Q>::cx(j)) * a.invT;
+    lam0[int64_t(j) * a.N + id] = l;
+  }
+}
+
+}  // namespace adj
+}  // namespace flb
+
+using namespace flbh;
+namespace A = flb::adj;
+
+struct flb_adjoint_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  flb_adjoint_desc desc{};
+  int64_t N = 0;
+  size_t real = 4;
+  uint8_t* d_solid = nullptr;
+  void* d_s = nullptr;
+  void* d_f0 = nullptr;
+  void* d_work[2] = {nullptr, nullptr};
+  void* d_lam[2] = {nullptr, nullptr};
+  void* d_grad = nullptr;
+  double* d_jt = nullptr;
+  int jt_cap = 0;
+  std::vector<void*> ckpt;  // ckpt[k] = state at step k*K (ckpt[0] aliases d_f0)
+  std::vector<void*> tape;  // recompute slots, or every state when K == 0
+  int T = 0;
+  int K = -1;
+  bool recorded = false;
+  void* last = nullptr;  // buffer holding the final forward state
+  double J = 0.0;
+  float fwd_ms = 0.0f, bwd_ms = 0.0f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+size_t state_bytes(const flb_adjoint_s* h) { return size_t(A::Q) * size_t(h->N) * h->real; }
+
+template <class R>
+A::AdjArgs<R> adj_args(const flb_adjoint_s* h) {
+  A::AdjArgs<R> a{};
+  a.nx = h->desc.nx;
+  a.ny = h->desc.ny;
+  a.nz = h->desc.nz;
+  a.px = h->desc.periodic[0] != 0;
+  a.py = h->desc.periodic[1] != 0;
+  a.pz = h->desc.periodic[2] != 0;
+  a.N = h->N;
+  a.solid = h->d_solid;
+  a.s = static_cast<const R*>(h->d_s);
+  a.omega = R(h->desc.omega);
+  for (int d = 0; d < 3; ++d) a.G[d] = R(h->desc.force[d]);
+  a.alpha = R(h->desc.brinkman_alpha);
+  a.outlet = h->desc.outlet_x;
+  a.invT = h->T > 0 ? R(1) / R(h->T) : R(0);
+  return a;
+}
+
+unsigned grid_of(const flb_adjoint_s* h) { return unsigned((h->N + A::kThreads - 1) / A::kThreads); }
+
+template <class R>
+hipError_t fwd_step(const flb_adjoint_s* h, const void* src, void* dst) {
+  hipLaunchKernelGGL(A::k_fwd<R>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<const R*>(src), static_cast<R*>(dst));
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t obj_of(const flb_adjoint_s* h, const void* f, double* out) {
+  hipLaunchKernelGGL(A::k_obj<R>, dim3(1), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<const R*>(f), out);
+  return hipGetLastError();
+}
+
+int alloc_state(flb_adjoint_s* h, void** p) {
+  if (*p) return FLB_OK;
+  if (hipMalloc(p, state_bytes(h)) != hipSuccess)
+    return fail(FLB_ERR_NOMEM, "adjoint state allocation of %zu bytes failed", state_bytes(h));
+  // solid cells are never written by the step kernels: keep them zero
+  HIP_TRY(hipMemsetAsync(*p, 0, state_bytes(h), h->stream));
+  return FLB_OK;
+}
+
+void free_vec(std::vector<void*>& v, const void* keep) {
+  for (void* p : v)
+    if (p && p != keep) (void)hipFree(p);
+  v.clear();
+}
+
+template <class R>
+int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
+  h->T = T;
+  h->K = K;
+  h->recorded = false;
+  // buffers
+  int rc = FLB_OK;
+  for (void*& w : h->d_work)
+    if ((rc = alloc_state(h, &w))) return rc;
+  const int nck = K > 0 ? (T + K - 1) / K : 0;
+  if (K > 0) {
+    // keep existing checkpoint buffers when the count matches
+    if (int(h->ckpt.size()) != nck) {
+      free_vec(h->ckpt, h->d_f0);
+      h->ckpt.assign(size_t(nck), nullptr);
+      h->ckpt[0] = h->d_f0;
+    }
+    for (int k = 1; k < nck; ++k)
+      if ((rc = alloc_state(h, &h->ckpt[size_t(k)]))) return rc;
+    const int ntape = std::min(K, T) - 1;
+    if (int(h->tape.size()) != ntape + 1) {
+      free_vec(h->tape, h->d_f0);
+      h->tape.assign(size_t(ntape + 1), nullptr);
+    }
+    for (int k = 1; k <= ntape; ++k)
+      if ((rc = alloc_state(h, &h->tape[size_t(k)]))) return rc;
+  } else if (K == 0) {
+    if (int(h->tape.size()) != T + 1) {
+      free_vec(h->tape, h->d_f0);
+      h->tape.assign(size_t(T + 1), nullptr);
+    }
+    h->tape[0] = h->d_f0;
+    for (int k = 1; k <= T; ++k)
+      if ((rc = alloc_state(h, &h->tape[size_t(k)]))) return rc;
+  }
+  if (h->jt_cap < T + 1) {
+    if (h->d_jt) (void)hipFree(h->d_jt);
+    h->d_jt = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&h->d_jt), size_t(T + 1) * sizeof(double)) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "objective buffer allocation failed");
+    h->jt_cap = T + 1;
+  }
+  HIP_TRY(hipEventRecord(h->e0, h->stream));
+  const void* src = h->d_f0;
+  for (int t = 0; t < T; ++t) {
+    void* dst;
+    if (K == 0) dst = h->tape[size_t(t + 1)];
+    else if (K > 0 && (t + 1) % K == 0 && t + 1 < T) dst = h->ckpt[size_t((t + 1) / K)];
+    else dst = (src == h->d_work[0]) ? h->d_work[1] : h->d_work[0];
+    HIP_TRY(fwd_step<R>(h, src, dst));
+    HIP_TRY(obj_of<R>(h, dst, h->d_jt + t + 1));
+    src = dst;
+  }
+  HIP_TRY(hipEventRecord(h->e1, h->stream));
+  std::vector<double> jt(size_t(T + 1), 0.0);
+  if (T > 0)
+    HIP_TRY(hipMemcpyAsync(jt.data() + 1, h->d_jt + 1, size_t(T) * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipEventElapsedTime(&h->fwd_ms, h->e0, h->e1));
+  double s = 0.0;
+  for (int t = 1; t <= T; ++t) s += jt[size_t(t)];
+  h->J = T > 0 ? s / double(T) : 0.0;
+  h->last = const_cast<void*>(src);
+  h->recorded = K >= 0;
+  if (J) *J = h->J;
+  return FLB_OK;
+}
+
+template <class R>
+hipError_t bwd_step(const flb_adjoint_s* h, const void* fst, const void* lam1, void* lam0, int with_obj) {
+  hipLaunchKernelGGL(A::k_bwd<R>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<const R*>(fst), static_cast<const R*>(lam1), static_cast<R*>(lam0),
+                     static_cast<R*>(h->d_grad), with_obj);
+  return hipGetLastError();
+}
+
+template <class R>
+int backward_impl(flb_adjoint_s* h, double* grad) {
+  const int T = h->T, K = h->K;
+  int rc = FLB_OK;
+  for (void*& l : h->d_lam)
+    if ((rc = alloc_state(h, &l))) return rc;
+  if (!h->d_grad) {
+    if (hipMalloc(&h->d_grad, size_t(h->N) * h->real) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "gradient allocation failed");
+  }
+  HIP_TRY(hipEventRecord(h->e0, h->stream));
+  HIP_TRY(hipMemsetAsync(h->d_grad, 0, size_t(h->N) * h->real, h->stream));
+  hipLaunchKernelGGL(A::k_seed<R>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<R*>(h->d_lam[0]));
+  HIP_TRY(hipGetLastError());
+  int cur = 0;
+  if (K == 0) {
+    for (int t = T - 1; t >= 0; --t) {
+      HIP_TRY(bwd_step<R>(h, h->tape[size_t(t)], h->d_lam[cur], h->d_lam[cur ^ 1], t >= 1));
+      cur ^= 1;
+    }
+  } else {
+    const int nseg = (T + K - 1) / K;
+    for (int k = nseg - 1; k >= 0; --k) {
+      const int t0 = k * K, t1 = std::min((k + 1) * K, T);
+      // states t0 .. t1-1 of this segment: tape[0] is the checkpoint itself
+      h->tape[0] = h->ckpt[size_t(k)];
+      for (int t = t0; t < t1 - 1; ++t)
+        HIP_TRY(fwd_step<R>(h, h->tape[size_t(t - t0)], h->tape[size_t(t - t0 + 1)]));
+      for (int t = t1 - 1; t >= t0; --t) {
+        HIP_TRY(bwd_step<R>(h, h->tape[size_t(t - t0)], h->d_lam[cur], h->d_lam[cur ^ 1], t >= 1));
+        cur ^= 1;
+      }
+    }
+    h->tape[0] = nullptr;
+  }
+  HIP_TRY(hipEventRecord(h->e1, h->stream));
+  std::vector<R> g(size_t(h->N));
+  HIP_TRY(hipMemcpyAsync(g.data(), h->d_grad, size_t(h->N) * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipEventElapsedTime(&h->bwd_ms, h->e0, h->e1));
+  for (int64_t i = 0; i < h->N; ++i) grad[i] = double(g[size_t(i)]);
+  return FLB_OK;
+}
+
+#define CHECK_ADJ(h) \
+  if (!(h)) return fail(FLB_ERR_ARG, "null adjoint handle")
+
+}  // namespace
+
+extern "C" {
+
+int flb_adjoint_create(const flb_adjoint_desc* desc, flb_adjoint* out) {
+  if (!desc || !out) return fail(FLB_ERR_ARG, "null descriptor or output");
+  *out = nullptr;
+  if (desc->storage != FLB_STORE_F32 && desc->storage != FLB_STORE_F64)
+    return fail(FLB_ERR_UNSUPPORTED, "the adjoint runs in FP32 or FP64 storage, got kind %d", desc->storage);
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1)
+    return fail(FLB_ERR_ARG, "extents must be positive (%d,%d,%d)", desc->nx, desc->ny, desc->nz);
+  if (!(desc->omega > 0.0 && desc->omega < 2.0)) return fail(FLB_ERR_ARG, "omega must lie in (0, 2)");
+  if (!(desc->brinkman_alpha >= 0.0)) return fail(FLB_ERR_ARG, "Brinkman coefficient must be >= 0");
+  if (desc->outlet_x < 0 || desc->outlet_x >= desc->nx) return fail(FLB_ERR_ARG, "outlet plane outside the box");
+  const int64_t N = int64_t(desc->nx) * desc->ny * desc->nz;
+  if (N * 19 >= (int64_t(1) << 40)) return fail(FLB_ERR_ARG, "box too large");
+  flb_adjoint_s* h = new flb_adjoint_s();
+  h->desc = *desc;
+  h->N = N;
+  h->real = desc->storage == FLB_STORE_F64 ? 8 : 4;
+  h->stream = static_cast<hipStream_t>(desc->stream);
+  int rc = FLB_OK;
+  if (hipGetDevice(&h->device) != hipSuccess) rc = fail(FLB_ERR_HIP, "no HIP device");
+  if (rc == FLB_OK && (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess))
+    rc = fail(FLB_ERR_HIP, "event creation failed");
+  if (rc == FLB_OK && (hipMalloc(reinterpret_cast<void**>(&h->d_solid), size_t(N)) != hipSuccess ||
+                       hipMalloc(&h->d_s, size_t(N) * h->real) != hipSuccess))
+    rc = fail(FLB_ERR_NOMEM, "adjoint field allocation failed");
+  if (rc == FLB_OK) rc = alloc_state(h, &h->d_f0);
+  if (rc == FLB_OK && (hipMemsetAsync(h->d_solid, 0, size_t(N), h->stream) != hipSuccess ||
+                       hipMemsetAsync(h->d_s, 0, size_t(N) * h->real, h->stream) != hipSuccess ||
+                       hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = fail(FLB_ERR_HIP, "initial clear failed");
+  if (rc != FLB_OK) {
+    flb_adjoint_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return FLB_OK;
+}
+
+int flb_adjoint_destroy(flb_adjoint h) {
+  if (!h) return FLB_OK;
+  DeviceGuard g(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  free_vec(h->ckpt, h->d_f0);
+  free_vec(h->tape, h->d_f0);
+  void* ptrs[] = {h->d_solid, h->d_s, h->d_f0, h->d_work[0], h->d_work[1], h->d_lam[0], h->d_lam[1],
+                  h->d_grad, h->d_jt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  delete h;
+  return FLB_OK;
+}
+
+int flb_adjoint_set_solid(flb_adjoint h, const uint8_t* solid) {
+  CHECK_ADJ(h);
+  if (!solid) return fail(FLB_ERR_ARG, "null solid mask");
+  DeviceGuard g(h->device);
+  std::vector<uint8_t> m(size_t(h->N));
+  for (int64_t i = 0; i < h->N; ++i) m[size_t(i)] = solid[i] ? 1 : 0;
+  HIP_TRY(hipMemcpyAsync(h->d_solid, m.data(), size_t(h->N), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->recorded = false;
+  return FLB_OK;
+}
+
+int flb_adjoint_set_fraction(flb_adjoint h, const double* s) {
+  CHECK_ADJ(h);
+  if (!s) return fail(FLB_ERR_ARG, "null solid fraction");
+  DeviceGuard g(h->device);
+  if (h->real == 8) {
+    HIP_TRY(hipMemcpyAsync(h->d_s, s, size_t(h->N) * 8, hipMemcpyHostToDevice, h->stream));
+  } else {
+    std::vector<float> t(size_t(h->N));
+    for (int64_t i = 0; i < h->N; ++i) t[size_t(i)] = float(s[i]);
+    HIP_TRY(hipMemcpyAsync(h->d_s, t.data(), size_t(h->N) * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->recorded = false;
+  return FLB_OK;
+}
+
+int flb_adjoint_set_fraction_at(flb_adjoint h, int64_t cell, double s) {
+  CHECK_ADJ(h);
+  if (cell < 0 || cell >= h->N) return fail(FLB_ERR_ARG, "cell %lld outside the box", (long long)cell);
+  DeviceGuard g(h->device);
+  char* dst = static_cast<char*>(h->d_s) + size_t(cell) * h->real;
+  if (h->real == 8) {
+    HIP_TRY(hipMemcpyAsync(dst, &s, 8, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  } else {
+    const float v = float(s);
+    HIP_TRY(hipMemcpyAsync(dst, &v, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  h->recorded = false;
+  return FLB_OK;
+}
+
+int flb_adjoint_init_equilibrium(flb_adjoint h, const double* rhou) {
+  CHECK_ADJ(h);
+  if (!rhou) return fail(FLB_ERR_ARG, "null rhou");
+  DeviceGuard g(h->device);
+  const size_t count = size_t(4) * size_t(h->N);
+  void* stage = nullptr;
+  if (hipMalloc(&stage, count * h->real) != hipSuccess) return fail(FLB_ERR_NOMEM, "staging allocation failed");
+  hipError_t e;
+  if (h->real == 8) {
+    e = hipMemcpyAsync(stage, rhou, count * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(A::k_init<double>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream,
+                         adj_args<double>(h), static_cast<const double*>(stage), static_cast<double*>(h->d_f0));
+      e = hipGetLastError();
+    }
+  } else {
+    std::vector<float> t(count);
+    for (size_t i = 0; i < count; ++i) t[i] = float(rhou[i]);
+    e = hipMemcpyAsync(stage, t.data(), count * 4, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(A::k_init<float>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream,
+                         adj_args<float>(h), static_cast<const float*>(stage), static_cast<float*>(h->d_f0));
+      e = hipGetLastError();
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return fail(FLB_ERR_HIP, "equilibrium initialisation failed: %s", hipGetErrorString(e));
+  h->recorded = false;
+  return FLB_OK;
+}
+
+int flb_adjoint_forward(flb_adjoint h, int steps, int checkpoint_every, double* J) {
+  CHECK_ADJ(h);
+  if (steps < 1) return fail(FLB_ERR_ARG, "forward needs at least one step");
+  if (checkpoint_every < -1) return fail(FLB_ERR_ARG, "checkpoint_every must be -1, 0 or positive");
+  DeviceGuard g(h->device);
+  return h->real == 8 ? forward_impl<double>(h, steps, checkpoint_every, J)
+                      : forward_impl<float>(h, steps, checkpoint_every, J);
+}
+
+int flb_adjoint_backward(flb_adjoint h, double* grad) {
+  CHECK_ADJ(h);
+  if (!grad) return fail(FLB_ERR_ARG, "null gradient output");
+  if (!h->recorded) return fail(FLB_ERR_STATE, "no recorded forward run (forward with checkpoint_every >= 0 first)");
+  DeviceGuard g(h->device);
+  return h->real == 8 ? backward_impl<double>(h, grad) : backward_impl<float>(h, grad);
+}
+
+int flb_adjoint_state(flb_adjoint h, double* pops) {
+  CHECK_ADJ(h);
+  if (!pops) return fail(FLB_ERR_ARG, "null output");
+  if (!h->last) return fail(FLB_ERR_STATE, "no forward run yet");
+  DeviceGuard g(h->device);
+  const size_t n = size_t(A::Q) * size_t(h->N);
+  if (h->real == 8) {
+    HIP_TRY(hipMemcpyAsync(pops, h->last, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> t(n);
+    HIP_TRY(hipMemcpyAsync(t.data(), h->last, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; ++i) pops[i] = double(t[i]);
+  }
+  return FLB_OK;
+}
+
+int flb_adjoint_timing(flb_adjoint h, double* forward_ms, double* backward_ms) {
+  CHECK_ADJ(h);
+  if (forward_ms) *forward_ms = h->fwd_ms;
+  if (backward_ms) *backward_ms = h->bwd_ms;
+  return FLB_OK;
+}
+
+}  // extern "C"

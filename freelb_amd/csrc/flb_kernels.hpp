@@ -276,35 +276,27 @@ __device__ __forceinline__ void store_row(typename S::E* p, int i, const typenam
 }
 
 // Population loads and stores of the step kernels. Each population is read
-// and written once per step, so the non-temporal forms (FLB_NT_LOAD /
-// FLB_NT_STORE) can keep them from displacing the neighbour lines that the
-// odd step reuses from L2.
-#ifndef FLB_NT_LOAD
-#define FLB_NT_LOAD 0
+// and written once per step; at the even step no other wavefront touches a
+// cell's lines, so non-temporal accesses (FLB_NT_EVEN) keep them from
+// displacing lines in the caches. The odd step re-reads neighbour lines from
+// L2 and keeps its loads cached (FLB_NT_ODD_STORE marks only its stores).
+#ifndef FLB_NT_EVEN
+#define FLB_NT_EVEN 0
 #endif
-#ifndef FLB_NT_STORE
-#define FLB_NT_STORE 0
+#ifndef FLB_NT_ODD_STORE
+#define FLB_NT_ODD_STORE 0
 #endif
-template <class E>
+template <int PAR, class E>
 __device__ __forceinline__ E ld_pop(const E* p) {
-#if FLB_NT_LOAD
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
+  if constexpr (PAR == 0 && FLB_NT_EVEN) return __builtin_nontemporal_load(p);
+  else return *p;
 }
-template <class E>
+template <int PAR, class E>
 __device__ __forceinline__ void st_pop(E* p, E v) {
-#if FLB_NT_STORE
-  __builtin_nontemporal_store(v, p);
-#else
-  *p = v;
-#endif
+  if constexpr ((PAR == 0 && FLB_NT_EVEN) || (PAR == 1 && FLB_NT_ODD_STORE)) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 
-// ---------------------------------------------------------------------------
-// The step kernel, cell form: one cell per thread, any axis periodic.
-// ---------------------------------------------------------------------------
 // Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
 // up to 64 cells; lane is the lane id.
 template <int Q, class S, int PAR, int FAM, bool PER>
@@ -332,14 +324,14 @@ __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      f[i] = S::template from<Q>(ld_pop((P + int64_t(i) * a.dpop) + c0), i);
+      f[i] = S::template from<Q>(ld_pop<PAR>((P + int64_t(i) * a.dpop) + c0), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
       const int o = opp(i);
       const bool wall = i != 0 && ((lk >> o) & 1u);
       const E* p = wall ? (P + int64_t(i) * a.dpop) + c0
                         : (P + int64_t(o) * a.dpop) + c0 + nb_off<Q, PER, true>(a, o, x, y, z);
-      f[i] = S::template from<Q>(ld_pop(p), i);
+      f[i] = S::template from<Q>(ld_pop<PAR>(p), i);
     }
   }
   if (PAR == 1 && lm) {
@@ -356,7 +348,7 @@ __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      st_pop((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
+      st_pop<PAR>((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
@@ -364,7 +356,7 @@ __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
       if (i != 0 && ((lm >> i) & 1u)) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
       E* p = wall ? (P + int64_t(opp(i)) * a.dpop) + c0
                   : (P + int64_t(i) * a.dpop) + c0 + nb_off<Q, PER, true>(a, i, x, y, z);
-      st_pop(p, S::template to<Q>(v, i));
+      st_pop<PAR>(p, S::template to<Q>(v, i));
     }
   }
 }

@@ -48,10 +48,15 @@ struct StoreF64 {
   }
 };
 
+#ifndef FLB_F16_VARIANT
+#define FLB_F16_VARIANT 0
+#endif
+
 struct StoreF16S {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 1;
+#if FLB_F16_VARIANT == 0
   template <int Q>
   __device__ __forceinline__ static R from(E e, int i) {
     return __half2float(__ushort_as_half(e)) + Lat<Q>::template w<float>(i);
@@ -60,6 +65,43 @@ struct StoreF16S {
   __device__ __forceinline__ static E to(R v, int i) {
     return __half_as_ushort(__float2half_rn(v - Lat<Q>::template w<float>(i)));
   }
+#elif FLB_F16_VARIANT == 1 || FLB_F16_VARIANT == 3
+  // IEEE half of (f - w) * 2^14: the shifted values stay in the normal range
+  template <int Q>
+  __device__ __forceinline__ static R from(E e, int i) {
+    return __half2float(__ushort_as_half(e)) * 6.103515625e-05f + Lat<Q>::template w<float>(i);
+  }
+  template <int Q>
+  __device__ __forceinline__ static E to(R v, int i) {
+    return __half_as_ushort(__float2half_rn((v - Lat<Q>::template w<float>(i)) * 16384.0f));
+  }
+#elif FLB_F16_VARIANT == 2
+  // 1 sign, 4 exponent, 11 fraction bits: |x| = (1 + m/2^11) 2^(E-18), E = 1..15,
+  // m 2^-28 for E = 0; largest 0.2499, round to nearest even, saturating
+  template <int Q>
+  __device__ __forceinline__ static R from(E e, int i) {
+    const uint32_t m = uint32_t(e) & 0x7FFFu;
+    float a = (m >> 11) == 0u ? float(m) * 3.7252902984619140625e-09f
+                              : __uint_as_float((m + (109u << 11)) << 12);
+    return ((e & 0x8000u) ? -a : a) + Lat<Q>::template w<float>(i);
+  }
+  template <int Q>
+  __device__ __forceinline__ static E to(R v, int i) {
+    const float x = v - Lat<Q>::template w<float>(i);
+    const uint32_t u = __float_as_uint(x);
+    const uint32_t sgn = (u >> 16) & 0x8000u;
+    const uint32_t a = u & 0x7FFFFFFFu;
+    uint32_t code;
+    if (a < (110u << 23)) {  // below 2^-17
+      code = uint32_t(__float2uint_rn(__uint_as_float(a) * 268435456.0f));
+    } else {
+      const uint32_t r = (a + 0x7FFu + ((a >> 12) & 1u)) >> 12;
+      code = r - (109u << 11);
+      if (code > 0x7FFFu) code = 0x7FFFu;
+    }
+    return E(sgn | code);
+  }
+#endif
 };
 
 struct StoreBF16S {

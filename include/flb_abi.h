@@ -218,6 +218,48 @@ int flb_batch_synchronize(flb_batch t);
  * afterwards, so the blocks' state is unchanged. */
 int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* best_ms_per_step);
 
+/* ---- differentiation -------------------------------------------------------
+ * A D3Q19 box (nx*ny*nz cells, FreeLB index order, any axis periodic; cells
+ * outside a non-periodic axis and cells marked solid are halfway
+ * bounce-back walls) stepped by BGK with a constant body-force density G
+ * (BGKForce with ConstForce: lbm/collision.h:61-90, lbm/force.h:29-36,
+ * lbm/moment.h:401-438) and a Brinkman drag -alpha*s(x)*rho*u per cell,
+ * s in [0, 1], taken implicitly in the force-corrected velocity
+ * u = (sum c f + G/2) / (rho (1 + alpha s / 2)). The objective is
+ * J = (1/T) sum_{t=1..T} sum over the plane x = outlet_x of sum_i c_ix f_i,
+ * summed in FP64. flb_adjoint_backward returns dJ/ds from the hand-written
+ * discrete adjoint of the step, recomputing forward states from
+ * checkpoints. No reference counterpart (FreeLB has no adjoint). */
+typedef struct flb_adjoint_s* flb_adjoint;
+typedef struct {
+  int nx, ny, nz;
+  int periodic[3];
+  int storage;            /* FLB_STORE_F32 or FLB_STORE_F64 */
+  double omega;
+  double force[3];        /* body-force density G */
+  double brinkman_alpha;  /* drag coefficient at s = 1 */
+  int outlet_x;           /* plane of the objective */
+  void* stream;           /* hipStream_t; NULL = the null stream */
+} flb_adjoint_desc;
+int flb_adjoint_create(const flb_adjoint_desc* desc, flb_adjoint* out);
+int flb_adjoint_destroy(flb_adjoint h);
+int flb_adjoint_set_solid(flb_adjoint h, const uint8_t* solid);       /* N bytes, nonzero = solid */
+int flb_adjoint_set_fraction(flb_adjoint h, const double* s);         /* N values */
+int flb_adjoint_set_fraction_at(flb_adjoint h, int64_t cell, double s);
+/* f^0 = feq(rho, u) from host rhou[4][N]. */
+int flb_adjoint_init_equilibrium(flb_adjoint h, const double* rhou);
+/* T steps from f^0; *J gets the objective. checkpoint_every: k > 0 keeps
+ * every k-th state and recomputes the others in the backward pass (k near
+ * sqrt(T) balances memory and recompute), 0 keeps every state, -1 records
+ * nothing (objective only, e.g. for finite differences). */
+int flb_adjoint_forward(flb_adjoint h, int steps, int checkpoint_every, double* J);
+/* dJ/ds for every cell (0 on solid cells) after a recorded forward run. */
+int flb_adjoint_backward(flb_adjoint h, double* grad);
+/* Final forward state f^T, [19][N]. */
+int flb_adjoint_state(flb_adjoint h, double* pops);
+/* Device time of the last forward and backward pass (ms, HIP events). */
+int flb_adjoint_timing(flb_adjoint h, double* forward_ms, double* backward_ms);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* Device copy bandwidth: a float4 copy kernel over `bytes` (rounded down to a
  * multiple of 16) between two fresh buffers, `reps` timed launches; *gbs gets

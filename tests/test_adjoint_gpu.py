@@ -1,0 +1,135 @@
+"""The hand-written GPU adjoint (through the C ABI) against autodiff of the
+FP64 CPU model and against central finite differences in FP64; checkpointed
+recompute against keeping every state; FP32 against FP64; and the channel of
+192 x 96 x 96 cells, T = 800, checkpoints every 28 steps, against FP64 finite
+differences (h = 1e-4) at 32 seeded cells."""
+import numpy as np
+import pytest
+import torch
+
+from adjoint_model import Model
+from adjoint_torch import AdjointObjective
+from freelb_amd import Adjoint
+from freelb_amd.cases import channel_adjoint
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (10, 6, 5)
+TAU, G, ALPHA = 0.7, (1e-3, 2e-4, 0.0), 0.3
+PER = (True, False, False)
+
+
+def _setup(storage="f64"):
+    N = int(np.prod(DIMS))
+    rng = np.random.default_rng(21)
+    solid = np.zeros(N, dtype=bool)
+    solid[[33, 34, 140, 201]] = True
+    rho = 1 + 0.01 * rng.standard_normal(N)
+    u = 0.02 * rng.standard_normal((3, N))
+    s = rng.uniform(0, 1, N)
+    adj = Adjoint(DIMS, omega=1 / TAU, force=G, brinkman_alpha=ALPHA, periodic=PER, storage=storage)
+    adj.set_solid(solid)
+    adj.set_fraction(s)
+    adj.init_equilibrium(rho, u)
+    model = Model(DIMS, omega=1 / TAU, force=G, alpha=ALPHA, periodic=PER, solid=solid)
+    f0 = model.equilibrium(rho, u)
+    return adj, model, f0, s, solid
+
+
+def test_forward_matches_model_fp64():
+    adj, model, f0, s, solid = _setup()
+    with adj:
+        J = adj.forward(15, -1)
+        Jm, fm = model.run(f0, torch.tensor(s), 15)
+        assert abs(J - Jm.item()) <= 1e-12 * abs(Jm.item())
+        np.testing.assert_allclose(adj.state(), fm.numpy(), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("K", [0, 1, 4, 15, 100])
+def test_gradient_matches_autodiff_fp64(K):
+    adj, model, f0, s, solid = _setup()
+    with adj:
+        adj.forward(15, K)
+        g = adj.backward()
+    st = torch.tensor(s, requires_grad=True)
+    Jm, _ = model.run(f0, st, 15)
+    (gm,) = torch.autograd.grad(Jm, st)
+    gm = gm.numpy()
+    assert np.all(g[solid] == 0)
+    np.testing.assert_allclose(g, gm, rtol=0, atol=1e-9 * np.abs(gm).max())
+
+
+def test_checkpoint_modes_agree_bitwise():
+    adj, *_ = _setup()
+    with adj:
+        adj.forward(23, 0)
+        g0 = adj.backward()
+        adj.forward(23, 5)
+        g5 = adj.backward()
+        adj.forward(23, 7)
+        g7 = adj.backward()
+    assert np.array_equal(g0, g5) and np.array_equal(g0, g7)
+
+
+def test_gradient_against_fd_fp64():
+    adj, model, f0, s, solid = _setup()
+    with adj:
+        adj.forward(15, 4)
+        g = adj.backward()
+        h = 1e-4
+        for cell in (0, 57, 123, 299):
+            adj.set_fraction_at(cell, s[cell] + h)
+            jp = adj.forward(15, -1)
+            adj.set_fraction_at(cell, s[cell] - h)
+            jm = adj.forward(15, -1)
+            adj.set_fraction_at(cell, s[cell])
+            fd = (jp - jm) / (2 * h)
+            assert abs(g[cell] - fd) <= 1e-6 * abs(fd) + 1e-14, (cell, g[cell], fd)
+
+
+def test_fp32_adjoint_close_to_fp64():
+    a64, *_ = _setup("f64")
+    a32, *_ = _setup("f32")
+    with a64, a32:
+        a64.forward(15, 4)
+        a32.forward(15, 4)
+        g64, g32 = a64.backward(), a32.backward()
+    assert np.abs(g32 - g64).max() <= 1e-4 * np.abs(g64).max()
+
+
+def test_autograd_function_wrapper():
+    adj, model, f0, s, solid = _setup()
+    with adj:
+        st = torch.tensor(s, requires_grad=True)
+        J = AdjointObjective.apply(st, adj, 15, 4)
+        (2.0 * J).backward()
+        g = adj.backward()
+    np.testing.assert_allclose(st.grad.numpy(), 2.0 * g, rtol=0, atol=0)
+
+
+def test_channel_gradient_against_fd():
+    """192 x 96 x 96 channel, periodic x, walls in y and z, g = 1e-6, tau 0.6,
+    Brinkman s from the seeded smoothed field, T = 800: FP32 adjoint with
+    checkpoints every 28 steps against FP64 central differences."""
+    T, K, h = 800, 28, 1e-4
+    adj32, s = channel_adjoint(storage="f32")
+    with adj32:
+        adj32.forward(T, K)
+        g = adj32.backward()
+        fwd_ms, bwd_ms = adj32.timing()
+    N = s.size
+    cells = np.random.default_rng(99).choice(N, 32, replace=False)
+    adj64, s64 = channel_adjoint(storage="f64")
+    with adj64:
+        fd = np.empty(32)
+        for k, c in enumerate(cells):
+            adj64.set_fraction_at(int(c), s64[c] + h)
+            jp = adj64.forward(T, -1)
+            adj64.set_fraction_at(int(c), s64[c] - h)
+            jm = adj64.forward(T, -1)
+            adj64.set_fraction_at(int(c), s64[c])
+            fd[k] = (jp - jm) / (2 * h)
+    rel = np.abs(g[cells] - fd) / np.abs(fd)
+    print(f"channel: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, max rel {rel.max():.3e}, "
+          f"median rel {np.median(rel):.3e}, |fd| in [{np.abs(fd).min():.3e}, {np.abs(fd).max():.3e}]")
+    assert rel.max() <= 1e-3
