@@ -32,6 +32,16 @@ FLAGS = [
 ]
 
 
+# Per-source flags. The 16-bit storage kinds have no FreeLB counterpart to
+# match bit for bit, so their collision arithmetic may contract to FMAs and
+# use the hardware reciprocal: their kernels are instruction-bound rather
+# than memory-bound.
+SOURCE_FLAGS = {
+    "flb_inst_f16s.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "flb_inst_bf16s.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
+
+
 def build(verbose: bool = False, jobs: int = 0, extra_flags=(), lib: str = LIB,
           objdir: str = "") -> str:
     """Compile every source to an object in parallel, then link the library.
@@ -41,11 +51,12 @@ def build(verbose: bool = False, jobs: int = 0, extra_flags=(), lib: str = LIB,
 
     objdir = objdir or os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    compile_flags = [f for f in FLAGS if f != "-shared"] + list(extra_flags)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
 
     def compile_one(src):
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *compile_flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        cmd = [HIPCC, *compile_flags, *SOURCE_FLAGS.get(src, []), *extra_flags, "-c", "-o", obj,
+               os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

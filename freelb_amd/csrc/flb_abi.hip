@@ -153,7 +153,14 @@ KArgs make_args(const flb_block_s* b) {
   a.py = b->desc.periodic[1] != 0;
   a.pz = b->d == 3 && b->desc.periodic[2] != 0;
   a.vec = vec_of(b);
-  if (a.vec > 0) {
+  if (b->vec == -2 && b->elem == 2 && !b->desc.periodic[0]) a.vec = -2;  // pair form
+  if (a.vec == -2) {
+    a.groups = (a.ex + 1) / 2;
+    a.wpr = (a.ex + 2 * flb::kWave - 1) / (2 * flb::kWave);
+    a.wpb = b->wpb;
+    const int64_t waves = int64_t(a.wpr) * a.ey * a.ez;
+    a.nblocks = int((waves + a.wpb - 1) / a.wpb);
+  } else if (a.vec > 0) {
     // row form: one workgroup per row
     a.groups = (a.ex + a.vec - 1) / a.vec;
     a.wpr = (a.groups + flb::kWave - 1) / flb::kWave;
@@ -359,9 +366,10 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
     return fail(FLB_ERR_HIP, "no HIP device");
   }
   if (const char* env = std::getenv("FLB_LAUNCH")) {
-    // "vec,waves_per_workgroup", e.g. "4,4" (vec -1: cell form); invalid settings are ignored
+    // "vec,waves_per_workgroup", e.g. "4,4" (vec -1: cell form, -2: pair form of the 16-bit
+    // kinds); invalid settings are ignored
     int v = 0, w = 0;
-    if (std::sscanf(env, "%d,%d", &v, &w) == 2 && (v == -1 || v == 1 || v == 2 || v == 4) && w >= 1 &&
+    if (std::sscanf(env, "%d,%d", &v, &w) == 2 && (v == -2 || v == -1 || v == 1 || v == 2 || v == 4) && w >= 1 &&
         w * flb::kWave <= flb::kMaxThreads) {
       b->vec = v;
       b->wpb = w;

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace flb {
 
 // ---------------------------------------------------------------------------
@@ -294,6 +296,19 @@ __host__ __device__ constexpr double m19_default_rate(int j) {
   return s[j];
 }
 
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1, each with a
+// compile-time index, so that matrix entries taken from constexpr functions
+// fold into constants (a loop the compiler leaves rolled would evaluate them
+// at run time).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // rates[j] < 0 or NaN marks a shear rate, which takes om
 template <class R>
 __device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, const double* rates) {
@@ -303,24 +318,26 @@ __device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, co
 #pragma unroll
   for (int i = 0; i < 19; ++i) g[i] = f[i] - e[i];
   R k[19];
-#pragma unroll
-  for (int j = 0; j < 19; ++j) {
+  static_for<19>([&](auto J) {
+    constexpr int j = decltype(J)::value;
     const double sr = rates[j];
     const R s = (sr >= 0.0) ? R(sr) : om;
     R dm = R(0);
-#pragma unroll
-    for (int i = 0; i < 19; ++i)
-      if (m19(j, i) != 0) dm += R(m19(j, i)) * g[i];
+    static_for<19>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      if constexpr (m19(j, i) != 0) dm += R(m19(j, i)) * g[i];
+    });
     k[j] = s * dm / R(m19_norm(j));
-  }
-#pragma unroll
-  for (int i = 0; i < 19; ++i) {
-    R s = R(0);
-#pragma unroll
-    for (int j = 0; j < 19; ++j)
-      if (m19(j, i) != 0) s += R(m19(j, i)) * k[j];
-    f[i] -= s;
-  }
+  });
+  static_for<19>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    R acc = R(0);
+    static_for<19>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      if constexpr (m19(j, i) != 0) acc += R(m19(j, i)) * k[j];
+    });
+    f[i] -= acc;
+  });
 }
 
 __host__ __device__ constexpr int m9(int i, int j) {

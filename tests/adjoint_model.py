@@ -74,7 +74,9 @@ class Model:
 
     def step(self, f, s):
         om = self.omega
-        rho = f.sum(0)
+        # solid cells hold no populations; rho = 1 there keeps their (unused)
+        # values finite, so that no NaN reaches the gradient through a 0 weight
+        rho = torch.where(self.fluid, f.sum(0), torch.ones(self.N, dtype=torch.float64))
         j = self.C.T @ f
         D = 1 + self.alpha * s / 2
         u = (j + self.G[:, None] / 2) / (rho * D)
