@@ -81,7 +81,7 @@ def time_storage(storage, steps, warmup, timer):
     mass = lat.mass_momentum()
     finite = bool(np.isfinite(mass).all())
     lat.close()
-    bytes_per = 152 if storage == "f32" else (76 if storage in ("f16s", "bf16s") else 304)
+    bytes_per = 152 if storage == "f32" else (76 if storage in ("f16s", "bf16s", "f16c") else 304)
     ms_step = ms / steps
     mlups = n_active * steps / (ms / 1e3) / 1e6
     achieved = n_active * bytes_per / (ms_step / 1e3) / 1e9  # GB/s, algorithmic bytes per launch
@@ -108,7 +108,7 @@ def _vorticity(rhou, n):
 
 
 def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
-                    storages=("f32", "f16s", "bf16s"), seed=2024, u0_amp=1e-3, timer=None):
+                    storages=("f32", "f16s", "bf16s", "f16c"), seed=2024, u0_amp=1e-3, timer=None):
     """Errors of FP32 / FP16-shift / BF16-shift storage against an FP64 run of
     the same cavity (D3Q19, halfway walls, lid 0.1): rho and u from
     flb_moments, vorticity by central differences. Start: rho = 1, u i.i.d.
@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--storages", default="f32,f16s,bf16s")
+    ap.add_argument("--storages", default="f32,f16s,bf16s,f16c")
     ap.add_argument("--no-probe", action="store_true", help="skip the device copy-bandwidth probe")
     ap.add_argument("--precision", action="store_true",
                     help="run the 256^3 x 2000-step FP32/FP16/BF16 vs FP64 error study instead")

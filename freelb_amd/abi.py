@@ -17,8 +17,8 @@ LIB_PATH = os.environ.get("FLB_LIB_PATH") or os.path.join(HERE, "libflb_hip.so")
 # ---- enums (include/flb_abi.h) ---------------------------------------------
 FLB_OK, FLB_ERR_ARG, FLB_ERR_STATE, FLB_ERR_HIP, FLB_ERR_UNSUPPORTED, FLB_ERR_LAYOUT, FLB_ERR_NOMEM = range(7)
 D2Q9, D3Q19 = 9, 19
-STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64 = range(4)
-STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64}
+STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64, STORE_F16C = range(5)
+STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64, "f16c": STORE_F16C}
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK,
  DYN_SMAG_MRT) = range(8)
 HWBC_NORMAL, HWBC_MOVING = 0, 1

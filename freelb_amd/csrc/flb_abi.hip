@@ -34,7 +34,8 @@ int fail(int code, const char* fmt, ...) {
 }
 
 bool storage_ok(int s) {
-  return s == FLB_STORE_F32 || s == FLB_STORE_F16S || s == FLB_STORE_BF16S || s == FLB_STORE_F64;
+  return s == FLB_STORE_F32 || s == FLB_STORE_F16S || s == FLB_STORE_BF16S || s == FLB_STORE_F64 ||
+         s == FLB_STORE_F16C;
 }
 
 hipError_t launch_step(const flb_block_s* b, int family, const KArgs& a) {
@@ -42,6 +43,7 @@ hipError_t launch_step(const flb_block_s* b, int family, const KArgs& a) {
     case FLB_STORE_F32: return flb::launch_step_f32(b->desc.q, b->parity, family, a, b->stream);
     case FLB_STORE_F16S: return flb::launch_step_f16s(b->desc.q, b->parity, family, a, b->stream);
     case FLB_STORE_BF16S: return flb::launch_step_bf16s(b->desc.q, b->parity, family, a, b->stream);
+    case FLB_STORE_F16C: return flb::launch_step_f16c(b->desc.q, b->parity, family, a, b->stream);
     default: return flb::launch_step_f64(b->desc.q, b->parity, family, a, b->stream);
   }
 }
@@ -51,6 +53,7 @@ hipError_t launch_gather(const flb_block_s* b, const KArgs& a, void* out) {
     case FLB_STORE_F32: return flb::launch_gather_f32(b->desc.q, b->parity, a, out, b->stream);
     case FLB_STORE_F16S: return flb::launch_gather_f16s(b->desc.q, b->parity, a, out, b->stream);
     case FLB_STORE_BF16S: return flb::launch_gather_bf16s(b->desc.q, b->parity, a, out, b->stream);
+    case FLB_STORE_F16C: return flb::launch_gather_f16c(b->desc.q, b->parity, a, out, b->stream);
     default: return flb::launch_gather_f64(b->desc.q, b->parity, a, out, b->stream);
   }
 }
@@ -60,6 +63,7 @@ hipError_t launch_scatter(const flb_block_s* b, const KArgs& a, const void* in) 
     case FLB_STORE_F32: return flb::launch_scatter_f32(b->desc.q, a, in, b->stream);
     case FLB_STORE_F16S: return flb::launch_scatter_f16s(b->desc.q, a, in, b->stream);
     case FLB_STORE_BF16S: return flb::launch_scatter_bf16s(b->desc.q, a, in, b->stream);
+    case FLB_STORE_F16C: return flb::launch_scatter_f16c(b->desc.q, a, in, b->stream);
     default: return flb::launch_scatter_f64(b->desc.q, a, in, b->stream);
   }
 }
@@ -69,6 +73,7 @@ hipError_t launch_init_feq(const flb_block_s* b, const KArgs& a, const void* rho
     case FLB_STORE_F32: return flb::launch_init_feq_f32(b->desc.q, a, rhou, b->stream);
     case FLB_STORE_F16S: return flb::launch_init_feq_f16s(b->desc.q, a, rhou, b->stream);
     case FLB_STORE_BF16S: return flb::launch_init_feq_bf16s(b->desc.q, a, rhou, b->stream);
+    case FLB_STORE_F16C: return flb::launch_init_feq_f16c(b->desc.q, a, rhou, b->stream);
     default: return flb::launch_init_feq_f64(b->desc.q, a, rhou, b->stream);
   }
 }
@@ -78,6 +83,7 @@ hipError_t launch_moments(const flb_block_s* b, const KArgs& a, uint8_t mask, vo
     case FLB_STORE_F32: return flb::launch_moments_f32(b->desc.q, b->parity, a, mask, out, b->stream);
     case FLB_STORE_F16S: return flb::launch_moments_f16s(b->desc.q, b->parity, a, mask, out, b->stream);
     case FLB_STORE_BF16S: return flb::launch_moments_bf16s(b->desc.q, b->parity, a, mask, out, b->stream);
+    case FLB_STORE_F16C: return flb::launch_moments_f16c(b->desc.q, b->parity, a, mask, out, b->stream);
     default: return flb::launch_moments_f64(b->desc.q, b->parity, a, mask, out, b->stream);
   }
 }
@@ -87,6 +93,7 @@ hipError_t launch_reduce(const flb_block_s* b, const KArgs& a, uint8_t mask, dou
     case FLB_STORE_F32: return flb::launch_reduce_f32(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
     case FLB_STORE_F16S: return flb::launch_reduce_f16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
     case FLB_STORE_BF16S: return flb::launch_reduce_bf16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+    case FLB_STORE_F16C: return flb::launch_reduce_f16c(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
     default: return flb::launch_reduce_f64(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
   }
 }
@@ -114,8 +121,9 @@ int family_of(const flb_block_s* b);
 // Step kernel form and cells per thread along x. 0 selects the cell form
 // (one cell per thread, any periodic axis); 1, 2, 4 the row form, where one
 // workgroup of at most 4 wavefronts covers a whole row. The row form is only
-// taken without halfway boundaries: its whole-line odd-step stores of a row
-// can land on a wall-redirect slot that another row writes in the same launch.
+// taken without halfway boundaries (its whole-line odd-step stores of a row
+// can land on a wall-redirect slot that another row writes in the same
+// launch) and with the wide row pitch its stores past the row end need.
 int vec_of(const flb_block_s* b) {
   const bool per = b->desc.periodic[0] || b->desc.periodic[1] || b->desc.periodic[2];
   if (per || b->nbc > 0) return 0;
@@ -123,9 +131,18 @@ int vec_of(const flb_block_s* b) {
   if (v <= 0) return 0;
   if (family_of(b) == flb::FAM_FULL && v > 2) v = 2;
   const int ex = b->desc.nx - 2 * b->desc.overlap;
+  if (b->pitch < (ex + 255) / 256 * 256 + 64) return 0;
   while (v <= 4 && (ex + flb::kWave * v - 1) / (flb::kWave * v) > flb::kMaxThreads / flb::kWave) v *= 2;
   if (v > 4) return 0;
   return v;
+}
+
+// Cell-form wave mapping: row segments when rows fill their wavefronts well,
+// else the linear form (wavefronts run across row ends), chosen when less
+// than 90 % of the lanes of row-segment wavefronts would hold a cell.
+bool linear_form(int ex) {
+  const int lanes = (ex + flb::kWave - 1) / flb::kWave * flb::kWave;
+  return 10 * ex < 9 * lanes;
 }
 
 KArgs make_args(const flb_block_s* b) {
@@ -153,14 +170,7 @@ KArgs make_args(const flb_block_s* b) {
   a.py = b->desc.periodic[1] != 0;
   a.pz = b->d == 3 && b->desc.periodic[2] != 0;
   a.vec = vec_of(b);
-  if (b->vec == -2 && b->elem == 2 && !b->desc.periodic[0]) a.vec = -2;  // pair form
-  if (a.vec == -2) {
-    a.groups = (a.ex + 1) / 2;
-    a.wpr = (a.ex + 2 * flb::kWave - 1) / (2 * flb::kWave);
-    a.wpb = b->wpb;
-    const int64_t waves = int64_t(a.wpr) * a.ey * a.ez;
-    a.nblocks = int((waves + a.wpb - 1) / a.wpb);
-  } else if (a.vec > 0) {
+  if (a.vec > 0) {
     // row form: one workgroup per row
     a.groups = (a.ex + a.vec - 1) / a.vec;
     a.wpr = (a.groups + flb::kWave - 1) / flb::kWave;
@@ -170,7 +180,11 @@ KArgs make_args(const flb_block_s* b) {
     a.groups = a.ex;
     a.wpr = (a.ex + flb::kWave - 1) / flb::kWave;
     a.wpb = b->wpb;
-    const int64_t waves = int64_t(a.wpr) * a.ey * a.ez;
+    int64_t waves = int64_t(a.wpr) * a.ey * a.ez;
+    if (b->vec == -3 || (b->vec == 0 && linear_form(a.ex))) {
+      a.vec = -3;
+      waves = (int64_t(a.ex) * a.ey * a.ez + flb::kWave - 1) / flb::kWave;
+    }
     a.nblocks = int((waves + a.wpb - 1) / a.wpb);
   }
   a.ntasks = b->ntasks;
@@ -319,7 +333,12 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
   // groups (the widest vector row a workgroup covers), plus one cache line of
   // padding that takes the odd step's whole-line stores past the row ends
   const int ex_x = desc->periodic[0] ? desc->nx : desc->nx - 2 * desc->overlap;
-  const int pitch = std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
+  // Short rows (at most 222 updated cells) take a compact pitch instead, a
+  // multiple of 32 slots with room for the two overlap cells, so that
+  // neighbouring rows share cache lines rather than each filling a mostly
+  // empty 320-slot row.
+  const int pitch = ex_x + 2 <= 224 ? (ex_x + 2 + 31) / 32 * 32
+                                    : std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
   const int64_t plane = int64_t(pitch) * desc->ny;
   // tail: one extra row plus the widest vector run past a row end
   const int64_t tail = 2 * int64_t(pitch) + 1024;
@@ -366,10 +385,11 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
     return fail(FLB_ERR_HIP, "no HIP device");
   }
   if (const char* env = std::getenv("FLB_LAUNCH")) {
-    // "vec,waves_per_workgroup", e.g. "4,4" (vec -1: cell form, -2: pair form of the 16-bit
-    // kinds); invalid settings are ignored
+    // "vec,waves_per_workgroup", e.g. "4,4": vec 0 automatic cell form, -1 cell form over
+    // row segments, -3 cell form over linear runs, 1/2/4 row form; invalid settings are
+    // ignored
     int v = 0, w = 0;
-    if (std::sscanf(env, "%d,%d", &v, &w) == 2 && (v == -2 || v == -1 || v == 1 || v == 2 || v == 4) && w >= 1 &&
+    if (std::sscanf(env, "%d,%d", &v, &w) == 2 && v >= -3 && v <= 4 && v != 3 && v != -2 && w >= 1 &&
         w * flb::kWave <= flb::kMaxThreads) {
       b->vec = v;
       b->wpb = w;

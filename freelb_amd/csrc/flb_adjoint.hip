@@ -31,7 +31,10 @@
 //
 // Layout: one dense array per direction, [19][N], N = nx*ny*nz,
 // id = x + nx*(y + ny*z) (FreeLB's index order); states are kept whole for
-// checkpoints, so no AA parity is involved.
+// checkpoints, so no AA parity is involved. The arrays hold g_i = f_i - w_i
+// and all moments are formed from g (rho = 1 + sum g, j = sum c g): the
+// velocities of a slow channel flow (u ~ 1e-5) then keep full precision in
+// FP32 instead of drowning in the rounding of f ~ w.
 
 #include <hip/hip_runtime.h>
 
@@ -85,19 +88,21 @@ __device__ __forceinline__ int64_t neighbour(const AdjArgs<R>& a, int x, int y, 
   return a.solid[id] ? -1 : id;
 }
 
-// rho, u, F and D of a cell (the forward model above).
+// rho - 1, rho, u, F and D of a cell from its shifted populations g (the
+// forward model above).
 template <class R>
-__device__ __forceinline__ void force_state(const R (&f)[Q], R s, const AdjArgs<R>& a, R& rho,
+__device__ __forceinline__ void force_state(const R (&g)[Q], R s, const AdjArgs<R>& a, R& drho, R& rho,
                                             R (&u)[3], R (&F)[3], R& D) {
-  rho = R(0);
+  drho = R(0);
   R j[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    rho += f[i];
-    j[0] += R(Lat<Q>::cx(i)) * f[i];
-    j[1] += R(Lat<Q>::cy(i)) * f[i];
-    j[2] += R(Lat<Q>::cz(i)) * f[i];
+    drho += g[i];
+    j[0] += R(Lat<Q>::cx(i)) * g[i];
+    j[1] += R(Lat<Q>::cy(i)) * g[i];
+    j[2] += R(Lat<Q>::cz(i)) * g[i];
   }
+  rho = R(1) + drho;
   D = R(1) + a.alpha * s * R(0.5);
   const R inv = R(1) / (rho * D);
 #pragma unroll
@@ -106,9 +111,19 @@ __device__ __forceinline__ void force_state(const R (&f)[Q], R s, const AdjArgs<
   for (int d = 0; d < 3; ++d) F[d] = a.G[d] - a.alpha * s * rho * u[d];
 }
 
+// feq_i(rho, u) - w_i, formed without cancellation.
+template <class R>
+__device__ __forceinline__ R geq(int i, R drho, R rho, const R (&u)[3], R usq);
+
 template <class R>
 __device__ __forceinline__ R cdot(int i, const R (&v)[3]) {
   return R(Lat<Q>::cx(i)) * v[0] + R(Lat<Q>::cy(i)) * v[1] + R(Lat<Q>::cz(i)) * v[2];
+}
+
+template <class R>
+__device__ __forceinline__ R geq(int i, R drho, R rho, const R (&u)[3], R usq) {
+  const R cu = cdot(i, u);
+  return Lat<Q>::template w<R>(i) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq));
 }
 
 __device__ __forceinline__ void cell_xyz(int64_t id, int nx, int ny, int& x, int& y, int& z) {
@@ -126,13 +141,12 @@ __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restr
   if (id >= a.N || a.solid[id]) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
-  R f[Q];
+  R g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = src[int64_t(i) * a.N + id];
-  R rho, u[3], F[3], D;
-  force_state(f, a.s[id], a, rho, u, F, D);
-  R e[Q];
-  feq(e, rho, u);
+  for (int i = 0; i < Q; ++i) g[i] = src[int64_t(i) * a.N + id];
+  R drho, rho, u[3], F[3], D;
+  force_state(g, a.s[id], a, drho, rho, u, F, D);
+  const R usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   const R om = a.omega, _om = R(1) - a.omega, fom = R(1) - a.omega * R(0.5);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -141,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restr
     const R t1 = R(3) * (R(Lat<Q>::cy(i)) - u[1]) + R(9) * cu * R(Lat<Q>::cy(i));
     const R t2 = R(3) * (R(Lat<Q>::cz(i)) - u[2]) + R(9) * cu * R(Lat<Q>::cz(i));
     const R Fi = Lat<Q>::template w<R>(i) * (F[0] * t0 + F[1] * t1 + F[2] * t2);
-    const R v = om * e[i] + _om * f[i] + fom * Fi;
+    const R v = om * geq(i, drho, rho, u, usq) + _om * g[i] + fom * Fi;
     const int64_t nb = i == 0 ? id : neighbour(a, x, y, z, i);
     if (nb < 0) dst[int64_t(opp(i)) * a.N + id] = v;
     else dst[int64_t(i) * a.N + nb] = v;
@@ -179,12 +193,11 @@ __global__ __launch_bounds__(kThreads) void k_init(AdjArgs<R> a, const R* __rest
                                                    R* __restrict__ f) {
   const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (id >= a.N) return;
-  R e[Q];
   const R u[3] = {rhou[a.N + id], rhou[2 * a.N + id], rhou[3 * a.N + id]};
-  feq(e, rhou[id], u);
+  const R rho = rhou[id], usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   const bool sol = a.solid[id] != 0;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) f[int64_t(i) * a.N + id] = sol ? R(0) : e[i];
+  for (int i = 0; i < Q; ++i) f[int64_t(i) * a.N + id] = sol ? R(0) : geq(i, rho - R(1), rho, u, usq);
 }
 
 // ---- adjoint seed: lambda^T = dJ/df^T -----------------------------------------
@@ -206,13 +219,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restr
   if (id >= a.N || a.solid[id]) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
-  R f[Q];
+  R g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = fst[int64_t(i) * a.N + id];
+  for (int i = 0; i < Q; ++i) g[i] = fst[int64_t(i) * a.N + id];
   const R s = a.s[id];
-  R rho, u[3], F[3], D;
-  force_state(f, s, a, rho, u, F, D);
-  // mu = P^T lambda^{t+1}, kept in f's registers' place
+  R drho, rho, u[3], F[3], D;
+  force_state(g, s, a, drho, rho, u, F, D);
+  // mu = P^T lambda^{t+1}
   R mu[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -281,6 +294,7 @@ struct flb_adjoint_s {
   flb_adjoint_desc desc{};
   int64_t N = 0;
   size_t real = 4;
+  std::vector<uint8_t> h_solid;
   uint8_t* d_solid = nullptr;
   void* d_s = nullptr;
   void* d_f0 = nullptr;
@@ -543,6 +557,7 @@ int flb_adjoint_set_solid(flb_adjoint h, const uint8_t* solid) {
   DeviceGuard g(h->device);
   std::vector<uint8_t> m(size_t(h->N));
   for (int64_t i = 0; i < h->N; ++i) m[size_t(i)] = solid[i] ? 1 : 0;
+  h->h_solid = m;
   HIP_TRY(hipMemcpyAsync(h->d_solid, m.data(), size_t(h->N), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->recorded = false;
@@ -647,6 +662,13 @@ int flb_adjoint_state(flb_adjoint h, double* pops) {
     HIP_TRY(hipMemcpyAsync(t.data(), h->last, n * 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (size_t i = 0; i < n; ++i) pops[i] = double(t[i]);
+  }
+  // stored g = f - w; solid cells hold no populations
+  for (int i = 0; i < A::Q; ++i) {
+    const double w = flb::Lat<A::Q>::template w<double>(i);
+    double* p = pops + size_t(i) * size_t(h->N);
+    for (int64_t c = 0; c < h->N; ++c)
+      p[c] = (!h->h_solid.empty() && h->h_solid[size_t(c)]) ? 0.0 : p[c] + w;
   }
   return FLB_OK;
 }

@@ -60,19 +60,27 @@ hipError_t launch_batch(const flb_batch_s* t, int parity, hipStream_t s) {
       return flb::launch_batch_f16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
     case FLB_STORE_BF16S:
       return flb::launch_batch_bf16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+    case FLB_STORE_F16C:
+      return flb::launch_batch_f16c(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
     default:
       return flb::launch_batch_f64(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
   }
 }
 
-// The kernel arguments of block b in the batch's cell form (one wavefront per
-// 64 cells of a row, whatever form the block itself would launch).
+// The kernel arguments of block b in the batch's cell form (row segments, or
+// linear runs for short rows, whatever form the block itself would launch).
 KArgs batch_args(const flb_block_s* b) {
   KArgs a = make_args(b);
-  a.vec = 0;
   a.groups = a.ex;
   a.wpr = (a.ex + flb::kWave - 1) / flb::kWave;
+  a.vec = linear_form(a.ex) ? -3 : 0;
   return a;
+}
+
+// Wavefronts of one block in the batch kernel.
+int64_t waves_of(const KArgs& a) {
+  if (a.vec == -3) return (int64_t(a.ex) * a.ey * a.ez + flb::kWave - 1) / flb::kWave;
+  return int64_t(a.wpr) * a.ey * a.ez;
 }
 
 // Common state of the blocks: same parity, nothing pending, flags present.
@@ -103,7 +111,7 @@ int refresh(flb_batch_s* t, bool force) {
     const flb_block_s* b = t->blocks[k];
     t->h_tab[k] = batch_args(b);
     const KArgs& a = t->h_tab[k];
-    t->h_wave0.push_back(t->h_wave0.back() + int64_t(a.wpr) * a.ey * a.ez);
+    t->h_wave0.push_back(t->h_wave0.back() + waves_of(a));
     t->cells += int64_t(a.ex) * a.ey * a.ez;
     t->family = std::max(t->family, family_of(b));
     t->per = t->per || a.px || a.py || a.pz;

@@ -43,7 +43,8 @@ struct flb_block_s {
   int64_t step = 0;
   int parity = 0;
   int pending = 0;  // 1: cell dynamics applied, stream not yet called
-  int vec = -1;  // step kernel form: -1 cell form, 1/2/4 row form with that many cells per thread
+  int vec = 0;  // step kernel form: 0 automatic cell form, -1 row segments, -3 linear runs,
+                // 1/2/4 row form with that many cells per thread
   int wpb = 4;   // wavefronts per workgroup
   uint64_t version = 0;  // bumped by every change a batch table must see
 };
@@ -77,6 +78,8 @@ struct DeviceGuard {
 };
 
 KArgs make_args(const flb_block_s* b);
+// True when rows of ex cells fill less than 90 % of their wavefronts' lanes.
+bool linear_form(int ex);
 int family_of(const flb_block_s* b);
 
 }  // namespace flbh

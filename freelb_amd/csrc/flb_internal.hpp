@@ -85,6 +85,7 @@ FLB_DECLARE_LAUNCHERS(f32)
 FLB_DECLARE_LAUNCHERS(f16s)
 FLB_DECLARE_LAUNCHERS(bf16s)
 FLB_DECLARE_LAUNCHERS(f64)
+FLB_DECLARE_LAUNCHERS(f16c)
 
 // float4 copy of n4 16-byte elements, one element per thread (bandwidth calibration).
 hipError_t launch_copy_probe(void* dst, const void* src, size_t n4, hipStream_t s);

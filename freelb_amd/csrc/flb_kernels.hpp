@@ -297,19 +297,42 @@ __device__ __forceinline__ void st_pop(E* p, E v) {
   else *p = v;
 }
 
+template <int Q, class S, int PAR, int FAM, bool PER>
+__device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z);
+
 // Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
-// up to 64 cells; lane is the lane id.
+// up to 64 cells; lane is the lane id. With vec == -3 (linear form) wavefronts run over the
+// block's updated cells in x-fastest order instead.
 template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
+  int c0, y, z;
+  if (a.vec == -3) {
+    // linear form: wavefront gw takes updated cells 64 gw .. 64 gw + 63 in
+    // x-fastest order, across row ends (short rows fill whole wavefronts)
+    const unsigned c = unsigned(gw) * unsigned(kWave) + unsigned(lane);
+    const unsigned exy = unsigned(a.ex) * unsigned(a.ey);
+    if (c >= exy * unsigned(a.ez)) return;
+    const unsigned zz = c / exy, r = c - zz * exy, yy = r / unsigned(a.ex);
+    c0 = int(r - yy * unsigned(a.ex));
+    y = a.ys + int(yy);
+    z = a.zs + int(zz);
+  } else {
+    const int row = gw / a.wpr;
+    if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
+    const int wv = gw - row * a.wpr;
+    y = a.ys + row % a.ey;
+    z = a.zs + row / a.ey;
+    c0 = wv * kWave + lane;
+    if (c0 >= a.ex) return;
+  }
+  cell_update_at<Q, S, PAR, FAM, PER>(a, c0, y, z);
+}
+
+// The update of the cell at x = xs + c0 in row (y, z).
+template <int Q, class S, int PAR, int FAM, bool PER>
+__device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z) {
   using R = typename S::R;
   using E = typename S::E;
-  const int row = gw / a.wpr;
-  if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
-  const int wv = gw - row * a.wpr;
-  const int y = a.ys + row % a.ey;
-  const int z = a.zs + row / a.ey;
-  const int c0 = wv * kWave + lane;
-  if (c0 >= a.ex) return;
   const int x = a.xs + c0;
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
@@ -367,177 +390,6 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
   const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
   const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
   cell_update<Q, S, PAR, FAM, PER>(a, gw, lane);
-}
-
-// ---------------------------------------------------------------------------
-// The step kernel, pair form, for the 16-bit storage kinds: two neighbouring
-// cells of a row per thread, so every population access of a wavefront is one
-// 32-bit word per lane (a 2-byte lane access costs nearly as much as a 4-byte
-// one). A wavefront covers 128 cells of one row; x must not be periodic.
-// At the odd step the directions with c_x != 0 are one cell off the word
-// grid: each lane loads (stores) its aligned word and exchanges the missing
-// half with the neighbouring lane through a DPP lane shift; the lane at the
-// wavefront's edge loads (stores) that one element by itself. Cells a lane
-// must not write (dead cells, cells past the row end, wall links) drop out
-// of the word store, which then falls back to single elements. Every lane
-// stays active through the lane shifts.
-// ---------------------------------------------------------------------------
-template <int Q, class S, int PAR, int FAM, bool PER>
-__device__ __forceinline__ void pair_update(const KArgs& a, int gw, int lane) {
-  using R = typename S::R;
-  using E = typename S::E;
-  static_assert(sizeof(E) == 2, "pair form is for 16-bit storage");
-  const int row = gw / a.wpr;
-  if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
-  const int wv = gw - row * a.wpr;
-  const int y = a.ys + row % a.ey;
-  const int z = a.zs + row / a.ey;
-  const int c0 = (wv * kWave + lane) * 2;
-  const int s0 = c0 + a.pitch * y + int(a.plane) * z;
-  E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
-  const uint16_t fw = *reinterpret_cast<const uint16_t*>(a.flags + s0);
-  const uint8_t fl[2] = {uint8_t(fw), uint8_t(fw >> 8)};
-  int dyn[2];
-  float cs[2] = {0.0f, 0.0f};
-  bool live[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    dyn[j] = c0 + j < a.ex ? match_task(a, fl[j], cs[j]) : int(DYN_NONE);
-    live[j] = dyn[j] != DYN_NONE;
-  }
-  unsigned ln[2] = {0u, 0u}, lm[2] = {0u, 0u};
-  if (PAR == 1 && a.nbc) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (live[j]) wall_links<Q, PER>(a, a.xs + c0 + j, y, z, s0 + j, fl[j], ln[j], lm[j]);
-  }
-  const unsigned lk[2] = {ln[0] | lm[0], ln[1] | lm[1]};
-  R f0[Q], f1[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    E e0, e1;
-    if (PAR == 0) {
-      const uint32_t w = ld_pop<PAR>(reinterpret_cast<const uint32_t*>(P + int64_t(i) * a.dpop + c0));
-      e0 = E(w);
-      e1 = E(w >> 16);
-    } else {
-      const int o = opp(i);
-      const int cxo = Lat<Q>::cx(o);  // source x + c_o = x - c_i
-      const E* Ro = P + int64_t(o) * a.dpop + yz_off<Q, PER, true>(a, o, y, z);
-      const uint32_t w = *reinterpret_cast<const uint32_t*>(Ro + c0);
-      if (cxo == 0) {
-        e0 = E(w);
-        e1 = E(w >> 16);
-      } else if (cxo > 0) {  // cells take slots c0 + 1, c0 + 2
-        uint32_t wn = wave_shl(unsigned(w));
-        if (lane == kWave - 1) wn = Ro[c0 + 2];
-        e0 = E(w >> 16);
-        e1 = E(wn);
-      } else {  // cells take slots c0 - 1, c0
-        uint32_t wp = wave_shr(unsigned(w));
-        if (lane == 0) wp = uint32_t(Ro[c0 - 1]) << 16;
-        e0 = E(wp >> 16);
-        e1 = E(w);
-      }
-      if (i != 0) {
-        // upstream neighbour a wall: the cell's own f*_{opp i}, left in A[i][x]
-        if ((lk[0] >> o) & 1u) e0 = P[int64_t(i) * a.dpop + c0];
-        if ((lk[1] >> o) & 1u) e1 = P[int64_t(i) * a.dpop + c0 + 1];
-      }
-    }
-    f0[i] = S::template from<Q>(e0, i);
-    f1[i] = S::template from<Q>(e1, i);
-  }
-  R rho_f[2] = {R(1), R(1)}, u_f[2][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
-  if (PAR == 1) {
-    if (lm[0]) {
-      load_fields<Q>(a, s0, rho_f[0], u_f[0]);
-#pragma unroll
-      for (int i = 1; i < Q; ++i)
-        if ((lm[0] >> opp(i)) & 1u) f0[i] = f0[i] + movingwall_term<Q>(i, rho_f[0], u_f[0]);
-    }
-    if (lm[1]) {
-      load_fields<Q>(a, s0 + 1, rho_f[1], u_f[1]);
-#pragma unroll
-      for (int i = 1; i < Q; ++i)
-        if ((lm[1] >> opp(i)) & 1u) f1[i] = f1[i] + movingwall_term<Q>(i, rho_f[1], u_f[1]);
-    }
-  }
-  const R om = R(a.omega);
-  if (live[0]) collide<Q, R, FAM>(a, s0, dyn[0], cs[0], f0, om, R(1) - om);
-  if (live[1]) collide<Q, R, FAM>(a, s0 + 1, dyn[1], cs[1], f1, om, R(1) - om);
-  // directions each cell may store to its ordinary target
-  const unsigned ok0 = live[0] ? ~lk[0] : 0u, ok1 = live[1] ? ~lk[1] : 0u;
-  const unsigned okp = PAR == 1 ? wave_shr(ok1) : 0u;  // previous lane's second cell
-  const unsigned okn = PAR == 1 ? wave_shl(ok0) : 0u;  // next lane's first cell
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    R v0 = f0[i], v1 = f1[i];
-    if (PAR == 1 && i != 0) {
-      if ((lm[0] >> i) & 1u) v0 = v0 + movingwall_term<Q>(opp(i), rho_f[0], u_f[0]);
-      if ((lm[1] >> i) & 1u) v1 = v1 + movingwall_term<Q>(opp(i), rho_f[1], u_f[1]);
-    }
-    const E e0 = S::template to<Q>(v0, i), e1 = S::template to<Q>(v1, i);
-    const uint32_t packed = uint32_t(e0) | (uint32_t(e1) << 16);
-    if (PAR == 0) {
-      E* d = P + int64_t(opp(i)) * a.dpop + c0;
-      if (live[0] && live[1]) {
-        st_pop<PAR>(reinterpret_cast<uint32_t*>(d), packed);
-      } else {
-        if (live[0]) d[0] = e0;
-        if (live[1]) d[1] = e1;
-      }
-    } else {
-      const int cxi = Lat<Q>::cx(i);
-      E* Ri = P + int64_t(i) * a.dpop + yz_off<Q, PER, true>(a, i, y, z);
-      const bool a0 = (ok0 >> i) & 1u, a1 = (ok1 >> i) & 1u;
-      if (cxi == 0) {
-        if (a0 && a1) {
-          st_pop<PAR>(reinterpret_cast<uint32_t*>(Ri + c0), packed);
-        } else {
-          if (a0) Ri[c0] = e0;
-          if (a1) Ri[c0 + 1] = e1;
-        }
-      } else if (cxi > 0) {
-        // slot c0 <- previous lane's second cell, slot c0 + 1 <- own first cell
-        const uint32_t pp = wave_shr(unsigned(packed));
-        const bool ap = (okp >> i) & 1u;
-        const E lo = E(pp >> 16);
-        if (ap && a0) {
-          st_pop<PAR>(reinterpret_cast<uint32_t*>(Ri + c0), uint32_t(lo) | (uint32_t(e0) << 16));
-        } else {
-          if (ap) Ri[c0] = lo;
-          if (a0) Ri[c0 + 1] = e0;
-        }
-        if (lane == kWave - 1 && a1) Ri[c0 + 2] = e1;
-      } else {
-        // slot c0 <- own second cell, slot c0 + 1 <- next lane's first cell
-        const uint32_t pn = wave_shl(unsigned(packed));
-        const bool an = (okn >> i) & 1u;
-        const E hi = E(pn);
-        if (a1 && an) {
-          st_pop<PAR>(reinterpret_cast<uint32_t*>(Ri + c0), uint32_t(e1) | (uint32_t(hi) << 16));
-        } else {
-          if (a1) Ri[c0] = e1;
-          if (an) Ri[c0 + 1] = hi;
-        }
-        if (lane == 0 && a0) Ri[c0 - 1] = e0;
-      }
-      if (i != 0) {
-        // downstream neighbour a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
-        if (live[0] && ((lk[0] >> i) & 1u)) P[int64_t(opp(i)) * a.dpop + c0] = e0;
-        if (live[1] && ((lk[1] >> i) & 1u)) P[int64_t(opp(i)) * a.dpop + c0 + 1] = e1;
-      }
-    }
-  }
-}
-
-template <int Q, class S, int PAR, int FAM, bool PER>
-__global__ __launch_bounds__(kMaxThreads) void k_step_pair(KArgs a) {
-  const int lane = int(threadIdx.x) & (kWave - 1);
-  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-  const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
-  pair_update<Q, S, PAR, FAM, PER>(a, gw, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,18 +861,6 @@ template <int Q, class S, int PAR>
 hipError_t step_q(int family, const KArgs& a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(kWave * a.wpb);
   const bool per = a.px || a.py || a.pz;
-  if constexpr (sizeof(typename S::E) == 2) {
-    if (a.vec == -2) {
-      if (family == FAM_FULL) {
-        if (per) hipLaunchKernelGGL((k_step_pair<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step_pair<Q, S, PAR, FAM_FULL, false>), grid, block, 0, s, a);
-      } else {
-        if (per) hipLaunchKernelGGL((k_step_pair<Q, S, PAR, FAM_BGK, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step_pair<Q, S, PAR, FAM_BGK, false>), grid, block, 0, s, a);
-      }
-      return hipGetLastError();
-    }
-  }
   if (per || a.vec <= 0) {
     if (family == FAM_FULL) {
       if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);

@@ -1,7 +1,7 @@
 // Population storage formats in HBM.
 //
 // Each direction i owns one array (SoA) in the padded slot layout described
-// in flb_internal.hpp. The shifted half formats store f_i - w_i (rho0 = 1):
+// in flb_internal.hpp. The shifted 16-bit formats store f_i - w_i (rho0 = 1):
 // around equilibrium the stored value is a small deviation, so the 11-bit
 // (FP16) or 8-bit (BF16) significand resolves it relative to its own size
 // instead of relative to w_i. All arithmetic is done in the compute type R
@@ -48,15 +48,10 @@ struct StoreF64 {
   }
 };
 
-#ifndef FLB_F16_VARIANT
-#define FLB_F16_VARIANT 0
-#endif
-
 struct StoreF16S {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 1;
-#if FLB_F16_VARIANT == 0
   template <int Q>
   __device__ __forceinline__ static R from(E e, int i) {
     return __half2float(__ushort_as_half(e)) + Lat<Q>::template w<float>(i);
@@ -65,43 +60,36 @@ struct StoreF16S {
   __device__ __forceinline__ static E to(R v, int i) {
     return __half_as_ushort(__float2half_rn(v - Lat<Q>::template w<float>(i)));
   }
-#elif FLB_F16_VARIANT == 1 || FLB_F16_VARIANT == 3
-  // IEEE half of (f - w) * 2^14: the shifted values stay in the normal range
-  template <int Q>
-  __device__ __forceinline__ static R from(E e, int i) {
-    return __half2float(__ushort_as_half(e)) * 6.103515625e-05f + Lat<Q>::template w<float>(i);
-  }
-  template <int Q>
-  __device__ __forceinline__ static E to(R v, int i) {
-    return __half_as_ushort(__float2half_rn((v - Lat<Q>::template w<float>(i)) * 16384.0f));
-  }
-#elif FLB_F16_VARIANT == 2
-  // 1 sign, 4 exponent, 11 fraction bits: |x| = (1 + m/2^11) 2^(E-18), E = 1..15,
-  // m 2^-28 for E = 0; largest 0.2499, round to nearest even, saturating
+};
+
+// 16-bit float of f - w with 1 sign, 4 exponent and 11 fraction bits:
+// |x| = (1 + m/2^11) 2^(E-18) for E = 1..15 and m 2^-28 for E = 0, so the
+// range is [2^-28, 0.2499] with 12 significant bits, one more than IEEE half
+// (f - w stays far below 0.25 at lattice speeds). Round to nearest even;
+// larger magnitudes saturate. The E = 0 codes are formed by an FP32 add of
+// 2^-5, whose ulp is exactly 2^-28.
+struct StoreF16C {
+  using E = uint16_t;
+  using R = float;
+  static constexpr int kind = 4;
+  static constexpr uint32_t kTwoM5 = 122u << 23;  // bits of 2^-5
   template <int Q>
   __device__ __forceinline__ static R from(E e, int i) {
     const uint32_t m = uint32_t(e) & 0x7FFFu;
-    float a = (m >> 11) == 0u ? float(m) * 3.7252902984619140625e-09f
-                              : __uint_as_float((m + (109u << 11)) << 12);
+    const float sub = __uint_as_float(kTwoM5 + m) - 0.03125f;
+    const float nor = __uint_as_float((m + (109u << 11)) << 12);
+    const float a = m < 2048u ? sub : nor;
     return ((e & 0x8000u) ? -a : a) + Lat<Q>::template w<float>(i);
   }
   template <int Q>
   __device__ __forceinline__ static E to(R v, int i) {
-    const float x = v - Lat<Q>::template w<float>(i);
-    const uint32_t u = __float_as_uint(x);
-    const uint32_t sgn = (u >> 16) & 0x8000u;
+    const uint32_t u = __float_as_uint(v - Lat<Q>::template w<float>(i));
     const uint32_t a = u & 0x7FFFFFFFu;
-    uint32_t code;
-    if (a < (110u << 23)) {  // below 2^-17
-      code = uint32_t(__float2uint_rn(__uint_as_float(a) * 268435456.0f));
-    } else {
-      const uint32_t r = (a + 0x7FFu + ((a >> 12) & 1u)) >> 12;
-      code = r - (109u << 11);
-      if (code > 0x7FFFu) code = 0x7FFFu;
-    }
-    return E(sgn | code);
+    const int cn = int((a + 0x7FFu + ((a >> 12) & 1u)) >> 12) - int(109u << 11);
+    const uint32_t nor = uint32_t(cn < 0x7FFF ? cn : 0x7FFF);
+    const uint32_t sub = __float_as_uint(__uint_as_float(a) + 0.03125f) - kTwoM5;
+    return E(((u >> 16) & 0x8000u) | (a < (110u << 23) ? sub : nor));
   }
-#endif
 };
 
 struct StoreBF16S {

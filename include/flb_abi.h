@@ -70,8 +70,11 @@ enum {
 enum { FLB_D2Q9 = 9, FLB_D3Q19 = 19 };
 
 /* Population storage. The shifted forms store f_i - w_i (rho0 = 1) and do
- * all arithmetic in FP32; FLB_STORE_F64 stores and computes in FP64. */
-enum { FLB_STORE_F32 = 0, FLB_STORE_F16S = 1, FLB_STORE_BF16S = 2, FLB_STORE_F64 = 3 };
+ * all arithmetic in FP32: F16S as IEEE binary16, BF16S as bfloat16, F16C as
+ * a 16-bit float with 4 exponent and 11 fraction bits (range 2^-28 .. 0.25,
+ * one more significant bit than binary16). FLB_STORE_F64 stores and
+ * computes in FP64. */
+enum { FLB_STORE_F32 = 0, FLB_STORE_F16S = 1, FLB_STORE_BF16S = 2, FLB_STORE_F64 = 3, FLB_STORE_F16C = 4 };
 
 /* Cell dynamics (the `Dynamics` of a Key_TypePair). */
 enum {
@@ -153,8 +156,8 @@ int flb_set_field_rho(flb_block b, const double* rho);             /* N values  
 int flb_set_field_u(flb_block b, const double* u);                 /* d*N values, SoA */
 
 /* ---- populations --------------------------------------------------------- */
-/* Natural-layout (post-stream) populations [q][N]. _f32 serves the F32, F16S
- * and BF16S storage kinds, _f64 serves F64. Upload resets the layout to
+/* Natural-layout (post-stream) populations [q][N]. _f32 serves the F32, F16S,
+ * BF16S and F16C storage kinds, _f64 serves F64. Upload resets the layout to
  * natural; download resolves the AA layout of the current parity. Values of
  * cells outside the visited interior download as 0. */
 int flb_upload_pops_f32(flb_block b, const float* pops);

@@ -46,7 +46,7 @@ def _close(bs):
         b.close()
 
 
-@pytest.mark.parametrize("storage,mixed", [("f32", False), ("f16s", False), ("f32", True), ("f64", True)])
+@pytest.mark.parametrize("storage,mixed", [("f32", False), ("f16s", False), ("f16c", True), ("f32", True), ("f64", True)])
 def test_batch_step_matches_single_blocks(storage, mixed):
     ref, bat = _blocks(storage, mixed), _blocks(storage, mixed)
     try:

@@ -98,7 +98,7 @@ def test_config0_cavity_100_1000_steps(mode):
     assert nbit == 0
 
 
-@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2)])
+@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2), ("f16c", 1e-3)])
 def test_shifted_half_storage_tracks_f64(storage, tol):
     """Shifted FP16 / BF16 storage against the FP64 oracle of the same case."""
     case64, p64 = O.cavity_case("hw", 24, 24, 24, dtype=np.float64)
