@@ -20,13 +20,14 @@ D2Q9, D3Q19 = 9, 19
 STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64, STORE_F16C = range(5)
 STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64, "f16c": STORE_F16C}
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK,
- DYN_SMAG_MRT) = range(8)
+ DYN_SMAG_MRT, DYN_BGK_FORCE) = range(9)
 HWBC_NORMAL, HWBC_MOVING = 0, 1
 
 # Every symbol include/flb_abi.h declares (checked by the CPU test suite).
 EXPORTS = (
     "flb_last_error", "flb_abi_version", "flb_block_create", "flb_block_destroy", "flb_get_info",
     "flb_synchronize", "flb_set_tasks", "flb_set_halfway_bc", "flb_set_mrt_rates", "flb_set_omega",
+    "flb_set_force",
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
     "flb_upload_pops_f64", "flb_download_pops_f32", "flb_download_pops_f64", "flb_init_equilibrium",
     "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
@@ -105,6 +106,7 @@ def lib() -> C.CDLL:
         "flb_set_halfway_bc": [P, C.POINTER(HWBC), C.c_int],
         "flb_set_mrt_rates": [P, C.POINTER(C.c_double), C.c_int],
         "flb_set_omega": [P, C.c_double],
+        "flb_set_force": [P, C.POINTER(C.c_double), C.c_int],
         "flb_upload_flags": [P, C.POINTER(C.c_uint8)],
         "flb_set_field_rho": [P, C.POINTER(C.c_double)],
         "flb_set_field_u": [P, C.POINTER(C.c_double)],

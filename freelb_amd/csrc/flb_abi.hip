@@ -199,8 +199,13 @@ KArgs make_args(const flb_block_s* b) {
     a.bcvoid[k] = b->bcs[k].void_mask;
     a.bckind[k] = b->bcs[k].kind;
   }
+  a.bcany = 0;
+  for (int k = 0; k < b->nbc; ++k) a.bcany |= b->bcs[k].cell_mask;
+  a.links = b->d_links ? b->d_links + flb::kFrontPad : nullptr;
+  if (!a.links) a.bcany = 0;
   a.omega = b->desc.omega;
   for (int j = 0; j < 19; ++j) a.rates[j] = b->rates[j];
+  for (int d = 0; d < 3; ++d) a.force[d] = b->force[d];
   return a;
 }
 
@@ -224,7 +229,8 @@ void to_slots(const flb_block_s* b, const U* dense, T* slots, T fill) {
 int family_of(const flb_block_s* b) {
   for (int t = 0; t < b->ntasks; ++t) {
     const int d = b->tasks[t].dynamics;
-    if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT) return flb::FAM_FULL;
+    if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT || d == FLB_DYN_BGK_FORCE)
+      return flb::FAM_FULL;
   }
   return flb::FAM_BGK;
 }
@@ -266,6 +272,19 @@ int check_layout(const flb_block_s* b) {
                         "(void mask 0x%02x) of halfway boundary %d",
                         x, y, z, fl, task, b->bcs[k].void_mask, k);
       }
+  return FLB_OK;
+}
+
+int ensure_links(flb_block_s* b) {
+  if (b->nbc == 0 || !b->flags_set) return FLB_OK;
+  if (!b->links_dirty && b->d_links) return FLB_OK;
+  if (!b->d_links) {
+    if (hipMalloc(reinterpret_cast<void**>(&b->d_links), size_t(b->stride) * sizeof(uint64_t)) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "wall-link allocation failed");
+  }
+  KArgs a = make_args(b);
+  HIP_TRY(flb::launch_links(b->desc.q, a, b->d_links + flb::kFrontPad, b->stream));
+  b->links_dirty = false;
   return FLB_OK;
 }
 
@@ -421,7 +440,7 @@ int flb_block_destroy(flb_block b) {
   DeviceGuard g(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   else (void)hipDeviceSynchronize();
-  void* ptrs[] = {b->d_pops, b->d_flags, b->d_rho, b->d_u, b->d_partials, b->d_red, b->d_stage};
+  void* ptrs[] = {b->d_pops, b->d_flags, b->d_rho, b->d_u, b->d_partials, b->d_red, b->d_stage, b->d_links};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -461,7 +480,7 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
   if (n < 0 || n > flb::kMaxTasks) return fail(FLB_ERR_ARG, "task count %d outside [0, %d]", n, flb::kMaxTasks);
   if (n > 0 && !tasks) return fail(FLB_ERR_ARG, "null task table");
   for (int t = 0; t < n; ++t) {
-    if (tasks[t].dynamics > FLB_DYN_SMAG_MRT)
+    if (tasks[t].dynamics > FLB_DYN_BGK_FORCE)
       return fail(FLB_ERR_ARG, "task %d: unknown dynamics %d", t, tasks[t].dynamics);
     if (tasks[t].mask == 0) return fail(FLB_ERR_ARG, "task %d: zero mask never matches", t);
   }
@@ -491,6 +510,7 @@ int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
   }
   for (int k = 0; k < n; ++k) b->bcs[k] = bcs[k];
   b->nbc = n;
+  b->links_dirty = true;
   return FLB_OK;
 }
 
@@ -503,6 +523,15 @@ int flb_set_mrt_rates(flb_block b, const double* rates, int q) {
   }
   if (q != b->desc.q) return fail(FLB_ERR_ARG, "expected %d rates, got %d", b->desc.q, q);
   for (int j = 0; j < q; ++j) b->rates[j] = std::isnan(rates[j]) ? -1.0 : rates[j];
+  return FLB_OK;
+}
+
+int flb_set_force(flb_block b, const double* force, int d) {
+  CHECK_BLOCK(b);
+  b->version++;
+  if (!force) return fail(FLB_ERR_ARG, "null force");
+  if (d != b->d) return fail(FLB_ERR_ARG, "expected %d force components, got %d", b->d, d);
+  for (int a = 0; a < 3; ++a) b->force[a] = a < d ? force[a] : 0.0;
   return FLB_OK;
 }
 
@@ -532,6 +561,7 @@ int flb_upload_flags(flb_block b, const uint8_t* flags) {
   }
   std::vector<uint8_t> slots(size_t(b->stride));
   to_slots<uint8_t, uint8_t>(b, b->h_flags.data(), slots.data(), uint8_t(0));
+  b->links_dirty = true;
   HIP_TRY(hipMemcpyAsync(b->d_flags, slots.data(), size_t(b->stride), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return FLB_OK;
@@ -634,6 +664,8 @@ int flb_init_equilibrium(flb_block b, const double* rhou) {
 
 static int launch_one(flb_block b) {
   if (!b->flags_set) return fail(FLB_ERR_STATE, "flags not uploaded");
+  const int rc = ensure_links(b);
+  if (rc) return rc;
   KArgs a = make_args(b);
   HIP_TRY(launch_step(b, family_of(b), a));
   return FLB_OK;

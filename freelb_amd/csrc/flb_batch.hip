@@ -108,7 +108,9 @@ int refresh(flb_batch_s* t, bool force) {
   t->h_wave0.assign(1, 0);
   t->cells = 0;
   for (size_t k = 0; k < t->blocks.size(); ++k) {
-    const flb_block_s* b = t->blocks[k];
+    flb_block_s* b = t->blocks[k];
+    const int rc = ensure_links(b);
+    if (rc) return rc;
     t->h_tab[k] = batch_args(b);
     const KArgs& a = t->h_tab[k];
     t->h_wave0.push_back(t->h_wave0.back() + waves_of(a));

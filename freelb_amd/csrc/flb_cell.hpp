@@ -161,6 +161,92 @@ __device__ __forceinline__ void bgk_with(R (&f)[Q], R rho, const R (&u)[3], R om
   for (int i = 0; i < Q; ++i) f[i] = omega * e[i] + _omega * f[i];
 }
 
+// moment::forcerhoU (forcerhoUImpl<D3Q19> moment.ur.h:639-663, <D2Q9>
+// 564-574): u = (sum c f + F/2) / rho, each component divided on its own.
+template <class R>
+__device__ __forceinline__ void force_rhou(const R (&c)[19], const R (&F)[3], R& rho, R (&u)[3]) {
+  rho = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] + c[11] + c[12] +
+        c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+  u[0] = (c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16] + F[0] * R(0.5)) / rho;
+  u[1] = (c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18] + F[1] * R(0.5)) / rho;
+  u[2] = (c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18] + F[2] * R(0.5)) / rho;
+}
+template <class R>
+__device__ __forceinline__ void force_rhou(const R (&c)[9], const R (&F)[3], R& rho, R (&u)[3]) {
+  rho = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+  u[0] = (c[1] - c[2] + c[5] - c[6] + c[7] - c[8] + F[0] * R(0.5)) / rho;
+  u[1] = (c[3] - c[4] + c[5] - c[6] - c[7] + c[8] + F[1] * R(0.5)) / rho;
+  u[2] = R(0);
+}
+
+// force::ForcePopImpl (the D3Q19 / D2Q9 forms of force.ur.h; generic form
+// force.h:29-36): Fi = w_i InvCs2 sum_a F_a (c_ia - u_a + InvCs2 (c_i.u) c_ia),
+// with the zero components of c_i left out as the unrolled form does.
+template <class R>
+__device__ __forceinline__ void force_pop(const R (&u)[3], const R (&F)[3], R (&Fi)[19]) {
+  const R I2 = R(3);
+  const R u0 = u[0], u1 = u[1], u2 = u[2], F0 = F[0], F1 = F[1], F2 = F[2];
+  const R T1_u0 = R(1) - u0, _T1_u0 = R(-1) - u0, _u0 = -u0;
+  const R T1_u1 = R(1) - u1, _T1_u1 = R(-1) - u1, _u1 = -u1;
+  const R T1_u2 = R(1) - u2, _T1_u2 = R(-1) - u2, _u2 = -u2;
+  const R c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u2), c7 = I2 * (u0 + u1), c9 = I2 * (u0 + u2),
+          c11 = I2 * (u1 + u2), c13 = I2 * (u0 - u1), c15 = I2 * (u0 - u2), c17 = I2 * (u1 - u2);
+  using L = Lat<19>;
+  Fi[0] = L::template w<R>(0) * I2 * (-(F0 * u0 + F1 * u1 + F2 * u2));
+  Fi[1] = L::template w<R>(1) * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
+  Fi[2] = L::template w<R>(2) * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
+  Fi[3] = L::template w<R>(3) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3) + F2 * (_u2));
+  Fi[4] = L::template w<R>(4) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3) + F2 * (_u2));
+  Fi[5] = L::template w<R>(5) * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (T1_u2 + c5));
+  Fi[6] = L::template w<R>(6) * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (_T1_u2 + c5));
+  Fi[7] = L::template w<R>(7) * I2 * (F0 * (T1_u0 + c7) + F1 * (T1_u1 + c7) + F2 * (_u2));
+  Fi[8] = L::template w<R>(8) * I2 * (F0 * (_T1_u0 + c7) + F1 * (_T1_u1 + c7) + F2 * (_u2));
+  Fi[9] = L::template w<R>(9) * I2 * (F0 * (T1_u0 + c9) + F1 * (_u1) + F2 * (T1_u2 + c9));
+  Fi[10] = L::template w<R>(10) * I2 * (F0 * (_T1_u0 + c9) + F1 * (_u1) + F2 * (_T1_u2 + c9));
+  Fi[11] = L::template w<R>(11) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c11) + F2 * (T1_u2 + c11));
+  Fi[12] = L::template w<R>(12) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c11) + F2 * (_T1_u2 + c11));
+  Fi[13] = L::template w<R>(13) * I2 * (F0 * (T1_u0 + c13) + F1 * (_T1_u1 - c13) + F2 * (_u2));
+  Fi[14] = L::template w<R>(14) * I2 * (F0 * (_T1_u0 + c13) + F1 * (T1_u1 - c13) + F2 * (_u2));
+  Fi[15] = L::template w<R>(15) * I2 * (F0 * (T1_u0 + c15) + F1 * (_u1) + F2 * (_T1_u2 - c15));
+  Fi[16] = L::template w<R>(16) * I2 * (F0 * (_T1_u0 + c15) + F1 * (_u1) + F2 * (T1_u2 - c15));
+  Fi[17] = L::template w<R>(17) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c17) + F2 * (_T1_u2 - c17));
+  Fi[18] = L::template w<R>(18) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c17) + F2 * (T1_u2 - c17));
+}
+template <class R>
+__device__ __forceinline__ void force_pop(const R (&u)[3], const R (&F)[3], R (&Fi)[9]) {
+  const R I2 = R(3);
+  const R u0 = u[0], u1 = u[1], F0 = F[0], F1 = F[1];
+  const R T1_u0 = R(1) - u0, _T1_u0 = R(-1) - u0, _u0 = -u0;
+  const R T1_u1 = R(1) - u1, _T1_u1 = R(-1) - u1, _u1 = -u1;
+  const R c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u0 + u1), c7 = I2 * (u0 - u1);
+  using L = Lat<9>;
+  Fi[0] = L::template w<R>(0) * I2 * (-(F0 * u0 + F1 * u1));
+  Fi[1] = L::template w<R>(1) * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1));
+  Fi[2] = L::template w<R>(2) * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1));
+  Fi[3] = L::template w<R>(3) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3));
+  Fi[4] = L::template w<R>(4) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3));
+  Fi[5] = L::template w<R>(5) * I2 * (F0 * (T1_u0 + c5) + F1 * (T1_u1 + c5));
+  Fi[6] = L::template w<R>(6) * I2 * (F0 * (_T1_u0 + c5) + F1 * (_T1_u1 + c5));
+  Fi[7] = L::template w<R>(7) * I2 * (F0 * (T1_u0 + c7) + F1 * (_T1_u1 - c7));
+  Fi[8] = L::template w<R>(8) * I2 * (F0 * (_T1_u0 + c7) + F1 * (T1_u1 - c7));
+}
+
+// collision::BGKForce (collision.h:68-87): omega feq + (1 - omega) f +
+// (1 - omega/2) Fi, fOmega formed as FreeLB's BlockLattice does
+// (block_lattice.hh:33-34).
+template <int Q, class R>
+__device__ __forceinline__ void bgk_force(R (&f)[Q], const R (&F)[3], R omega, R _omega) {
+  R rho, u[3];
+  force_rhou(f, F, rho, u);
+  R Fi[Q];
+  force_pop(u, F, Fi);
+  R e[Q];
+  feq(e, rho, u);
+  const R fomega = R(1) - R(0.5) * omega;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = omega * e[i] + _omega * f[i] + fomega * Fi[i];
+}
+
 // collision::BounceBack (collision.ur.h:44-93).
 template <int Q, class R>
 __device__ __forceinline__ void bounceback(R (&f)[Q]) {

@@ -40,6 +40,7 @@ struct flb_block_s {
   flb_hwbc bcs[flb::kMaxBc]{};
   int nbc = 0;
   double rates[19]{};
+  double force[3]{};
   int64_t step = 0;
   int parity = 0;
   int pending = 0;  // 1: cell dynamics applied, stream not yet called
@@ -47,6 +48,8 @@ struct flb_block_s {
                 // 1/2/4 row form with that many cells per thread
   int wpb = 4;   // wavefronts per workgroup
   uint64_t version = 0;  // bumped by every change a batch table must see
+  uint64_t* d_links = nullptr;  // precomputed halfway wall links (slot layout)
+  bool links_dirty = true;      // flags or halfway managers changed since d_links was built
 };
 
 namespace flbh {
@@ -78,6 +81,9 @@ struct DeviceGuard {
 };
 
 KArgs make_args(const flb_block_s* b);
+// Builds the block's wall-link words when flags or halfway managers changed
+// (enqueued on the block's stream); a no-op without halfway managers.
+int ensure_links(flb_block_s* b);
 // True when rows of ex cells fill less than 90 % of their wavefronts' lanes.
 bool linear_form(int ex);
 int family_of(const flb_block_s* b);

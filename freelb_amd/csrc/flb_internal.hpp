@@ -61,8 +61,11 @@ struct KArgs {
   uint8_t bcmask[kMaxBc];
   uint8_t bcvoid[kMaxBc];
   uint8_t bckind[kMaxBc];
+  uint8_t bcany;         // OR of bcmask: cells that may have halfway links
+  const uint64_t* links; // slot 0 of the precomputed wall links (normal | moving << 32)
   double omega;
   double rates[19];      // MRT rates; < 0 or NaN marks the shear rates
+  double force[3];       // constant body-force density of DYN_BGK_FORCE
 };
 
 // Launchers, one set per storage kind (defined in flb_inst_*.hip).
@@ -86,6 +89,9 @@ FLB_DECLARE_LAUNCHERS(f16s)
 FLB_DECLARE_LAUNCHERS(bf16s)
 FLB_DECLARE_LAUNCHERS(f64)
 FLB_DECLARE_LAUNCHERS(f16c)
+
+// Halfway wall links of every updated cell into links (slot layout), see k_links.
+hipError_t launch_links(int q, const KArgs& a, uint64_t* links, hipStream_t s);
 
 // float4 copy of n4 16-byte elements, one element per thread (bandwidth calibration).
 hipError_t launch_copy_probe(void* dst, const void* src, size_t n4, hipStream_t s);

@@ -53,7 +53,8 @@ enum : int {
   DYN_BB_MW = 4,
   DYN_MRT = 5,
   DYN_SMAG_BGK = 6,
-  DYN_SMAG_MRT = 7
+  DYN_SMAG_MRT = 7,
+  DYN_BGK_FORCE = 8
 };
 enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1 };
 
@@ -183,6 +184,11 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
       break;
     default:
       if constexpr (FAM == FAM_FULL) {
+        if (dyn == DYN_BGK_FORCE) {
+          const R F[3] = {R(a.force[0]), R(a.force[1]), R(a.force[2])};
+          bgk_force<Q>(f, F, om, _om);
+          break;
+        }
         rhou(f, rho, u);
         R eo = om;
         if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
@@ -341,7 +347,12 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   const int dyn = match_task(a, fl, cs);
   if (dyn == DYN_NONE) return;
   unsigned ln = 0u, lm = 0u;
-  if (PAR == 1 && a.nbc) wall_links<Q, PER>(a, x, y, z, s0, fl, ln, lm);
+  if (PAR == 1 && (fl & a.bcany)) {
+    // halfway links of a boundary cell, precomputed from the flags (k_links)
+    const uint64_t m = a.links[s0];
+    ln = uint32_t(m);
+    lm = uint32_t(m >> 32);
+  }
   const unsigned lk = ln | lm;
   R f[Q];
 #pragma unroll
@@ -852,6 +863,28 @@ __global__ __launch_bounds__(kMaxThreads) void k_init_feq(KArgs a, const typenam
   feq(e, rhou_in[id], u);
 #pragma unroll
   for (int i = 0; i < Q; ++i) arr<S>(a, i)[slot] = S::template to<Q>(e[i], i);
+}
+
+// ---------------------------------------------------------------------------
+// Halfway wall links of every updated cell, bit k of the low (high) word set
+// when the neighbour x + c_k is a wall of a normal (moving) halfway manager
+// (0 for cells no manager applies to). Computed once per flag or manager
+// change, so the odd step reads one word per boundary cell instead of
+// probing its q - 1 neighbours' flags.
+// ---------------------------------------------------------------------------
+template <int Q, bool PER>
+__global__ __launch_bounds__(kMaxThreads) void k_links(KArgs a, uint64_t* __restrict__ links) {
+  const int64_t k = int64_t(blockIdx.x) * kMaxThreads + threadIdx.x;
+  const int64_t n = int64_t(a.ex) * a.ey * a.ez;
+  if (k >= n) return;
+  const int c0 = int(k % a.ex);
+  const int64_t r = k / a.ex;
+  const int y = a.ys + int(r % a.ey), z = a.zs + int(r / a.ey);
+  const int s0 = c0 + a.pitch * y + int(a.plane) * z;
+  unsigned ln = 0u, lm = 0u;
+  const uint8_t fl = a.flags[s0];
+  if (fl & a.bcany) wall_links<Q, PER>(a, a.xs + c0, y, z, s0, fl, ln, lm);
+  links[s0] = (uint64_t(lm) << 32) | ln;
 }
 
 // ---------------------------------------------------------------------------

@@ -3,15 +3,20 @@
 // 1 void / 2 fluid / 4 wall / 8 moving lid, a TaskSelector of Key_TypePairs,
 // ApplyCellDynamics + Stream (+ BM.Apply in the halfway mode) per step.
 //
-//   cavity3d <fw|hw> N steps [out.bin]
+//   cavity3d <fw|hw|force|batch> N steps [out.bin]
 //
 // fw: walls collide as BounceBack / BounceBackMovingWall (the published run);
-// hw: every cell BGK, halfway bounce-back and moving wall after streaming.
+// hw: every cell BGK, halfway bounce-back and moving wall after streaming;
+// force: as hw with BGKForce<forcerhoU, SecondOrder, ConstForce> and the
+//   constant force (1e-4, -5e-5, 2e-5);
+// batch: three hw cavities stepped together by a BlockLatticeManager (one
+//   launch per step); out.bin holds the first one.
 // Prints MLUPS; with out.bin, writes rho and u (float32, 4*N values, SoA).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -23,7 +28,7 @@ using CELL = flb::Cell<T, LatSet>;
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s <fw|hw> N steps [out.bin]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <fw|hw|force|batch> N steps [out.bin]\n", argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
@@ -71,6 +76,60 @@ int main(int argc, char** argv) {
     for (int s = 1; s < steps; ++s) {
       NSLattice.ApplyCellDynamics<NSTask>(flags);
       NSLattice.Stream();
+    }
+    flb::check(flb_synchronize(NSLattice.handle()));
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  } else if (mode == "batch") {
+    using AllTask =
+        flb::tmp::Key_TypePair<std::uint8_t(AABBFlag | BouncebackFlag | BBMovingWallFlag), BGKrhoU>;
+    using NSTask = flb::TaskSelector<std::uint8_t, CELL, AllTask>;
+    using LAT = flb::BlockLattice<T, LatSet>;
+    flb::BlockLattice<T, LatSet> B1(m, m, m, omega), B2(m, m, m, omega);
+    std::vector<LAT*> lats{&NSLattice, &B1, &B2};
+    std::vector<std::unique_ptr<flb::BBLikeFixedBlockBdManager<flb::bounceback::normal<CELL>, LAT>>> bb;
+    std::vector<std::unique_ptr<flb::BBLikeFixedBlockBdManager<flb::bounceback::movingwall<CELL>, LAT>>> mw;
+    for (LAT* L : lats) {
+      if (L != &NSLattice) {
+        L->setRhoField(std::vector<double>(N, 1.0));
+        L->setVelocityField(u);
+        L->initUniform(1.0);
+      }
+      bb.emplace_back(new flb::BBLikeFixedBlockBdManager<flb::bounceback::normal<CELL>, LAT>(
+          "NS_BB", *L, BouncebackFlag, VoidFlag));
+      mw.emplace_back(new flb::BBLikeFixedBlockBdManager<flb::bounceback::movingwall<CELL>, LAT>(
+          "NS_BBMW", *L, BBMovingWallFlag, VoidFlag));
+      L->ApplyCellDynamics<NSTask>(flags);  // first step per block: uploads flags and tasks
+      L->Stream();
+    }
+    flb::BlockLatticeManager<T, LatSet> Manager(lats);
+    Manager.synchronize();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int s = 1; s < steps; ++s) {
+      Manager.ApplyCellDynamics();
+      Manager.Stream();
+    }
+    Manager.synchronize();
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 3.0;
+  } else if (mode == "force") {
+    using ForceS = flb::force::ConstForce<CELL>;
+    using BGKF = flb::collision::BGKForce<flb::moment::forcerhoU<CELL, ForceS>,
+                                          flb::equilibrium::SecondOrder<CELL>, ForceS>;
+    using AllTask = flb::tmp::Key_TypePair<std::uint8_t(AABBFlag | BouncebackFlag | BBMovingWallFlag), BGKF>;
+    using NSTask = flb::TaskSelector<std::uint8_t, CELL, AllTask>;
+    using LAT = flb::BlockLattice<T, LatSet>;
+    NSLattice.setConstForce({double(std::strtof("1e-4", nullptr)), double(std::strtof("-5e-5", nullptr)),
+                             double(std::strtof("2e-5", nullptr))});
+    flb::BBLikeFixedBlockBdManager<flb::bounceback::normal<CELL>, LAT> NS_BB("NS_BB", NSLattice,
+                                                                            BouncebackFlag, VoidFlag);
+    flb::BBLikeFixedBlockBdManager<flb::bounceback::movingwall<CELL>, LAT> NS_BBMW(
+        "NS_BBMW", NSLattice, BBMovingWallFlag, VoidFlag);
+    flb::BlockBoundaryManager BM(&NS_BB, &NS_BBMW);
+    flb::check(flb_synchronize(NSLattice.handle()));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < steps; ++s) {
+      NSLattice.ApplyCellDynamics<NSTask>(flags);
+      NSLattice.Stream();
+      BM.Apply();
     }
     flb::check(flb_synchronize(NSLattice.handle()));
     ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

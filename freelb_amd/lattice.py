@@ -126,6 +126,11 @@ class BlockLattice:
         r = np.ascontiguousarray(rates, dtype=np.float64)
         abi.check(abi.lib().flb_set_mrt_rates(self._h, abi.ptr(r, C.c_double), r.size))
 
+    def set_force(self, force) -> None:
+        """CONSTFORCE of the BGKForce cells (force::ConstForce, lbm/force.h:86-107)."""
+        F = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1))
+        abi.check(abi.lib().flb_set_force(self._h, abi.ptr(F, C.c_double), int(F.size)))
+
     def set_omega(self, omega: float) -> None:
         abi.check(abi.lib().flb_set_omega(self._h, float(omega)))
 

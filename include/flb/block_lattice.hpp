@@ -70,7 +70,14 @@ template <typename CELL, bool WriteToField = false>
 struct rhoU {};  // lbm/moment.h:381-398
 template <typename CELL>
 struct useFieldrhoU {};  // lbm/moment.h:71-90
+template <typename CELL, typename ForceScheme, bool WriteToField = false>
+struct forcerhoU {};  // lbm/moment.h:441-480
 }  // namespace moment
+
+namespace force {
+template <typename CELL>
+struct ConstForce {};  // lbm/force.h:86-107 (the CONSTFORCE data field)
+}  // namespace force
 
 namespace equilibrium {
 template <typename CELL>
@@ -103,6 +110,12 @@ struct MRT_Feq_RhoU {  // lbm/collisionMRT.h:132-190
 template <typename MomentaScheme, typename EquilibriumScheme, bool WriteToField = false>
 struct SmagorinskyBGK {  // lbm/collisionLES.h:29-67
   static constexpr uint8_t kind = FLB_DYN_SMAG_BGK;
+};
+template <typename MomentaScheme, typename EquilibriumScheme, typename ForceScheme>
+struct BGKForce;  // lbm/collision.h:61-90
+template <typename CELL, typename EQ>
+struct BGKForce<moment::forcerhoU<CELL, force::ConstForce<CELL>>, EQ, force::ConstForce<CELL>> {
+  static constexpr uint8_t kind = FLB_DYN_BGK_FORCE;
 };
 // MRT relaxation whose shear rates take the Smagorinsky omega (no FreeLB form).
 template <typename CELL>
@@ -142,7 +155,13 @@ struct movingwall {
 };
 }  // namespace bounceback
 
-enum class Storage { F32 = FLB_STORE_F32, F16S = FLB_STORE_F16S, BF16S = FLB_STORE_BF16S, F64 = FLB_STORE_F64 };
+enum class Storage {
+  F32 = FLB_STORE_F32,
+  F16S = FLB_STORE_F16S,
+  BF16S = FLB_STORE_BF16S,
+  F64 = FLB_STORE_F64,
+  F16C = FLB_STORE_F16C
+};
 
 // ---- one block on the GPU ----------------------------------------------------------
 // T is the host type of populations and fields (float for F32 / F16S / BF16S
@@ -189,6 +208,8 @@ class BlockLattice {
   void setRhoField(const std::vector<double>& rho) { check(flb_set_field_rho(h_, rho.data())); }
   void setVelocityField(const std::vector<double>& u) { check(flb_set_field_u(h_, u.data())); }
   void setSmagorinsky(float cs) { smagorinsky_ = cs; }
+  // CONSTFORCE data field of BGKForce<..., ConstForce> cells (d components).
+  void setConstForce(const std::vector<double>& F) { check(flb_set_force(h_, F.data(), int(F.size()))); }
 
   // f_i = rho0 * w_i everywhere (BlockLatticeManager ctor, block_lattice.hh:676-681).
   void initUniform(double rho0 = 1.0) {
@@ -251,6 +272,50 @@ class BlockLattice {
   std::vector<flb_task> tasks_;
   std::vector<flb_hwbc> bcs_;
   float smagorinsky_ = 0.0f;
+};
+
+// ---- many blocks: BlockLatticeManager (data_struct/block_lattice.hh:744-778) -------
+// Steps all of its blocks with one launch per step (flb_batch_*). The blocks
+// keep their own set-up (flags, tasks, fields) and read-out; FreeLB's
+// per-block ApplyCellDynamics / Stream pair becomes ApplyCellDynamics +
+// Stream on the manager, and many steps can be recorded once and replayed.
+template <typename T, typename LatSet>
+class BlockLatticeManager {
+ public:
+  explicit BlockLatticeManager(std::vector<BlockLattice<T, LatSet>*> blocks, void* stream = nullptr)
+      : blocks_(std::move(blocks)) {
+    std::vector<flb_block> h;
+    for (auto* b : blocks_) h.push_back(b->handle());
+    check(flb_batch_create(h.data(), int(h.size()), stream, &t_));
+  }
+  ~BlockLatticeManager() { flb_batch_destroy(t_); }
+  BlockLatticeManager(const BlockLatticeManager&) = delete;
+  BlockLatticeManager& operator=(const BlockLatticeManager&) = delete;
+
+  // One step of every block; the blocks' task tables must already be set
+  // (by a first per-block ApplyCellDynamics<Selector> + Stream, or set-up).
+  void ApplyCellDynamics() { pending_ = true; }
+  void Stream() {
+    if (!pending_) throw Error(FLB_ERR_STATE, "flb: Stream without ApplyCellDynamics");
+    pending_ = false;
+    check(flb_batch_step(t_, 1));
+  }
+  void step(int n) { check(flb_batch_step(t_, n)); }
+  void capture(int n) { check(flb_batch_capture(t_, n)); }
+  void replay() { check(flb_batch_replay(t_)); }
+  flb_launch_cfg autotune(int steps = 10) {
+    flb_launch_cfg best{};
+    double ms = 0.0;
+    check(flb_batch_autotune(t_, steps, &best, &ms));
+    return best;
+  }
+  void synchronize() { check(flb_batch_synchronize(t_)); }
+  BlockLattice<T, LatSet>& getBlockLat(size_t k) { return *blocks_[k]; }
+
+ private:
+  std::vector<BlockLattice<T, LatSet>*> blocks_;
+  flb_batch t_ = nullptr;
+  bool pending_ = false;
 };
 
 // ---- halfway managers (boundary/boundary.h:75-129, basic_boundary.h:331-348) -------

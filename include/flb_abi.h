@@ -86,7 +86,10 @@ enum {
   FLB_DYN_MRT = 5,           /* collision::MRT_Feq_RhoU (D2Q9 as lbm/collisionMRT.ur.h:45-129;
                                 D3Q19 on the d'Humieres 2002 basis, no reference form) */
   FLB_DYN_SMAG_BGK = 6,      /* collision::SmagorinskyBGK                                   lbm/collisionLES.h:29-67 */
-  FLB_DYN_SMAG_MRT = 7       /* MRT whose shear rates take the Smagorinsky omega (no reference form) */
+  FLB_DYN_SMAG_MRT = 7,      /* MRT whose shear rates take the Smagorinsky omega (no reference form) */
+  FLB_DYN_BGK_FORCE = 8      /* collision::BGKForce<moment::forcerhoU, SecondOrder, force::ConstForce>
+                                lbm/collision.h:61-90, lbm/force.h:29-36, lbm/moment.h:401-438;
+                                the force is the block's constant (flb_set_force) */
 };
 
 /* Halfway bounce-back kinds: bounceback::normal / bounceback::movingwall
@@ -151,6 +154,9 @@ int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n);   /* n <= 4 */
  * take omega. NULL restores the defaults. */
 int flb_set_mrt_rates(flb_block b, const double* rates, int q);
 int flb_set_omega(flb_block b, double omega);
+/* Constant body-force density of FLB_DYN_BGK_FORCE cells, d values: FreeLB's
+ * CONSTFORCE data field read by force::ConstForce (lbm/force.h:86-107). */
+int flb_set_force(flb_block b, const double* force, int d);
 int flb_upload_flags(flb_block b, const uint8_t* flags);           /* N bytes */
 int flb_set_field_rho(flb_block b, const double* rho);             /* N values  */
 int flb_set_field_u(flb_block b, const double* u);                 /* d*N values, SoA */

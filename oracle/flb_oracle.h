@@ -35,7 +35,9 @@ enum {
   FLBO_DYN_BOUNCEBACK_MW = 4,  /* collision::BounceBackMovingWall                 collision.ur.h:130-187 */
   FLBO_DYN_MRT = 5,            /* collision::MRT_Feq_RhoU                         collisionMRT.h:132-190 */
   FLBO_DYN_SMAG_BGK = 6,       /* collision::SmagorinskyBGK                       collisionLES.h:29-67 */
-  FLBO_DYN_SMAG_MRT = 7        /* MRT with the Smagorinsky effective shear rate (no reference form) */
+  FLBO_DYN_SMAG_MRT = 7,       /* MRT with the Smagorinsky effective shear rate (no reference form) */
+  FLBO_DYN_BGK_FORCE = 8       /* collision::BGKForce<forcerhoU, SecondOrder, ConstForce>
+                                  collision.h:61-90, moment.ur.h:564-574/639-663, force.ur.h */
 };
 
 enum { FLBO_HWBC_NORMAL = 0, FLBO_HWBC_MOVING = 1 };
@@ -69,6 +71,7 @@ typedef struct {
   const double* mrt_rates;  /* q relaxation rates for the MRT kinds, or NULL for the defaults;
                                a NaN entry marks the shear rates that take omega */
   int nthreads;       /* OpenMP threads, <= 0 for the default */
+  double force[3];    /* CONSTFORCE data field of BGKForce (force density, lattice units) */
 } flbo_lattice;
 
 /* Run `steps` reference time steps (collide, PS stream, halfway BCs) in place

@@ -42,6 +42,71 @@ static void FN(rhou_cell)(int q, const REAL* c, REAL* rho, REAL* u) {
   }
 }
 
+/* moment::forcerhoU (forcerhoUImpl<D3Q19> moment.ur.h:639-663, <D2Q9> 564-574):
+ * u = (sum c f + F/2) / rho, each component divided separately. */
+static void FN(force_rhou_cell)(int q, const REAL* c, const REAL* F, REAL* rho, REAL* u) {
+  if (q == 19) {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] + c[11] +
+             c[12] + c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+    *rho = r;
+    u[0] = (c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16] + F[0] * (REAL)0.5) / r;
+    u[1] = (c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18] + F[1] * (REAL)0.5) / r;
+    u[2] = (c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18] + F[2] * (REAL)0.5) / r;
+  } else {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+    *rho = r;
+    u[0] = (c[1] - c[2] + c[5] - c[6] + c[7] - c[8] + F[0] * (REAL)0.5) / r;
+    u[1] = (c[3] - c[4] + c[5] - c[6] - c[7] + c[8] + F[1] * (REAL)0.5) / r;
+  }
+}
+
+/* force::ForcePopImpl<D3Q19> / <D2Q9> (force.ur.h, the unrolled form of
+ * force.h:29-36): Fi = w_i InvCs2 (F.(c_i - u) + InvCs2 (c_i.u)(c_i.F)),
+ * written per direction with the terms c_ia - u_a + InvCs2 (c_i.u) c_ia. */
+static void FN(force_pop_cell)(int q, const REAL* w, const REAL* u, const REAL* F, REAL* Fi) {
+  const REAL I2 = (REAL)3;
+  const REAL u0 = u[0], u1 = u[1], F0 = F[0], F1 = F[1];
+  const REAL T1_u0 = (REAL)1 - u0, _T1_u0 = (REAL)-1 - u0, _u0 = -u0;
+  const REAL T1_u1 = (REAL)1 - u1, _T1_u1 = (REAL)-1 - u1, _u1 = -u1;
+  if (q == 19) {
+    const REAL u2 = u[2], F2 = F[2];
+    const REAL T1_u2 = (REAL)1 - u2, _T1_u2 = (REAL)-1 - u2, _u2 = -u2;
+    const REAL c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u2), c7 = I2 * (u0 + u1),
+               c9 = I2 * (u0 + u2), c11 = I2 * (u1 + u2), c13 = I2 * (u0 - u1),
+               c15 = I2 * (u0 - u2), c17 = I2 * (u1 - u2);
+    Fi[0] = w[0] * I2 * (-(F0 * u0 + F1 * u1 + F2 * u2));
+    Fi[1] = w[1] * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
+    Fi[2] = w[2] * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
+    Fi[3] = w[3] * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3) + F2 * (_u2));
+    Fi[4] = w[4] * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3) + F2 * (_u2));
+    Fi[5] = w[5] * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (T1_u2 + c5));
+    Fi[6] = w[6] * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (_T1_u2 + c5));
+    Fi[7] = w[7] * I2 * (F0 * (T1_u0 + c7) + F1 * (T1_u1 + c7) + F2 * (_u2));
+    Fi[8] = w[8] * I2 * (F0 * (_T1_u0 + c7) + F1 * (_T1_u1 + c7) + F2 * (_u2));
+    Fi[9] = w[9] * I2 * (F0 * (T1_u0 + c9) + F1 * (_u1) + F2 * (T1_u2 + c9));
+    Fi[10] = w[10] * I2 * (F0 * (_T1_u0 + c9) + F1 * (_u1) + F2 * (_T1_u2 + c9));
+    Fi[11] = w[11] * I2 * (F0 * (_u0) + F1 * (T1_u1 + c11) + F2 * (T1_u2 + c11));
+    Fi[12] = w[12] * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c11) + F2 * (_T1_u2 + c11));
+    Fi[13] = w[13] * I2 * (F0 * (T1_u0 + c13) + F1 * (_T1_u1 - c13) + F2 * (_u2));
+    Fi[14] = w[14] * I2 * (F0 * (_T1_u0 + c13) + F1 * (T1_u1 - c13) + F2 * (_u2));
+    Fi[15] = w[15] * I2 * (F0 * (T1_u0 + c15) + F1 * (_u1) + F2 * (_T1_u2 - c15));
+    Fi[16] = w[16] * I2 * (F0 * (_T1_u0 + c15) + F1 * (_u1) + F2 * (T1_u2 - c15));
+    Fi[17] = w[17] * I2 * (F0 * (_u0) + F1 * (T1_u1 + c17) + F2 * (_T1_u2 - c17));
+    Fi[18] = w[18] * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c17) + F2 * (T1_u2 - c17));
+  } else {
+    const REAL c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u0 + u1), c7 = I2 * (u0 - u1);
+    Fi[0] = w[0] * I2 * (-(F0 * u0 + F1 * u1));
+    Fi[1] = w[1] * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1));
+    Fi[2] = w[2] * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1));
+    Fi[3] = w[3] * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3));
+    Fi[4] = w[4] * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3));
+    Fi[5] = w[5] * I2 * (F0 * (T1_u0 + c5) + F1 * (T1_u1 + c5));
+    Fi[6] = w[6] * I2 * (F0 * (_T1_u0 + c5) + F1 * (_T1_u1 + c5));
+    Fi[7] = w[7] * I2 * (F0 * (T1_u0 + c7) + F1 * (_T1_u1 - c7));
+    Fi[8] = w[8] * I2 * (F0 * (_T1_u0 + c7) + F1 * (T1_u1 - c7));
+  }
+}
+
 /* equilibrium::SecondOrderImpl<D3Q19> (equilibrium.ur.h:165-214) and <D2Q9>
  * (equilibrium.ur.h:65-94). */
 static void FN(feq_cell)(int q, const REAL* w, REAL rho, const REAL* u, REAL* feq) {
@@ -199,6 +264,18 @@ static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const RE
     case FLBO_DYN_SMAG_MRT:
       FN(mrt_cell)(q, w, omega, t, L, c, id, N);
       break;
+    case FLBO_DYN_BGK_FORCE: {
+      /* collision::BGKForce (collision.h:68-87): forcerhoU, ForcePop, feq, then
+       * omega feq + (1 - omega) f + (1 - omega/2) Fi (fOmega, block_lattice.hh:33-34) */
+      const REAL F[3] = {(REAL)L->force[0], (REAL)L->force[1], (REAL)L->force[2]};
+      REAL Fi[19];
+      FN(force_rhou_cell)(q, c, F, &rho, u);
+      FN(force_pop_cell)(q, w, u, F, Fi);
+      FN(feq_cell)(q, w, rho, u, feq);
+      const REAL fomega = (REAL)1 - (REAL)0.5 * omega;
+      for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i] + fomega * Fi[i];
+      break;
+    }
     default:
       break;
   }

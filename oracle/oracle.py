@@ -14,11 +14,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK, DYN_SMAG_MRT = range(8)
+(DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK, DYN_SMAG_MRT,
+ DYN_BGK_FORCE) = range(9)
 HWBC_NORMAL, HWBC_MOVING = 0, 1
 
 # Cavity flags of the reference drivers (benchmarks/cavity3d/cavity3d.cpp:121-124).
 VOID, AABB, BB, BBMW = 1, 2, 4, 8
+# Body-force density of the 'force' cavity (float values, as FreeLB's CONSTFORCE<float>);
+# the reference harness takes the same decimal strings.
+FORCE_STR = ("1e-4", "-5e-5", "2e-5")
+FORCE = tuple(np.float32(v) for v in FORCE_STR)
 
 
 class _Task(C.Structure):
@@ -37,7 +42,7 @@ class _Lattice(C.Structure):
                 ("tasks", _Task * 8), ("nbc", C.c_int), ("bcs", _HWBC * 4),
                 ("omega", C.c_double), ("rho_field", C.POINTER(C.c_double)),
                 ("u_field", C.POINTER(C.c_double)), ("mrt_rates", C.POINTER(C.c_double)),
-                ("nthreads", C.c_int)]
+                ("nthreads", C.c_int), ("force", C.c_double * 3)]
 
 
 _lib = None
@@ -73,7 +78,7 @@ class Lattice:
     """One block of the reference lattice: flags, task table, halfway BCs, fields."""
 
     def __init__(self, q, dims, flags, tasks, bcs=(), omega=1.0, rho_field=None, u_field=None,
-                 mrt_rates=None, overlap=1, nthreads=0):
+                 mrt_rates=None, overlap=1, nthreads=0, force=(0.0, 0.0, 0.0)):
         self.q = q
         self.nx, self.ny, self.nz = dims
         self.N = self.nx * self.ny * self.nz
@@ -106,6 +111,9 @@ class Lattice:
                 self._keep.append(a)
                 setattr(L, name, _ptr(a, C.c_double))
         L.nthreads = nthreads
+        self.force = tuple(float(v) for v in force) + (0.0,) * (3 - len(force))
+        for a in range(3):
+            L.force[a] = self.force[a]
         self.L = L
 
     @property
@@ -167,6 +175,8 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
     mode 'hwfield' : examples/couette2d/couette2d.cpp:183-216 (walls BGK<useFieldrhoU>).
     mode 'mrt'     : fullway walls, fluid MRT_Feq_RhoU (D2Q9 in FreeLB).
     mode 'smag'    : fullway walls, fluid SmagorinskyBGK with C_s = cs.
+    mode 'force'   : like 'hw' with BGKForce<forcerhoU, SecondOrder, ConstForce> and the
+                     constant force density FORCE (float values).
     """
     q = 19 if Nk > 0 else 9
     flags = cavity_flags(Ni, Nj, Nk)
@@ -193,6 +203,9 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
         tasks = [(AABB, DYN_SMAG_BGK, float(np.float32(cs))), (BB, DYN_BOUNCEBACK),
                  (BBMW, DYN_BOUNCEBACK_MW)]
         bcs = []
+    elif mode == "force":
+        tasks = [(AABB | BB | BBMW, DYN_BGK_FORCE)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
     else:
         raise ValueError(mode)
     if dtype == np.float64:
@@ -201,7 +214,9 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
         u_field[0, flags == BBMW] = ul
     else:
         om = float(omega)
-    lat = Lattice(q, dims, flags, tasks, bcs, omega=om, rho_field=rho_field, u_field=u_field)
+    force = [float(v) for v in FORCE[:d]] if mode == "force" else (0.0, 0.0, 0.0)
+    lat = Lattice(q, dims, flags, tasks, bcs, omega=om, rho_field=rho_field, u_field=u_field,
+                  force=force)
     w = weights(q, dtype)
     pops = np.repeat(w[:, None], N, axis=1).astype(dtype)
     pops = np.ascontiguousarray(pops)

@@ -8,7 +8,7 @@
 // the HIP kernels can be compared against it.
 //
 // Usage:
-//   cavity_ref <d3q19|d2q9> <fw|hw|hwfield|mrt|smag> Ni Nj Nk steps RT KineVisc Uwall out.bin [Cs]
+//   cavity_ref <d3q19|d2q9> <fw|hw|hwfield|mrt|smag|force> Ni Nj Nk steps RT KineVisc Uwall out.bin [Cs]
 // (Nk is ignored for d2q9.)
 //
 // Modes
@@ -24,6 +24,9 @@
 //             lbm/collisionMRT.ur.h:45-129).
 //   smag    : fw walls, fluid cells collide with SmagorinskyBGK<rhoU>
 //             (lbm/collisionLES.h:29-67), SMAGORINSKY data field = Cs.
+//   force   : like hw, every non-void cell collides with
+//             BGKForce<forcerhoU, SecondOrder, ConstForce> (lbm/collision.h:61-90),
+//             CONSTFORCE data field = (1e-4, -5e-5, 2e-5) (d components).
 //
 // Output layout (little endian):
 //   char[8]  "FLBREF01"
@@ -114,14 +117,16 @@ void dump(const Params& p, BLM& lat, FM& flagfm, T omega, T ulat, int nz) {
   std::fclose(f);
 }
 
-template <typename LatSet, int D, bool SMAG>
+template <typename LatSet, int D, bool SMAG, bool FORCE = false>
 int run(const Params& p) {
   BaseConverter<T> BaseConv(LatSet::cs2);
   BaseConv.ConvertFromRT(T(1), p.RT, T(0.001), T(p.Ni), p.uwall, p.visc);
 
   using FIELDS = std::conditional_t<SMAG,
       TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>, SMAGORINSKY<T>>,
-      TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>>>;
+      std::conditional_t<FORCE,
+        TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>, CONSTFORCE<T, LatSet::d>>,
+        TypePack<RHO<T>, VELOCITY<T, LatSet::d>, POP<T, LatSet::q>>>>;
   using CELL = Cell<T, LatSet, FIELDS>;
 
   Vector<T, D> U_Wall{};
@@ -139,6 +144,12 @@ int run(const Params& p) {
     auto make_init = [&]() {
       if constexpr (SMAG) {
         return ValuePack(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{}, p.cs);
+      } else if constexpr (FORCE) {
+        const T fv[3] = {std::strtof("1e-4", nullptr), std::strtof("-5e-5", nullptr),
+                         std::strtof("2e-5", nullptr)};
+        Vector<T, D> F{};
+        for (int a = 0; a < D; ++a) F[a] = fv[a];
+        return ValuePack(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{}, F);
       } else {
         return ValuePack(BaseConv.getLatRhoInit(), Vector<T, D>{}, T{});
       }
@@ -200,6 +211,20 @@ int run(const Params& p) {
         NSLattice.Stream();
         BM.Apply();
       }
+    } else if (p.mode == "force") {
+      if constexpr (FORCE) {
+        using ForceS = force::ConstForce<CELL>;
+        using BGKF = collision::BGKForce<moment::forcerhoU<CELL, ForceS>, equilibrium::SecondOrder<CELL>,
+                                         ForceS>;
+        using AllTask =
+          tmp::Key_TypePair<std::uint8_t(AABBFlag | BouncebackFlag | BBMovingWallFlag), BGKF>;
+        using NSTask = TaskSelector<std::uint8_t, CELL, AllTask>;
+        for (int s = 0; s < p.steps; ++s) {
+          NSLattice.template ApplyCellDynamics<NSTask>(FlagFM);
+          NSLattice.Stream();
+          BM.Apply();
+        }
+      }
     } else if (p.mode == "hwfield") {
       using BulkTask = tmp::Key_TypePair<AABBFlag, BGKrhoU>;
       using WallTask =
@@ -259,8 +284,11 @@ int main(int argc, char** argv) {
   Thread_Num = omp_get_max_threads();
 #endif
   const bool smag = p.mode == "smag";
-  if (p.lattice == "d3q19") return smag ? run<D3Q19<T>, 3, true>(p) : run<D3Q19<T>, 3, false>(p);
-  if (p.lattice == "d2q9") return smag ? run<D2Q9<T>, 2, true>(p) : run<D2Q9<T>, 2, false>(p);
+  const bool frc = p.mode == "force";
+  if (p.lattice == "d3q19")
+    return smag ? run<D3Q19<T>, 3, true>(p) : (frc ? run<D3Q19<T>, 3, false, true>(p) : run<D3Q19<T>, 3, false>(p));
+  if (p.lattice == "d2q9")
+    return smag ? run<D2Q9<T>, 2, true>(p) : (frc ? run<D2Q9<T>, 2, false, true>(p) : run<D2Q9<T>, 2, false>(p));
   std::fprintf(stderr, "unknown lattice %s\n", p.lattice.c_str());
   return 2;
 }

@@ -8,7 +8,7 @@ the populations, the flag field and FreeLB's own moment::rhoU read-out.
 This script only runs that binary and stores its outputs; it needs the
 reference sources, so it runs in the build container, not on the GPU box.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures)
 """
 from __future__ import annotations
 
@@ -32,6 +32,7 @@ SMALL = [
     ("d2q9", "fw", 16, 13, 1, 100), ("d2q9", "hw", 16, 13, 1, 100), ("d2q9", "hwfield", 16, 13, 1, 100),
     ("d2q9", "mrt", 16, 13, 1, 100),
     ("d2q9", "smag", 16, 13, 1, 100), ("d3q19", "smag", 8, 7, 6, 40),
+    ("d3q19", "force", 8, 7, 6, 40), ("d2q9", "force", 16, 13, 1, 100),
 ]
 CS = "0.17"  # Smagorinsky constant of the smag fixtures
 # the 100^3 cavity, 1000 steps: hashes and a mid-plane slice
@@ -66,16 +67,21 @@ def run(lat, mode, ni, nj, nk, steps, tmp):
 
 
 def main():
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if not os.path.exists(EXE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
         for lat, mode, ni, nj, nk, steps in SMALL:
+            if only and mode != only:
+                continue
             r = run(lat, mode, ni, nj, nk, steps, tmp)
             name = f"cavity_{lat}_{mode}_{ni}x{nj}x{nk}_{steps}.npz"
             np.savez_compressed(os.path.join(HERE, name), flags=r["flags"], pops=r["pops"],
                                 rhou=r["rhou"], dims=np.array(r["dims"]), steps=r["steps"],
                                 omega=r["omega"], uw=r["uw"])
             print("wrote", name, flush=True)
+        if only:
+            return
         meta = {}
         for lat, mode, ni, nj, nk, steps in LARGE:
             r = run(lat, mode, ni, nj, nk, steps, tmp)

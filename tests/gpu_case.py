@@ -18,6 +18,8 @@ def gpu_lattice(case, storage="f32", pops=None):
         lat.set_field_u(case.u_field)
     if case.mrt_rates is not None:
         lat.set_mrt_rates(case.mrt_rates)
+    if any(getattr(case, "force", ())):
+        lat.set_force(case.force[:(3 if case.q == 19 else 2)])
     if pops is not None:
         lat.upload_pops(pops)
     return lat

@@ -23,7 +23,7 @@ def test_adapter_compiles_with_host_compiler():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["hw", "fw"])
+@pytest.mark.parametrize("mode", ["hw", "fw", "force", "batch"])
 def test_adapter_driver_matches_oracle(mode):
     from oracle import oracle as O
     n, steps = 20, 57
@@ -33,7 +33,7 @@ def test_adapter_driver_matches_oracle(mode):
                            text=True, timeout=120)
         print(r.stdout, flush=True)
         rhou = np.fromfile(out, dtype=np.float32).reshape(4, -1)
-    case, p = O.cavity_case(mode, n, n, n)
+    case, p = O.cavity_case("hw" if mode == "batch" else mode, n, n, n)
     case.run(p, steps)
     ref = case.rhou(p)
     sel = case.flags == O.AABB if mode == "fw" else case.flags != O.VOID

@@ -29,7 +29,9 @@ def test_row_and_linear_forms_agree_with_oracle(monkeypatch, storage, mode, dims
     case, p = O.cavity_case(mode, *dims, dtype=dt)
     p0 = p.copy()
     case.run(p, 9)
-    inner = interior_mask(case)
+    # fullway walls: FreeLB's wall cells also hold values streamed in from the
+    # void layer, which the kernels do not keep; compare the fluid cells
+    inner = interior_mask(case) if mode == "hw" else case.flags == O.AABB
     for form in (None, "-1,4", "-3,4", "-3,1"):
         lat = _with_form(monkeypatch, form, lambda: gpu_lattice(case, storage, p0))
         try:
@@ -45,7 +47,7 @@ def test_d2q9_linear_form(monkeypatch, mode):
     case, p = O.cavity_case(mode, 40, 13, 0, dtype=np.float32)
     p0 = p.copy()
     case.run(p, 11)
-    inner = interior_mask(case)
+    inner = interior_mask(case) if mode == "hw" else case.flags == O.AABB
     lat = _with_form(monkeypatch, "-3,2", lambda: gpu_lattice(case, "f32", p0))
     try:
         lat.step(11)
