@@ -8,6 +8,7 @@ FreeLB's BlockLattice interface.
 """
 from .abi import FLBError  # noqa: F401
 from .adjoint import Adjoint  # noqa: F401
-from .lattice import BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair, weights  # noqa: F401
+from .lattice import (BlockHalo, BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair,  # noqa: F401
+                      weights)
 
-__all__ = ["Adjoint", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]
+__all__ = ["Adjoint", "BlockHalo", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]

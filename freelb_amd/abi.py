@@ -33,7 +33,8 @@ EXPORTS = (
     "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
     "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
     "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
-    "flb_batch_synchronize", "flb_batch_autotune", "flb_adjoint_create", "flb_adjoint_destroy",
+    "flb_batch_synchronize", "flb_batch_autotune", "flb_batch_set_halo", "flb_set_ghost_overlap",
+    "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange", "flb_adjoint_create", "flb_adjoint_destroy",
     "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
     "flb_adjoint_timing", "flb_bandwidth_probe",
@@ -130,6 +131,13 @@ def lib() -> C.CDLL:
         "flb_batch_replay": [P],
         "flb_batch_synchronize": [P],
         "flb_batch_autotune": [P, C.c_int, C.POINTER(LaunchCfg), C.POINTER(C.c_double)],
+        "flb_batch_set_halo": [P, P],
+        "flb_set_ghost_overlap": [P, C.c_int],
+        "flb_halo_create": [C.POINTER(P), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                            C.POINTER(C.c_int), P, C.POINTER(P)],
+        "flb_halo_destroy": [P],
+        "flb_halo_info": [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)],
+        "flb_halo_exchange": [P],
         "flb_adjoint_create": [C.POINTER(AdjointDesc), C.POINTER(P)],
         "flb_adjoint_destroy": [P],
         "flb_adjoint_set_solid": [P, C.POINTER(C.c_uint8)],

@@ -259,7 +259,7 @@ int check_layout(const flb_block_s* b) {
           }
         }
         if (dyn == FLB_DYN_NONE) continue;
-        if (outer)
+        if (outer && !b->ghost_overlap)
           return fail(FLB_ERR_LAYOUT,
                       "cell (%d,%d,%d) in the overlap layer has flag 0x%02x matching task %d",
                       x, y, z, fl, task);
@@ -524,6 +524,16 @@ int flb_set_mrt_rates(flb_block b, const double* rates, int q) {
   if (q != b->desc.q) return fail(FLB_ERR_ARG, "expected %d rates, got %d", b->desc.q, q);
   for (int j = 0; j < q; ++j) b->rates[j] = std::isnan(rates[j]) ? -1.0 : rates[j];
   return FLB_OK;
+}
+
+int flb_set_ghost_overlap(flb_block b, int enable) {
+  CHECK_BLOCK(b);
+  b->version++;
+  const bool saved = b->ghost_overlap;
+  b->ghost_overlap = enable != 0;
+  const int rc = check_layout(b);
+  if (rc != FLB_OK) b->ghost_overlap = saved;
+  return rc;
 }
 
 int flb_set_force(flb_block b, const double* force, int d) {

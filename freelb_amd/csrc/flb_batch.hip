@@ -38,6 +38,7 @@ struct flb_batch_s {
   int64_t cells = 0;
   int wpb = 4;
   int grid = 0;  // 0: one workgroup per group of wpb wavefronts
+  flb_halo halo = nullptr;  // exchanged after every step when set
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
@@ -255,10 +256,27 @@ int flb_batch_step(flb_batch t, int n) {
   int rc = check_state(t, &parity);
   if (rc) return rc;
   if ((rc = refresh(t, false))) return rc;
+  if (t->halo && (rc = halo_refresh(t->halo))) return rc;
   if ((rc = join_in(t))) return rc;
-  for (int s = 0; s < n; ++s) HIP_TRY(launch_batch(t, (parity + s) & 1, t->stream));
+  for (int s = 0; s < n; ++s) {
+    HIP_TRY(launch_batch(t, (parity + s) & 1, t->stream));
+    if (t->halo) HIP_TRY(halo_launch(t->halo, (parity + s + 1) & 1, t->stream));
+  }
   advance(t, n);
   return join_out(t);
+}
+
+int flb_batch_set_halo(flb_batch t, flb_halo h) {
+  CHECK_BATCH(t);
+  if (h && h->stream != t->stream) return fail(FLB_ERR_ARG, "halo and batch run on different streams");
+  if (h) {
+    for (flb_block_s* b : h->blocks)
+      if (std::find(t->blocks.begin(), t->blocks.end(), b) == t->blocks.end())
+        return fail(FLB_ERR_ARG, "halo block is not part of the batch");
+  }
+  drop_graph(t);
+  t->halo = h;
+  return FLB_OK;
 }
 
 int flb_batch_capture(flb_batch t, int n) {
@@ -273,7 +291,11 @@ int flb_batch_capture(flb_batch t, int n) {
   hipStream_t cs = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  for (int s = 0; e == hipSuccess && s < n; ++s) e = launch_batch(t, (parity + s) & 1, cs);
+  if (t->halo && (rc = halo_refresh(t->halo))) return rc;
+  for (int s = 0; e == hipSuccess && s < n; ++s) {
+    e = launch_batch(t, (parity + s) & 1, cs);
+    if (e == hipSuccess && t->halo) e = halo_launch(t->halo, (parity + s + 1) & 1, cs);
+  }
   hipGraph_t graph = nullptr;
   const hipError_t e2 = hipStreamEndCapture(cs, &graph);
   (void)hipStreamDestroy(cs);
@@ -304,6 +326,7 @@ int flb_batch_replay(flb_batch t) {
     return fail(FLB_ERR_STATE, "graph was captured at parity %d, blocks are at %d (capture an even step count to replay repeatedly)",
                 t->graph_parity, parity);
   if ((rc = refresh(t, false))) return rc;
+  if (t->halo && (rc = halo_refresh(t->halo))) return rc;
   if (t->family != t->graph_family || t->per != t->graph_per)
     return fail(FLB_ERR_STATE, "block dynamics changed kernel family since capture: capture again");
   if ((rc = join_in(t))) return rc;
@@ -363,9 +386,11 @@ int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* bes
       t->wpb = wpb;
       t->grid = m == 0 ? 0 : cus * m * (4 / wpb);
       hipError_t e = hipSuccess;
-      for (int s = 0; s < 2 && e == hipSuccess; ++s) e = launch_batch(t, (parity + s) & 1, t->stream);
-      if (e == hipSuccess) e = hipEventRecord(e0, t->stream);
-      for (int s = 0; s < steps && e == hipSuccess; ++s) e = launch_batch(t, (parity + s) & 1, t->stream);
+      for (int s = 0; s < 2 + steps && e == hipSuccess; ++s) {
+        if (s == 2) e = hipEventRecord(e0, t->stream);
+        if (e == hipSuccess) e = launch_batch(t, (parity + s) & 1, t->stream);
+        if (e == hipSuccess && t->halo) e = halo_launch(t->halo, (parity + s + 1) & 1, t->stream);
+      }
       if (e == hipSuccess) e = hipEventRecord(e1, t->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.0f;

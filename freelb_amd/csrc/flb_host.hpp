@@ -50,6 +50,22 @@ struct flb_block_s {
   uint64_t version = 0;  // bumped by every change a batch table must see
   uint64_t* d_links = nullptr;  // precomputed halfway wall links (slot layout)
   bool links_dirty = true;      // flags or halfway managers changed since d_links was built
+  bool ghost_overlap = false;   // the overlap layer mirrors neighbouring blocks (flb_halo)
+};
+
+struct flb_halo_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<flb_block> blocks;
+  std::vector<uint64_t> versions;
+  int q = 19;
+  size_t elem = 4;
+  std::vector<flb::KArgs> h_tab;
+  std::vector<flb::HaloItem> h_items;
+  std::vector<int64_t> h_cell0;
+  flb::KArgs* d_tab = nullptr;
+  flb::HaloItem* d_items = nullptr;
+  int64_t* d_cell0 = nullptr;
 };
 
 namespace flbh {
@@ -84,6 +100,13 @@ KArgs make_args(const flb_block_s* b);
 // Builds the block's wall-link words when flags or halfway managers changed
 // (enqueued on the block's stream); a no-op without halfway managers.
 int ensure_links(flb_block_s* b);
+
+// Block halo (flb_halo.hip): enqueue the exchange that follows a step leaving
+// the blocks at `parity`; refresh the device copy of the blocks' arguments;
+// the common parity of the blocks.
+hipError_t halo_launch(const flb_halo_s* h, int parity, hipStream_t s);
+int halo_refresh(flb_halo_s* h);
+int halo_parity(const flb_halo_s* h, int* parity);
 // True when rows of ex cells fill less than 90 % of their wavefronts' lanes.
 bool linear_form(int ex);
 int family_of(const flb_block_s* b);

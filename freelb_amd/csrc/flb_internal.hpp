@@ -90,6 +90,15 @@ FLB_DECLARE_LAUNCHERS(bf16s)
 FLB_DECLARE_LAUNCHERS(f64)
 FLB_DECLARE_LAUNCHERS(f16c)
 
+// Ghost cells of block a that mirror updated cells of block b (block halo):
+// a box in a-local coordinates and the shift from a-local to b-local ones.
+struct HaloItem {
+  int a, b;
+  int gx, gy, gz;
+  int nx, ny, nz;
+  int sx, sy, sz;
+};
+
 // Halfway wall links of every updated cell into links (slot layout), see k_links.
 hipError_t launch_links(int q, const KArgs& a, uint64_t* links, hipStream_t s);
 

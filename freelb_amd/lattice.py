@@ -126,6 +126,10 @@ class BlockLattice:
         r = np.ascontiguousarray(rates, dtype=np.float64)
         abi.check(abi.lib().flb_set_mrt_rates(self._h, abi.ptr(r, C.c_double), r.size))
 
+    def set_ghost_overlap(self, enable: bool = True) -> None:
+        """The overlap layer mirrors neighbouring blocks (see BlockHalo)."""
+        abi.check(abi.lib().flb_set_ghost_overlap(self._h, int(bool(enable))))
+
     def set_force(self, force) -> None:
         """CONSTFORCE of the BGKForce cells (force::ConstForce, lbm/force.h:86-107)."""
         F = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1))
@@ -208,19 +212,67 @@ class BlockLattice:
         return out
 
 
+class BlockHalo:
+    """Same-level communication between the blocks of one domain (FreeLB's
+    normalCommunicate / normalFullCommunicate, data_struct/block_lattice.hh:1669-1703).
+    origins[k] is the global index of block k's local cell (0, 0, 0); a
+    periodic domain axis of `extent` cells wraps from block to block.
+    exchange() follows every step of all blocks."""
+
+    def __init__(self, blocks: Sequence[BlockLattice], origins, periodic=(False, False, False),
+                 extent=(0, 0, 0), stream=None):
+        L = abi.lib()
+        self.blocks = list(blocks)
+        arr = (C.c_void_p * len(self.blocks))(*[b.handle for b in self.blocks])
+        org = np.ascontiguousarray(np.asarray(origins, dtype=np.int32).reshape(-1, 3))
+        assert org.shape[0] == len(self.blocks)
+        per = np.ascontiguousarray(np.asarray(periodic, dtype=np.int32).reshape(3))
+        ext = np.ascontiguousarray(np.asarray(extent, dtype=np.int32).reshape(3))
+        h = C.c_void_p()
+        abi.check(L.flb_halo_create(arr, len(self.blocks), abi.ptr(org, C.c_int), abi.ptr(per, C.c_int),
+                                    abi.ptr(ext, C.c_int), stream, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, cells = C.c_int(), C.c_int64()
+        abi.check(abi.lib().flb_halo_info(self._h, C.byref(n), C.byref(cells)))
+        return {"items": n.value, "ghost_cells": cells.value}
+
+    def exchange(self) -> None:
+        abi.check(abi.lib().flb_halo_exchange(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.lib().flb_halo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BlockLatticeManager:
     """Many BlockLattice blocks stepped together, one launch per time step
     (FreeLB's BlockLatticeManager::ApplyCellDynamics / Stream over its blocks,
     data_struct/block_lattice.hh:744-778). The blocks keep their own set-up
     and read-out and must stay open while the manager lives."""
 
-    def __init__(self, blocks: Sequence[BlockLattice], stream=None):
+    def __init__(self, blocks: Sequence[BlockLattice], stream=None, halo: "BlockHalo" = None):
         L = abi.lib()
         self.blocks = list(blocks)
         arr = (C.c_void_p * len(self.blocks))(*[b.handle for b in self.blocks])
         h = C.c_void_p()
         abi.check(L.flb_batch_create(arr, len(self.blocks), stream, C.byref(h)))
         self._h = h
+        self.halo = halo
+        if halo is not None:
+            abi.check(L.flb_batch_set_halo(self._h, halo.handle))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

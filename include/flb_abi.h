@@ -205,6 +205,7 @@ int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]);
  * pending. Work on the batch is ordered after earlier work on the blocks'
  * streams and before later work on them. */
 typedef struct flb_batch_s* flb_batch;
+typedef struct flb_halo_s* flb_halo;
 typedef struct {
   int waves_per_group;  /* wavefronts per workgroup, 1..4 */
   int grid;             /* workgroups; 0 = one per group of wavefronts (no persistence) */
@@ -226,6 +227,30 @@ int flb_batch_synchronize(flb_batch t);
  * compute unit, persistent) and keeps the fastest. Populations are restored
  * afterwards, so the blocks' state is unchanged. */
 int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* best_ms_per_step);
+
+/* With a halo set, every batch step is followed by its exchange (and a
+ * recorded graph holds both). NULL removes it. Drops a recorded graph. */
+int flb_batch_set_halo(flb_batch t, flb_halo h);
+
+/* ---- block halo ----------------------------------------------------------------
+ * Blocks of one domain, each with a one-cell overlap layer that mirrors the
+ * neighbouring blocks' cells: FreeLB's same-level communication
+ * (BlockLatticeManager::normalCommunicate / normalFullCommunicate,
+ * data_struct/block_lattice.hh:1669-1703; cell pairs as BlockGeometry3D::InitComm,
+ * geometry/block_geometry3d.hh:531-605). origins[3k..3k+2] is the global
+ * index of block k's local cell (0,0,0) (its overlap corner); periodic axes of
+ * the domain (extent cells long) wrap from block to block. Blocks are not
+ * periodic themselves, share the halo's stream, and need
+ * flb_set_ghost_overlap (cells of the overlap layer may then match tasks;
+ * they are not updated, only mirrored). flb_halo_exchange follows every step
+ * (all blocks at one parity): in the AA pattern it fills the ghost slots the
+ * next odd step reads, or returns the values an odd step pushed into them. */
+int flb_set_ghost_overlap(flb_block b, int enable);
+int flb_halo_create(const flb_block* blocks, int n, const int* origins, const int* periodic,
+                    const int* extent, void* stream, flb_halo* out);
+int flb_halo_destroy(flb_halo h);
+int flb_halo_info(flb_halo h, int* n_items, int64_t* n_ghost_cells);
+int flb_halo_exchange(flb_halo h);
 
 /* ---- differentiation -------------------------------------------------------
  * A D3Q19 box (nx*ny*nz cells, FreeLB index order, any axis periodic; cells
