@@ -303,7 +303,29 @@ __device__ __forceinline__ void st_pop(E* p, E v) {
   else *p = v;
 }
 
-template <int Q, class S, int PAR, int FAM, bool PER>
+// base + a byte offset held in 32 bits: with a wavefront-uniform base this is
+// the scalar-base + vector-offset form of a global access.
+template <class E>
+__device__ __forceinline__ E* at_bytes(E* base, uint32_t boff) {
+  return reinterpret_cast<E*>(reinterpret_cast<char*>(base) + boff);
+}
+template <class E>
+__device__ __forceinline__ const E* at_bytes(const E* base, uint32_t boff) {
+  return reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + boff);
+}
+
+// A pointer that is the same in every lane, made explicit (scalar registers):
+// the compiler then keeps row bases scalar instead of folding the lane offset
+// in first and redoing the 64-bit arithmetic per direction in vector registers.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+  return reinterpret_cast<T*>((uint64_t(hi) << 32) | lo);
+}
+
+template <int Q, class S, int PAR, int FAM, bool PER, bool UNI>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z);
 
 // Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
@@ -311,7 +333,6 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 // block's updated cells in x-fastest order instead.
 template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
-  int c0, y, z;
   if (a.vec == -3) {
     // linear form: wavefront gw takes updated cells 64 gw .. 64 gw + 63 in
     // x-fastest order, across row ends (short rows fill whole wavefronts)
@@ -319,27 +340,32 @@ __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
     const unsigned exy = unsigned(a.ex) * unsigned(a.ey);
     if (c >= exy * unsigned(a.ez)) return;
     const unsigned zz = c / exy, r = c - zz * exy, yy = r / unsigned(a.ex);
-    c0 = int(r - yy * unsigned(a.ex));
-    y = a.ys + int(yy);
-    z = a.zs + int(zz);
+    cell_update_at<Q, S, PAR, FAM, PER, false>(a, int(r - yy * unsigned(a.ex)), a.ys + int(yy),
+                                               a.zs + int(zz));
   } else {
     const int row = gw / a.wpr;
     if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
     const int wv = gw - row * a.wpr;
-    y = a.ys + row % a.ey;
-    z = a.zs + row / a.ey;
-    c0 = wv * kWave + lane;
+    const int c0 = wv * kWave + lane;
     if (c0 >= a.ex) return;
+    cell_update_at<Q, S, PAR, FAM, PER, true>(a, c0, a.ys + row % a.ey, a.zs + row / a.ey);
   }
-  cell_update_at<Q, S, PAR, FAM, PER>(a, c0, y, z);
 }
 
-// The update of the cell at x = xs + c0 in row (y, z).
-template <int Q, class S, int PAR, int FAM, bool PER>
+// The update of the cell at x = xs + c0 in row (y, z). UNI: the row is the
+// same for the whole wavefront, so each direction's row base is one scalar
+// pointer and a lane only adds its x; a wavefront without wall links then
+// takes the plain AA addresses, with no per-lane choice between targets.
+template <int Q, class S, int PAR, int FAM, bool PER, bool UNI>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z) {
   using R = typename S::R;
   using E = typename S::E;
+  if constexpr (UNI) {
+    y = __builtin_amdgcn_readfirstlane(y);
+    z = __builtin_amdgcn_readfirstlane(z);
+  }
   const int x = a.xs + c0;
+  const uint32_t boff = uint32_t(c0) * uint32_t(sizeof(E));  // byte offset of the cell in its row
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
   const uint8_t fl = a.flags[s0];
@@ -354,11 +380,20 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     lm = uint32_t(m >> 32);
   }
   const unsigned lk = ln | lm;
+  // plain addresses for every lane: a uniform row, no periodic x, no wall link
+  bool plain = false;
+  if constexpr (UNI && !PER && PAR == 1) plain = __builtin_amdgcn_ballot_w64(lk != 0u) == 0u;
   R f[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      f[i] = S::template from<Q>(ld_pop<PAR>((P + int64_t(i) * a.dpop) + c0), i);
+      const E* base = P + int64_t(i) * a.dpop;
+      if constexpr (UNI) base = uniform_ptr(base);
+      f[i] = S::template from<Q>(ld_pop<PAR>(at_bytes(base, boff)), i);
+    } else if (plain) {
+      const int o = opp(i);
+      const E* base = uniform_ptr(P + (int64_t(o) * a.dpop + yz_off<Q, false, true>(a, o, y, z) + Lat<Q>::cx(o)));
+      f[i] = S::template from<Q>(ld_pop<PAR>(at_bytes(base, boff)), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
       const int o = opp(i);
@@ -382,7 +417,12 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      st_pop<PAR>((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
+      E* base = P + int64_t(opp(i)) * a.dpop;
+      if constexpr (UNI) base = uniform_ptr(base);
+      st_pop<PAR>(at_bytes(base, boff), S::template to<Q>(f[i], i));
+    } else if (plain) {
+      E* base = uniform_ptr(P + (int64_t(i) * a.dpop + yz_off<Q, false, true>(a, i, y, z) + Lat<Q>::cx(i)));
+      st_pop<PAR>(at_bytes(base, boff), S::template to<Q>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
