@@ -108,19 +108,21 @@ def _vorticity(rhou, n):
 
 
 def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
-                    storages=("f32", "f16s", "bf16s", "f16c"), seed=2024, u0_amp=1e-3, timer=None):
-    """Errors of FP32 / FP16-shift / BF16-shift storage against an FP64 run of
-    the same cavity (D3Q19, halfway walls, lid 0.1): rho and u from
+                    storages=("f32", "f16s", "bf16s", "f16c"), seed=2024, u0_amp=1e-3, timer=None,
+                    u_lid=0.1, tau=None):
+    """Errors of FP32 / FP16-shift / BF16-shift / F16C storage against an FP64
+    run of the same cavity (D3Q19, halfway walls, lid u_lid): rho and u from
     flb_moments, vorticity by central differences. Start: rho = 1, u i.i.d.
-    U(-u0_amp, u0_amp) per component (numpy default_rng(seed)), f = feq."""
+    U(-u0_amp, u0_amp) per component (numpy default_rng(seed)), f = feq.
+    tau: BGK 0.53 and MRT+LES tau0 0.5005 unless given."""
     from freelb_amd.cases import VOID, cavity_block, cavity_flags
     live = cavity_flags(n) != VOID
-    out = {"cells": n ** 3, "steps": steps, "seed": seed, "u0_amp": u0_amp, "runs": {}}
+    out = {"cells": n ** 3, "steps": steps, "seed": seed, "u0_amp": u0_amp, "u_lid": u_lid, "runs": {}}
     for coll in collisions:
-        tau = 0.53 if coll == "bgk" else 0.5005
+        t = tau if tau is not None else (0.53 if coll == "bgk" else 0.5005)
         ref = None
         for s in ("f64",) + tuple(storages):
-            lat = cavity_block(n, storage=s, collision=coll, tau=tau, u0_amp=u0_amp, seed=seed)
+            lat = cavity_block(n, storage=s, collision=coll, tau=t, u0_amp=u0_amp, seed=seed, u_lid=u_lid)
             try:
                 t0 = time.perf_counter()
                 lat.step(steps)
@@ -129,7 +131,9 @@ def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
                 m = lat.moments()
             finally:
                 lat.close()
+            bpc = {"f64": 304, "f32": 152}.get(s, 76)
             rec = {"mlups": round(n ** 3 * steps / dt / 1e6, 1),
+                   "hbm_frac": round(n ** 3 * steps * bpc / dt / (HBM_PEAK_GBS * 1e9), 4),
                    "finite": bool(np.isfinite(m[:, live]).all())}
             if s == "f64":
                 ref = m
@@ -197,6 +201,94 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
             "per_block_launch_mlups": round(cells / (loop_ms / 1e3) / 1e6, 1),
             "launch": tuned, "setup_s": round(setup_s, 1),
             "finite": bool(np.isfinite(mass).all())}
+
+
+# algorithmic HBM bytes per cell and step of the FP32 adjoint kernels: the
+# forward push reads and writes 19 populations plus s (4 B) and the solid
+# byte; the adjoint step reads the stored state and 19 adjoint values,
+# writes 19 adjoint values, reads s and the solid byte and updates dJ/ds (8 B)
+ADJ_FWD_BYTES = 19 * 4 * 2 + 4 + 1
+ADJ_BWD_BYTES = 19 * 4 * 3 + 4 + 1 + 8
+
+
+def _adjoint_rates(cells, T, K, fwd_ms, bwd_ms):
+    """Cell updates per second of the forward pass, of the adjoint pass
+    (T adjoint steps plus the T - ceil(T/K) recomputed forward steps) and of
+    the whole gradient, with the bandwidth each pass reaches."""
+    rec = T - (-(-T // K)) if K > 0 else 0
+    fwd_b = cells * T * ADJ_FWD_BYTES
+    bwd_b = cells * (T * ADJ_BWD_BYTES + rec * ADJ_FWD_BYTES)
+    return {"forward_mlups": round(cells * T / fwd_ms / 1e3, 1),
+            "adjoint_mlups": round(cells * T / bwd_ms / 1e3, 1),
+            "gradient_mlups": round(cells * T / (fwd_ms + bwd_ms) / 1e3, 1),
+            "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+            "forward_gbs": round(fwd_b / fwd_ms / 1e6, 1), "backward_gbs": round(bwd_b / bwd_ms / 1e6, 1),
+            "forward_frac": round(fwd_b / fwd_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "backward_frac": round(bwd_b / bwd_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "recomputed_steps": rec}
+
+
+def adjoint_bench(T=800, K=28):
+    """The differentiable channel, 192 x 96 x 96 cells, periodic in x, walls in
+    y and z, body force 1e-6 along x, tau 0.6, Brinkman fraction from the
+    seed-7 smoothed field: FP32 forward of T steps with a checkpoint every K
+    steps, then the adjoint pass (second run timed; the first allocates)."""
+    from freelb_amd.cases import channel_adjoint
+    adj, s = channel_adjoint(storage="f32")
+    with adj:
+        adj.forward(T, K)
+        adj.backward()
+        J = adj.forward(T, K)
+        g = adj.backward()
+        fwd_ms, bwd_ms = adj.timing()
+    out = {"workload": f"channel 192x96x96, T {T}, checkpoint every {K}, FP32", "cells": int(s.size),
+           "J": J, "grad_finite": bool(np.isfinite(g).all())}
+    out.update(_adjoint_rates(s.size, T, K, fwd_ms, bwd_ms))
+    return out
+
+
+def adjoint_batch_bench(n=64, T=400, seed=11, fp64_check=False):
+    """n channels with seeded extents uniform in [64,32,32]..[128,64,64]
+    (numpy default_rng(seed)), each with its own seeded Brinkman field,
+    stepped as one batch (one launch per step over all boxes), T steps with
+    a checkpoint every round(sqrt(T)) steps, then the batched adjoint. With
+    fp64_check the same batch runs in FP64 and the FP32 objective and
+    gradient errors against it are reported."""
+    from freelb_amd.cases import channel_batch, channel_dims
+    K = int(round(T ** 0.5))
+    dims = channel_dims(n, seed=seed)
+    cells = int(sum(int(np.prod(d)) for d in dims))
+    t0 = time.perf_counter()
+    b, mem, _ = channel_batch(dims, storage="f32", seed=seed)
+    setup_s = time.perf_counter() - t0
+    try:
+        b.forward(T, K)
+        b.backward()
+        J32 = b.forward(T, K)
+        g32 = b.backward()
+        fwd_ms, bwd_ms = b.timing()
+    finally:
+        b.close()
+        for m in mem:
+            m.close()
+    out = {"workload": f"{n} channels, extents U[64..128]xU[32..64]xU[32..64] (seed {seed}), T {T}, "
+                       f"checkpoint every {K}, FP32, one launch per step", "boxes": n, "cells": cells,
+           "setup_s": round(setup_s, 1), "grad_finite": bool(all(np.isfinite(g).all() for g in g32))}
+    out.update(_adjoint_rates(cells, T, K, fwd_ms, bwd_ms))
+    if fp64_check:
+        b, mem, _ = channel_batch(dims, storage="f64", seed=seed)
+        try:
+            J64 = b.forward(T, K)
+            g64 = b.backward()
+        finally:
+            b.close()
+            for m in mem:
+                m.close()
+        out["J_max_rel_err_vs_fp64"] = float(np.max(np.abs(J32 - J64) / np.abs(J64)))
+        out["grad_max_err_vs_fp64"] = float(max(np.abs(a - c).max() / np.abs(c).max() for a, c in zip(g32, g64)))
+        out["grad_l2_rel_err_vs_fp64"] = float(max(np.linalg.norm(a - c) / np.linalg.norm(c)
+                                                   for a, c in zip(g32, g64)))
+    return out
 
 
 def profiled_traffic(storage):
@@ -268,12 +360,24 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=2000)
     ap.add_argument("--collisions", default="bgk,mrt_les", help="collisions of the precision study")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-block batched run")
+    ap.add_argument("--config4", action="store_true",
+                    help="512^3 MRT+LES cavity (Re 5000, lid 0.05), 5000 steps, f32/f16s/f16c vs FP64")
+    ap.add_argument("--no-cavity", action="store_true", help="with --config4: only the batched adjoint")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each GPU would run its own share of blocks; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    if args.config4:
+        # Re = u N / nu = 5000 with u = 0.05, N = 512: tau0 = 0.5 + 3 nu = 0.51536
+        if not args.no_cavity:
+            res = precision_study(n=512, steps=5000, collisions=("mrt_les",), storages=("f32", "f16s", "f16c"),
+                                  u0_amp=0.0, u_lid=0.05, tau=0.5 + 3 * 0.05 * 512 / 5000)
+            print(json.dumps({"config4_cavity": res}), flush=True)
+        print(json.dumps({"config4_adjoint_batch": adjoint_batch_bench(fp64_check=True)}), flush=True)
+        return
     if args.precision:
         st = tuple(s for s in args.storages.split(",") if s != "f64")
         res = precision_study(steps=args.precision_steps, collisions=tuple(args.collisions.split(",")),
@@ -325,6 +429,11 @@ def main():
     if not args.no_batch:
         line["batch"] = batch_bench(timer)
         print(f"# batch: {line['batch']}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench()
+        print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
+        line["adjoint_batch"] = adjoint_batch_bench()
+        print(f"# adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

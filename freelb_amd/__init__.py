@@ -7,8 +7,8 @@ kernels' sources (csrc/), the build recipe and a thin host-side mirror of
 FreeLB's BlockLattice interface.
 """
 from .abi import FLBError  # noqa: F401
-from .adjoint import Adjoint  # noqa: F401
+from .adjoint import Adjoint, AdjointBatch  # noqa: F401
 from .lattice import (BlockHalo, BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair,  # noqa: F401
                       weights)
 
-__all__ = ["Adjoint", "BlockHalo", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]
+__all__ = ["Adjoint", "AdjointBatch", "BlockHalo", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "weights", "FLBError"]

@@ -37,7 +37,8 @@ EXPORTS = (
     "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange", "flb_adjoint_create", "flb_adjoint_destroy",
     "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
-    "flb_adjoint_timing", "flb_bandwidth_probe",
+    "flb_adjoint_timing", "flb_adjoint_batch_create", "flb_adjoint_batch_destroy", "flb_adjoint_batch_forward",
+    "flb_adjoint_batch_backward", "flb_adjoint_batch_timing", "flb_bandwidth_probe",
 )
 
 
@@ -148,6 +149,11 @@ def lib() -> C.CDLL:
         "flb_adjoint_backward": [P, C.POINTER(C.c_double)],
         "flb_adjoint_state": [P, C.POINTER(C.c_double)],
         "flb_adjoint_timing": [P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "flb_adjoint_batch_create": [C.POINTER(P), C.c_int, P, C.POINTER(P)],
+        "flb_adjoint_batch_destroy": [P],
+        "flb_adjoint_batch_forward": [P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "flb_adjoint_batch_backward": [P, C.POINTER(C.c_double)],
+        "flb_adjoint_batch_timing": [P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "flb_bandwidth_probe": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():

@@ -97,3 +97,55 @@ class Adjoint:
         f, b = C.c_double(), C.c_double()
         abi.check(abi.lib().flb_adjoint_timing(self._h, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+
+class AdjointBatch:
+    """Several Adjoint boxes (any sizes, one storage kind) stepped together,
+    one launch per time step over all of their cells. Each member keeps its
+    own fields; after forward() every member also holds its own objective,
+    final state and recorded run."""
+
+    def __init__(self, members: Sequence[Adjoint], stream=None):
+        L = abi.lib()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h for m in self.members])
+        h = C.c_void_p()
+        abi.check(L.flb_adjoint_batch_create(arr, len(self.members), stream, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.lib().flb_adjoint_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def forward(self, steps: int, checkpoint_every: int = -1) -> np.ndarray:
+        J = np.empty(len(self.members), dtype=np.float64)
+        abi.check(abi.lib().flb_adjoint_batch_forward(self._h, int(steps), int(checkpoint_every),
+                                                      abi.ptr(J, C.c_double)))
+        return J
+
+    def backward(self) -> list:
+        g = np.empty(sum(m.N for m in self.members), dtype=np.float64)
+        abi.check(abi.lib().flb_adjoint_batch_backward(self._h, abi.ptr(g, C.c_double)))
+        out, off = [], 0
+        for m in self.members:
+            out.append(g[off:off + m.N].copy())
+            off += m.N
+        return out
+
+    def timing(self):
+        f, b = C.c_double(), C.c_double()
+        abi.check(abi.lib().flb_adjoint_batch_timing(self._h, C.byref(f), C.byref(b)))
+        return f.value, b.value

@@ -61,14 +61,13 @@ def cavity_block(n, storage: str = "f32", collision: str = "bgk", tau: float = 0
         u[0, flags == LID] = u_lid
         lat.set_field_rho(np.ones(N))
         lat.set_field_u(u)
+        # f = feq(1, u0) (with u0 = 0 this is exactly w_i, FreeLB's initial state)
+        u0 = np.zeros((3, N))
         if u0_amp:
             rng = np.random.default_rng(seed)
-            u0 = np.zeros((3, N))
             live = flags != VOID
             u0[:, live] = rng.uniform(-u0_amp, u0_amp, size=(3, int(live.sum())))
-            lat.init_equilibrium(np.ones(N), u0)
-        else:
-            lat.init_uniform(1.0)
+        lat.init_equilibrium(np.ones(N), u0)
     except Exception:
         lat.close()
         raise
@@ -107,3 +106,30 @@ def channel_adjoint(dims=(192, 96, 96), storage: str = "f32", tau: float = 0.6, 
         adj.close()
         raise
     return adj, s
+
+
+def channel_dims(n: int = 64, lo=(64, 32, 32), hi=(128, 64, 64), seed: int = 11):
+    """n seeded channel extents, each axis uniform in [lo, hi] (numpy
+    default_rng(seed), integers drawn axis by axis, box by box)."""
+    rng = np.random.default_rng(seed)
+    return [tuple(int(rng.integers(a, b + 1)) for a, b in zip(lo, hi)) for _ in range(n)]
+
+
+def channel_batch(dims_list, storage: str = "f32", tau: float = 0.6, g: float = 1e-6,
+                  alpha: float = 0.1, seed: int = 11, sigma: float = 4.0):
+    """One channel_adjoint box per extent in dims_list (box k draws its solid
+    fraction with seed 1000*seed + k) and the AdjointBatch over all of them.
+    Returns (batch, members, fractions); closing is the caller's."""
+    from .adjoint import AdjointBatch
+    members, fields = [], []
+    try:
+        for k, dims in enumerate(dims_list):
+            adj, s = channel_adjoint(dims, storage, tau, g, alpha, seed=1000 * seed + k, sigma=sigma)
+            members.append(adj)
+            fields.append(s)
+        batch = AdjointBatch(members)
+    except Exception:
+        for m in members:
+            m.close()
+        raise
+    return batch, members, fields

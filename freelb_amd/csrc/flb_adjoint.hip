@@ -135,9 +135,8 @@ __device__ __forceinline__ void cell_xyz(int64_t id, int nx, int ny, int& x, int
 
 // ---- forward step: state src -> dst (push, halfway bounce-back) -------------
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
-                                                  R* __restrict__ dst) {
-  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+__device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restrict__ src,
+                                         R* __restrict__ dst, int64_t id) {
   if (id >= a.N || a.solid[id]) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
@@ -162,10 +161,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restr
   }
 }
 
-// ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restrict__ f,
-                                                  double* __restrict__ out) {
+__global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
+                                                  R* __restrict__ dst) {
+  fwd_cell(a, src, dst, int64_t(blockIdx.x) * kThreads + threadIdx.x);
+}
+
+// ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
+// one workgroup: the plane sum into out[0]
+template <class R>
+__device__ __forceinline__ void obj_plane(const AdjArgs<R>& a, const R* __restrict__ f,
+                                          double* __restrict__ out) {
   __shared__ double part[kThreads];
   double acc = 0.0;
   const int64_t plane = int64_t(a.ny) * a.nz;
@@ -187,6 +193,12 @@ __global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restr
   if (threadIdx.x == 0) out[0] = part[0];
 }
 
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restrict__ f,
+                                                  double* __restrict__ out) {
+  obj_plane(a, f, out);
+}
+
 // ---- equilibrium initial state ----------------------------------------------
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_init(AdjArgs<R> a, const R* __restrict__ rhou,
@@ -202,20 +214,23 @@ __global__ __launch_bounds__(kThreads) void k_init(AdjArgs<R> a, const R* __rest
 
 // ---- adjoint seed: lambda^T = dJ/df^T -----------------------------------------
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_seed(AdjArgs<R> a, R* __restrict__ lam) {
-  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+__device__ __forceinline__ void seed_cell(const AdjArgs<R>& a, R* __restrict__ lam, int64_t id) {
   if (id >= a.N) return;
   const bool on = !a.solid[id] && int(id % a.nx) == a.outlet;
 #pragma unroll
   for (int i = 0; i < Q; ++i) lam[int64_t(i) * a.N + id] = on ? R(Lat<Q>::cx(i)) * a.invT : R(0);
 }
 
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_seed(AdjArgs<R> a, R* __restrict__ lam) {
+  seed_cell(a, lam, int64_t(blockIdx.x) * kThreads + threadIdx.x);
+}
+
 // ---- one backward step: lambda^{t+1}, f^t -> lambda^t, grad += ds ----------------
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restrict__ fst,
-                                                  const R* __restrict__ lam1, R* __restrict__ lam0,
-                                                  R* __restrict__ grad, int with_obj) {
-  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+__device__ __forceinline__ void bwd_cell(const AdjArgs<R>& a, const R* __restrict__ fst,
+                                         const R* __restrict__ lam1, R* __restrict__ lam0,
+                                         R* __restrict__ grad, int with_obj, int64_t id) {
   if (id >= a.N || a.solid[id]) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
@@ -280,6 +295,71 @@ __global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restr
     if (obj) l += R(Lat<Q>::cx(j)) * a.invT;
     lam0[int64_t(j) * a.N + id] = l;
   }
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restrict__ fst,
+                                                  const R* __restrict__ lam1, R* __restrict__ lam0,
+                                                  R* __restrict__ grad, int with_obj) {
+  bwd_cell(a, fst, lam1, lam0, grad, with_obj, int64_t(blockIdx.x) * kThreads + threadIdx.x);
+}
+
+// ---- many boxes in one launch ------------------------------------------------------
+// Entry e: the box's arguments, its state buffers by slot number (the same
+// slot schedule for every box, since all share T and the checkpoint
+// interval), its gradient and objective arrays, and its first global cell
+// (a multiple of 64, so a wavefront never spans two boxes).
+template <class R>
+struct Entry {
+  AdjArgs<R> a;
+  R* const* slots;
+  R* grad;
+  double* jt;
+  int64_t cell0;
+};
+
+template <class R>
+__device__ __forceinline__ int entry_of(const Entry<R>* tab, int n, int64_t k) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].cell0 <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_fwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
+                                                    int src, int dst) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= total) return;
+  const Entry<R>& E = tab[entry_of(tab, n, k)];
+  fwd_cell(E.a, E.slots[src], E.slots[dst], k - E.cell0);
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_obj_b(const Entry<R>* __restrict__ tab, int f, int t) {
+  const Entry<R>& E = tab[blockIdx.x];
+  obj_plane(E.a, E.slots[f], E.jt + t);
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_seed_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
+                                                     int lam) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= total) return;
+  const Entry<R>& E = tab[entry_of(tab, n, k)];
+  seed_cell(E.a, E.slots[lam], k - E.cell0);
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_bwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
+                                                    int fst, int lam1, int lam0, int with_obj) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= total) return;
+  const Entry<R>& E = tab[entry_of(tab, n, k)];
+  bwd_cell(E.a, E.slots[fst], E.slots[lam1], E.slots[lam0], E.grad, with_obj, k - E.cell0);
 }
 
 }  // namespace adj
@@ -369,12 +449,12 @@ void free_vec(std::vector<void*>& v, const void* keep) {
   v.clear();
 }
 
-template <class R>
-int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
+// Allocates the buffers a forward run of T steps with checkpoint interval K
+// needs (kept across runs of the same shape) and records T and K.
+int prepare_forward(flb_adjoint_s* h, int T, int K) {
   h->T = T;
   h->K = K;
   h->recorded = false;
-  // buffers
   int rc = FLB_OK;
   for (void*& w : h->d_work)
     if ((rc = alloc_state(h, &w))) return rc;
@@ -411,13 +491,35 @@ int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
       return fail(FLB_ERR_NOMEM, "objective buffer allocation failed");
     h->jt_cap = T + 1;
   }
+  return FLB_OK;
+}
+
+// Records the result of a forward run: the objective from the per-step plane
+// sums jt[1..T] and the buffer holding the final state.
+void finish_forward(flb_adjoint_s* h, const double* jt, void* last) {
+  double s = 0.0;
+  for (int t = 1; t <= h->T; ++t) s += jt[t];
+  h->J = h->T > 0 ? s / double(h->T) : 0.0;
+  h->last = last;
+  h->recorded = h->K >= 0;
+}
+
+// Buffer receiving state t+1 of a forward run whose state t sits in src.
+void* forward_target(const flb_adjoint_s* h, int t, const void* src) {
+  const int K = h->K, T = h->T;
+  if (K == 0) return h->tape[size_t(t + 1)];
+  if (K > 0 && (t + 1) % K == 0 && t + 1 < T) return h->ckpt[size_t((t + 1) / K)];
+  return (src == h->d_work[0]) ? h->d_work[1] : h->d_work[0];
+}
+
+template <class R>
+int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
+  int rc = prepare_forward(h, T, K);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(h->e0, h->stream));
   const void* src = h->d_f0;
   for (int t = 0; t < T; ++t) {
-    void* dst;
-    if (K == 0) dst = h->tape[size_t(t + 1)];
-    else if (K > 0 && (t + 1) % K == 0 && t + 1 < T) dst = h->ckpt[size_t((t + 1) / K)];
-    else dst = (src == h->d_work[0]) ? h->d_work[1] : h->d_work[0];
+    void* dst = forward_target(h, t, src);
     HIP_TRY(fwd_step<R>(h, src, dst));
     HIP_TRY(obj_of<R>(h, dst, h->d_jt + t + 1));
     src = dst;
@@ -429,11 +531,7 @@ int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
                            h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   HIP_TRY(hipEventElapsedTime(&h->fwd_ms, h->e0, h->e1));
-  double s = 0.0;
-  for (int t = 1; t <= T; ++t) s += jt[size_t(t)];
-  h->J = T > 0 ? s / double(T) : 0.0;
-  h->last = const_cast<void*>(src);
-  h->recorded = K >= 0;
+  finish_forward(h, jt.data(), const_cast<void*>(src));
   if (J) *J = h->J;
   return FLB_OK;
 }
@@ -446,9 +544,7 @@ hipError_t bwd_step(const flb_adjoint_s* h, const void* fst, const void* lam1, v
   return hipGetLastError();
 }
 
-template <class R>
-int backward_impl(flb_adjoint_s* h, double* grad) {
-  const int T = h->T, K = h->K;
+int prepare_backward(flb_adjoint_s* h) {
   int rc = FLB_OK;
   for (void*& l : h->d_lam)
     if ((rc = alloc_state(h, &l))) return rc;
@@ -456,6 +552,14 @@ int backward_impl(flb_adjoint_s* h, double* grad) {
     if (hipMalloc(&h->d_grad, size_t(h->N) * h->real) != hipSuccess)
       return fail(FLB_ERR_NOMEM, "gradient allocation failed");
   }
+  return FLB_OK;
+}
+
+template <class R>
+int backward_impl(flb_adjoint_s* h, double* grad) {
+  const int T = h->T, K = h->K;
+  int rc = prepare_backward(h);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(h->e0, h->stream));
   HIP_TRY(hipMemsetAsync(h->d_grad, 0, size_t(h->N) * h->real, h->stream));
   hipLaunchKernelGGL(A::k_seed<R>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
@@ -493,6 +597,217 @@ int backward_impl(flb_adjoint_s* h, double* grad) {
 
 #define CHECK_ADJ(h) \
   if (!(h)) return fail(FLB_ERR_ARG, "null adjoint handle")
+
+}  // namespace
+
+// ---- many boxes stepped together -------------------------------------------
+// Every member keeps its own fields and state buffers; the batch steps all of
+// them with one launch per time step over the concatenated cells (each box
+// starts at a multiple of 64 cells). All members share T and the checkpoint
+// interval, so one slot schedule (which buffer holds which state) serves them
+// all: slot 0 = f^0, 1-2 = forward ping-pong, 3-4 = adjoint ping-pong,
+// 5.. = checkpoints, then the recompute tape.
+struct flb_adjoint_batch_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<flb_adjoint_s*> members;
+  std::vector<int64_t> cell0;
+  int64_t total = 0;
+  int64_t cells = 0;  // sum of member cell counts
+  size_t real = 4;
+  void* d_tab = nullptr;
+  size_t tab_cap = 0;
+  void* d_slots = nullptr;
+  size_t slots_cap = 0;
+  int nslots = 0;
+  int T = 0, K = -1;
+  bool recorded = false;
+  float fwd_ms = 0.0f, bwd_ms = 0.0f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+enum { kSlotF0 = 0, kSlotW0 = 1, kSlotW1 = 2, kSlotL0 = 3, kSlotL1 = 4, kSlotCk = 5 };
+
+int tape_base(int T, int K) { return kSlotCk + (K > 0 ? (T + K - 1) / K : 0); }
+
+// the slot receiving state t+1 when state t sits in slot src (forward_target
+// in slot numbers)
+int forward_slot(int T, int K, int t, int src) {
+  if (K == 0) return tape_base(T, K) + t + 1;
+  if (K > 0 && (t + 1) % K == 0 && t + 1 < T) return kSlotCk + (t + 1) / K;
+  return src == kSlotW0 ? kSlotW1 : kSlotW0;
+}
+
+// Uploads the entry table and every member's slot pointers.
+template <class R>
+int upload_table(flb_adjoint_batch_s* b) {
+  const int T = b->T, K = b->K;
+  const size_t n = b->members.size();
+  const int tb = tape_base(T, K);
+  // tape length of this schedule (prepare_forward sized every member's tape so)
+  const int ntape = K > 0 ? std::min(K, T) : (K == 0 ? T + 1 : 0);
+  const int nslots = tb + ntape;
+  size_t bytes_slots = n * size_t(nslots) * sizeof(R*);
+  size_t bytes_tab = n * sizeof(A::Entry<R>);
+  if (bytes_slots > b->slots_cap) {
+    if (b->d_slots) (void)hipFree(b->d_slots);
+    b->d_slots = nullptr;
+    b->slots_cap = 0;
+    if (hipMalloc(&b->d_slots, bytes_slots) != hipSuccess) return fail(FLB_ERR_NOMEM, "slot table allocation failed");
+    b->slots_cap = bytes_slots;
+  }
+  if (bytes_tab > b->tab_cap) {
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    b->d_tab = nullptr;
+    b->tab_cap = 0;
+    if (hipMalloc(&b->d_tab, bytes_tab) != hipSuccess) return fail(FLB_ERR_NOMEM, "entry table allocation failed");
+    b->tab_cap = bytes_tab;
+  }
+  std::vector<R*> slots(n * size_t(nslots), nullptr);
+  std::vector<A::Entry<R>> tab(n);
+  R** dslots = static_cast<R**>(b->d_slots);
+  for (size_t m = 0; m < n; ++m) {
+    flb_adjoint_s* h = b->members[m];
+    R** sl = slots.data() + m * size_t(nslots);
+    sl[kSlotF0] = static_cast<R*>(h->d_f0);
+    sl[kSlotW0] = static_cast<R*>(h->d_work[0]);
+    sl[kSlotW1] = static_cast<R*>(h->d_work[1]);
+    sl[kSlotL0] = static_cast<R*>(h->d_lam[0]);
+    sl[kSlotL1] = static_cast<R*>(h->d_lam[1]);
+    for (int k = 0; k < tb - kSlotCk; ++k) sl[kSlotCk + k] = static_cast<R*>(h->ckpt[size_t(k)]);
+    for (int k = 0; k < ntape; ++k) sl[tb + k] = static_cast<R*>(h->tape[size_t(k)]);
+    tab[m].a = adj_args<R>(h);
+    tab[m].slots = dslots + m * size_t(nslots);
+    tab[m].grad = static_cast<R*>(h->d_grad);
+    tab[m].jt = h->d_jt;
+    tab[m].cell0 = b->cell0[m];
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_slots, slots.data(), bytes_slots, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_tab, tab.data(), bytes_tab, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->nslots = nslots;
+  return FLB_OK;
+}
+
+unsigned batch_grid(const flb_adjoint_batch_s* b) {
+  return unsigned((b->total + A::kThreads - 1) / A::kThreads);
+}
+
+template <class R>
+hipError_t fwd_batch(const flb_adjoint_batch_s* b, int src, int dst) {
+  hipLaunchKernelGGL(A::k_fwd_b<R>, dim3(batch_grid(b)), dim3(A::kThreads), 0, b->stream,
+                     static_cast<const A::Entry<R>*>(b->d_tab), int(b->members.size()), b->total, src, dst);
+  return hipGetLastError();
+}
+
+template <class R>
+hipError_t bwd_batch(const flb_adjoint_batch_s* b, int fst, int lam1, int lam0, int with_obj) {
+  hipLaunchKernelGGL(A::k_bwd_b<R>, dim3(batch_grid(b)), dim3(A::kThreads), 0, b->stream,
+                     static_cast<const A::Entry<R>*>(b->d_tab), int(b->members.size()), b->total, fst, lam1,
+                     lam0, with_obj);
+  return hipGetLastError();
+}
+
+template <class R>
+int batch_forward_impl(flb_adjoint_batch_s* b, int T, int K, double* J) {
+  b->recorded = false;
+  int rc = FLB_OK;
+  for (flb_adjoint_s* h : b->members) {
+    if ((rc = prepare_forward(h, T, K))) return rc;
+    if (K >= 0 && (rc = prepare_backward(h))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  b->T = T;
+  b->K = K;
+  if ((rc = upload_table<R>(b))) return rc;
+  const auto* tab = static_cast<const A::Entry<R>*>(b->d_tab);
+  const unsigned nm = unsigned(b->members.size());
+  HIP_TRY(hipEventRecord(b->e0, b->stream));
+  int src = kSlotF0;
+  for (int t = 0; t < T; ++t) {
+    const int dst = forward_slot(T, K, t, src);
+    HIP_TRY(fwd_batch<R>(b, src, dst));
+    hipLaunchKernelGGL(A::k_obj_b<R>, dim3(nm), dim3(A::kThreads), 0, b->stream, tab, dst, t + 1);
+    HIP_TRY(hipGetLastError());
+    src = dst;
+  }
+  HIP_TRY(hipEventRecord(b->e1, b->stream));
+  std::vector<double> jt(size_t(nm) * size_t(T + 1), 0.0);
+  for (unsigned m = 0; m < nm; ++m)
+    HIP_TRY(hipMemcpyAsync(jt.data() + size_t(m) * size_t(T + 1) + 1, b->members[m]->d_jt + 1,
+                           size_t(T) * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipEventElapsedTime(&b->fwd_ms, b->e0, b->e1));
+  for (unsigned m = 0; m < nm; ++m) {
+    flb_adjoint_s* h = b->members[m];
+    // the final state's buffer, by the same slot numbering as upload_table
+    void* buf = nullptr;
+    if (src == kSlotF0) buf = h->d_f0;
+    else if (src == kSlotW0) buf = h->d_work[0];
+    else if (src == kSlotW1) buf = h->d_work[1];
+    else if (src >= tape_base(T, K)) buf = h->tape[size_t(src - tape_base(T, K))];
+    else buf = h->ckpt[size_t(src - kSlotCk)];
+    finish_forward(h, jt.data() + size_t(m) * size_t(T + 1), buf);
+    h->fwd_ms = b->fwd_ms;
+    if (J) J[m] = h->J;
+  }
+  b->recorded = K >= 0;
+  return FLB_OK;
+}
+
+template <class R>
+int batch_backward_impl(flb_adjoint_batch_s* b, double* grad) {
+  const int T = b->T, K = b->K;
+  const unsigned nm = unsigned(b->members.size());
+  int rc = FLB_OK;
+  for (flb_adjoint_s* h : b->members) {
+    if (!h->recorded || h->T != T || h->K != K)
+      return fail(FLB_ERR_STATE, "a member was run on its own since the batch forward run");
+    if ((rc = prepare_backward(h))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  if ((rc = upload_table<R>(b))) return rc;
+  const auto* tab = static_cast<const A::Entry<R>*>(b->d_tab);
+  HIP_TRY(hipEventRecord(b->e0, b->stream));
+  for (flb_adjoint_s* h : b->members)
+    HIP_TRY(hipMemsetAsync(h->d_grad, 0, size_t(h->N) * h->real, b->stream));
+  hipLaunchKernelGGL(A::k_seed_b<R>, dim3(batch_grid(b)), dim3(A::kThreads), 0, b->stream, tab, int(nm), b->total,
+                     int(kSlotL0));
+  HIP_TRY(hipGetLastError());
+  const int tb = tape_base(T, K);
+  int cur = 0;
+  if (K == 0) {
+    for (int t = T - 1; t >= 0; --t) {
+      HIP_TRY(bwd_batch<R>(b, tb + t, kSlotL0 + cur, kSlotL0 + (cur ^ 1), t >= 1));
+      cur ^= 1;
+    }
+  } else {
+    const int nseg = (T + K - 1) / K;
+    for (int k = nseg - 1; k >= 0; --k) {
+      const int t0 = k * K, t1 = std::min((k + 1) * K, T);
+      auto state = [&](int t) { return t == t0 ? kSlotCk + k : tb + (t - t0); };
+      for (int t = t0; t < t1 - 1; ++t) HIP_TRY(fwd_batch<R>(b, state(t), state(t + 1)));
+      for (int t = t1 - 1; t >= t0; --t) {
+        HIP_TRY(bwd_batch<R>(b, state(t), kSlotL0 + cur, kSlotL0 + (cur ^ 1), t >= 1));
+        cur ^= 1;
+      }
+    }
+  }
+  HIP_TRY(hipEventRecord(b->e1, b->stream));
+  std::vector<R> g(size_t(b->cells));
+  size_t off = 0;
+  for (flb_adjoint_s* h : b->members) {
+    HIP_TRY(hipMemcpyAsync(g.data() + off, h->d_grad, size_t(h->N) * sizeof(R), hipMemcpyDeviceToHost, b->stream));
+    off += size_t(h->N);
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipEventElapsedTime(&b->bwd_ms, b->e0, b->e1));
+  for (flb_adjoint_s* h : b->members) h->bwd_ms = b->bwd_ms;
+  for (size_t i = 0; i < g.size(); ++i) grad[i] = double(g[i]);
+  return FLB_OK;
+}
 
 }  // namespace
 
@@ -677,6 +992,72 @@ int flb_adjoint_timing(flb_adjoint h, double* forward_ms, double* backward_ms) {
   CHECK_ADJ(h);
   if (forward_ms) *forward_ms = h->fwd_ms;
   if (backward_ms) *backward_ms = h->bwd_ms;
+  return FLB_OK;
+}
+
+int flb_adjoint_batch_create(const flb_adjoint* members, int n, void* stream, flb_adjoint_batch* out) {
+  if (!members || !out) return fail(FLB_ERR_ARG, "null member list or output");
+  *out = nullptr;
+  if (n < 1) return fail(FLB_ERR_ARG, "a batch needs at least one member");
+  for (int m = 0; m < n; ++m) {
+    if (!members[m]) return fail(FLB_ERR_ARG, "member %d is null", m);
+    for (int k = 0; k < m; ++k)
+      if (members[k] == members[m]) return fail(FLB_ERR_ARG, "member %d appears twice", m);
+    if (members[m]->device != members[0]->device) return fail(FLB_ERR_ARG, "members live on different devices");
+    if (members[m]->real != members[0]->real) return fail(FLB_ERR_ARG, "members mix storage kinds");
+  }
+  flb_adjoint_batch_s* b = new flb_adjoint_batch_s();
+  b->device = members[0]->device;
+  b->stream = static_cast<hipStream_t>(stream);
+  b->real = members[0]->real;
+  b->members.assign(members, members + n);
+  for (int m = 0; m < n; ++m) {
+    b->cell0.push_back(b->total);
+    b->total += (members[m]->N + 63) / 64 * 64;
+    b->cells += members[m]->N;
+  }
+  DeviceGuard g(b->device);
+  if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+    flb_adjoint_batch_destroy(b);
+    return fail(FLB_ERR_HIP, "event creation failed");
+  }
+  *out = b;
+  return FLB_OK;
+}
+
+int flb_adjoint_batch_destroy(flb_adjoint_batch b) {
+  if (!b) return FLB_OK;
+  DeviceGuard g(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  if (b->d_tab) (void)hipFree(b->d_tab);
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  delete b;
+  return FLB_OK;
+}
+
+int flb_adjoint_batch_forward(flb_adjoint_batch b, int steps, int checkpoint_every, double* J) {
+  if (!b) return fail(FLB_ERR_ARG, "null batch handle");
+  if (steps < 1) return fail(FLB_ERR_ARG, "forward needs at least one step");
+  if (checkpoint_every < -1) return fail(FLB_ERR_ARG, "checkpoint_every must be -1, 0 or positive");
+  DeviceGuard g(b->device);
+  return b->real == 8 ? batch_forward_impl<double>(b, steps, checkpoint_every, J)
+                      : batch_forward_impl<float>(b, steps, checkpoint_every, J);
+}
+
+int flb_adjoint_batch_backward(flb_adjoint_batch b, double* grad) {
+  if (!b) return fail(FLB_ERR_ARG, "null batch handle");
+  if (!grad) return fail(FLB_ERR_ARG, "null gradient output");
+  if (!b->recorded) return fail(FLB_ERR_STATE, "no recorded batch forward run (checkpoint_every >= 0 first)");
+  DeviceGuard g(b->device);
+  return b->real == 8 ? batch_backward_impl<double>(b, grad) : batch_backward_impl<float>(b, grad);
+}
+
+int flb_adjoint_batch_timing(flb_adjoint_batch b, double* forward_ms, double* backward_ms) {
+  if (!b) return fail(FLB_ERR_ARG, "null batch handle");
+  if (forward_ms) *forward_ms = b->fwd_ms;
+  if (backward_ms) *backward_ms = b->bwd_ms;
   return FLB_OK;
 }
 

@@ -35,6 +35,9 @@ struct flb_batch_s {
   std::vector<int64_t> h_wave0;
   KArgs* d_tab = nullptr;
   int64_t* d_wave0 = nullptr;
+  std::vector<int> h_wblk;     // block of every wavefront
+  int* d_wblk = nullptr;
+  size_t wblk_cap = 0;
   int64_t cells = 0;
   int wpb = 4;
   int grid = 0;  // 0: one workgroup per group of wpb wavefronts
@@ -56,15 +59,15 @@ hipError_t launch_batch(const flb_batch_s* t, int parity, hipStream_t s) {
   const int nb = int(t->blocks.size());
   switch (t->storage) {
     case FLB_STORE_F32:
-      return flb::launch_batch_f32(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+      return flb::launch_batch_f32(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, t->d_wblk, nb, groups, t->wpb, grid, s);
     case FLB_STORE_F16S:
-      return flb::launch_batch_f16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+      return flb::launch_batch_f16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, t->d_wblk, nb, groups, t->wpb, grid, s);
     case FLB_STORE_BF16S:
-      return flb::launch_batch_bf16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+      return flb::launch_batch_bf16s(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, t->d_wblk, nb, groups, t->wpb, grid, s);
     case FLB_STORE_F16C:
-      return flb::launch_batch_f16c(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+      return flb::launch_batch_f16c(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, t->d_wblk, nb, groups, t->wpb, grid, s);
     default:
-      return flb::launch_batch_f64(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, nb, groups, t->wpb, grid, s);
+      return flb::launch_batch_f64(t->q, parity, t->family, t->per, t->d_tab, t->d_wave0, t->d_wblk, nb, groups, t->wpb, grid, s);
   }
 }
 
@@ -120,10 +123,23 @@ int refresh(flb_batch_s* t, bool force) {
     t->per = t->per || a.px || a.py || a.pz;
     t->versions[k] = b->version;
   }
+  const size_t waves = size_t(t->h_wave0.back());
+  t->h_wblk.resize(waves);
+  for (size_t k = 0; k < t->blocks.size(); ++k)
+    std::fill(t->h_wblk.begin() + t->h_wave0[k], t->h_wblk.begin() + t->h_wave0[k + 1], int(k));
+  if (t->wblk_cap < waves) {
+    if (t->d_wblk) HIP_TRY(hipFree(t->d_wblk));
+    t->d_wblk = nullptr;
+    t->wblk_cap = 0;
+    if (hipMalloc(reinterpret_cast<void**>(&t->d_wblk), std::max<size_t>(waves, 1) * sizeof(int)) != hipSuccess)
+      return fail(FLB_ERR_NOMEM, "batch wave table allocation failed");
+    t->wblk_cap = waves;
+  }
   HIP_TRY(hipMemcpyAsync(t->d_tab, t->h_tab.data(), t->h_tab.size() * sizeof(KArgs), hipMemcpyHostToDevice,
                          t->stream));
   HIP_TRY(hipMemcpyAsync(t->d_wave0, t->h_wave0.data(), t->h_wave0.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice, t->stream));
+  HIP_TRY(hipMemcpyAsync(t->d_wblk, t->h_wblk.data(), waves * sizeof(int), hipMemcpyHostToDevice, t->stream));
   // the host vectors are rebuilt only here, after the copies have completed
   HIP_TRY(hipStreamSynchronize(t->stream));
   return FLB_OK;
@@ -222,6 +238,7 @@ int flb_batch_destroy(flb_batch t) {
     if (e) (void)hipEventDestroy(e);
   if (t->d_tab) (void)hipFree(t->d_tab);
   if (t->d_wave0) (void)hipFree(t->d_wave0);
+  if (t->d_wblk) (void)hipFree(t->d_wblk);
   delete t;
   return FLB_OK;
 }

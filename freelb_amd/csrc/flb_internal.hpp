@@ -81,8 +81,8 @@ struct KArgs {
   hipError_t launch_reduce_##SUFFIX(int q, int parity, const KArgs& a, uint8_t mask,         \
                                     double* partials, double* out, hipStream_t s);          \
   hipError_t launch_batch_##SUFFIX(int q, int parity, int family, bool per, const KArgs* tab, \
-                                   const int64_t* wave0, int nb, int64_t groups, int wpb,     \
-                                   int grid, hipStream_t s);
+                                   const int64_t* wave0, const int* wblk, int nb,             \
+                                   int64_t groups, int wpb, int grid, hipStream_t s);
 
 FLB_DECLARE_LAUNCHERS(f32)
 FLB_DECLARE_LAUNCHERS(f16s)

@@ -303,29 +303,7 @@ __device__ __forceinline__ void st_pop(E* p, E v) {
   else *p = v;
 }
 
-// base + a byte offset held in 32 bits: with a wavefront-uniform base this is
-// the scalar-base + vector-offset form of a global access.
-template <class E>
-__device__ __forceinline__ E* at_bytes(E* base, uint32_t boff) {
-  return reinterpret_cast<E*>(reinterpret_cast<char*>(base) + boff);
-}
-template <class E>
-__device__ __forceinline__ const E* at_bytes(const E* base, uint32_t boff) {
-  return reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + boff);
-}
-
-// A pointer that is the same in every lane, made explicit (scalar registers):
-// the compiler then keeps row bases scalar instead of folding the lane offset
-// in first and redoing the 64-bit arithmetic per direction in vector registers.
-template <class T>
-__device__ __forceinline__ T* uniform_ptr(T* p) {
-  const uint64_t v = reinterpret_cast<uint64_t>(p);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
-  return reinterpret_cast<T*>((uint64_t(hi) << 32) | lo);
-}
-
-template <int Q, class S, int PAR, int FAM, bool PER, bool UNI>
+template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z);
 
 // Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
@@ -333,39 +311,53 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 // block's updated cells in x-fastest order instead.
 template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update(const KArgs& a, int gw, int lane) {
+  int c0, y, z;
   if (a.vec == -3) {
     // linear form: wavefront gw takes updated cells 64 gw .. 64 gw + 63 in
     // x-fastest order, across row ends (short rows fill whole wavefronts)
     const unsigned c = unsigned(gw) * unsigned(kWave) + unsigned(lane);
     const unsigned exy = unsigned(a.ex) * unsigned(a.ey);
     if (c >= exy * unsigned(a.ez)) return;
-    const unsigned zz = c / exy, r = c - zz * exy, yy = r / unsigned(a.ex);
-    cell_update_at<Q, S, PAR, FAM, PER, false>(a, int(r - yy * unsigned(a.ex)), a.ys + int(yy),
-                                               a.zs + int(zz));
+    // c / exy and r / ex through the FP64 reciprocal (exact for c < 2^31 after
+    // one correction), far cheaper than the integer division sequence
+    unsigned zz = unsigned(double(c) * (1.0 / double(exy)));
+    int r = int(c - zz * exy);
+    if (r < 0) {
+      --zz;
+      r += int(exy);
+    } else if (r >= int(exy)) {
+      ++zz;
+      r -= int(exy);
+    }
+    int yy = int(double(r) * (1.0 / double(a.ex)));
+    c0 = r - yy * a.ex;
+    if (c0 < 0) {
+      --yy;
+      c0 += a.ex;
+    } else if (c0 >= a.ex) {
+      ++yy;
+      c0 -= a.ex;
+    }
+    y = a.ys + yy;
+    z = a.zs + int(zz);
   } else {
     const int row = gw / a.wpr;
     if (row >= a.ey * a.ez) return;  // the whole wavefront leaves together
     const int wv = gw - row * a.wpr;
-    const int c0 = wv * kWave + lane;
+    y = a.ys + row % a.ey;
+    z = a.zs + row / a.ey;
+    c0 = wv * kWave + lane;
     if (c0 >= a.ex) return;
-    cell_update_at<Q, S, PAR, FAM, PER, true>(a, c0, a.ys + row % a.ey, a.zs + row / a.ey);
   }
+  cell_update_at<Q, S, PAR, FAM, PER>(a, c0, y, z);
 }
 
-// The update of the cell at x = xs + c0 in row (y, z). UNI: the row is the
-// same for the whole wavefront, so each direction's row base is one scalar
-// pointer and a lane only adds its x; a wavefront without wall links then
-// takes the plain AA addresses, with no per-lane choice between targets.
-template <int Q, class S, int PAR, int FAM, bool PER, bool UNI>
+// The update of the cell at x = xs + c0 in row (y, z).
+template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z) {
   using R = typename S::R;
   using E = typename S::E;
-  if constexpr (UNI) {
-    y = __builtin_amdgcn_readfirstlane(y);
-    z = __builtin_amdgcn_readfirstlane(z);
-  }
   const int x = a.xs + c0;
-  const uint32_t boff = uint32_t(c0) * uint32_t(sizeof(E));  // byte offset of the cell in its row
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
   const uint8_t fl = a.flags[s0];
@@ -380,20 +372,11 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     lm = uint32_t(m >> 32);
   }
   const unsigned lk = ln | lm;
-  // plain addresses for every lane: a uniform row, no periodic x, no wall link
-  bool plain = false;
-  if constexpr (UNI && !PER && PAR == 1) plain = __builtin_amdgcn_ballot_w64(lk != 0u) == 0u;
   R f[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      const E* base = P + int64_t(i) * a.dpop;
-      if constexpr (UNI) base = uniform_ptr(base);
-      f[i] = S::template from<Q>(ld_pop<PAR>(at_bytes(base, boff)), i);
-    } else if (plain) {
-      const int o = opp(i);
-      const E* base = uniform_ptr(P + (int64_t(o) * a.dpop + yz_off<Q, false, true>(a, o, y, z) + Lat<Q>::cx(o)));
-      f[i] = S::template from<Q>(ld_pop<PAR>(at_bytes(base, boff)), i);
+      f[i] = S::template from<Q>(ld_pop<PAR>((P + int64_t(i) * a.dpop) + c0), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
       const int o = opp(i);
@@ -417,12 +400,7 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (PAR == 0) {
-      E* base = P + int64_t(opp(i)) * a.dpop;
-      if constexpr (UNI) base = uniform_ptr(base);
-      st_pop<PAR>(at_bytes(base, boff), S::template to<Q>(f[i], i));
-    } else if (plain) {
-      E* base = uniform_ptr(P + (int64_t(i) * a.dpop + yz_off<Q, false, true>(a, i, y, z) + Lat<Q>::cx(i)));
-      st_pop<PAR>(at_bytes(base, boff), S::template to<Q>(f[i], i));
+      st_pop<PAR>((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
@@ -450,11 +428,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
 // Wavefronts are taken wpb at a time ("groups"); workgroup b walks groups
 // xcd_block(b) + k * gridDim.x, so a grid smaller than the group count runs
 // persistently and a grid of one workgroup per group runs each once. Every
-// wavefront finds its block by binary search over wave0 (uniform, scalar).
+// wavefront reads its block from wblk (one scalar load).
 // ---------------------------------------------------------------------------
 template <int Q, class S, int PAR, int FAM, bool PER>
 __global__ __launch_bounds__(kMaxThreads) void k_step_batch(const KArgs* __restrict__ tab,
                                                             const int64_t* __restrict__ wave0,
+                                                            const int* __restrict__ wblk,
                                                             int nb, int64_t groups, int wpb) {
   const int lane = int(threadIdx.x) & (kWave - 1);
   const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
@@ -463,14 +442,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_batch(const KArgs* __restr
   for (int64_t g = xcd_block(blockIdx.x, gridDim.x); g < groups; g += gridDim.x) {
     const int64_t w = g * wpb + wib;
     if (w >= total) break;
-    int lo = 0, hi = nb - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (wave0[mid] <= w) lo = mid;
-      else hi = mid - 1;
-    }
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    cell_update<Q, S, PAR, FAM, PER>(tab[lo], int(w - wave0[lo]), lane);
+    const int e = __builtin_amdgcn_readfirstlane(wblk[w]);
+    cell_update<Q, S, PAR, FAM, PER>(tab[e], int(w - wave0[e]), lane);
   }
 }
 
@@ -956,29 +929,29 @@ hipError_t step_q(int family, const KArgs& a, hipStream_t s) {
 }
 
 template <int Q, class S, int PAR>
-hipError_t batch_q(int family, bool per, const KArgs* tab, const int64_t* wave0, int nb,
+hipError_t batch_q(int family, bool per, const KArgs* tab, const int64_t* wave0, const int* wblk, int nb,
                    int64_t groups, int wpb, int grid, hipStream_t s) {
   const dim3 g(grid), b(kWave * wpb);
   if (family == FAM_FULL) {
-    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, true>), g, b, 0, s, tab, wave0, nb, groups, wpb);
-    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, false>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, true>), g, b, 0, s, tab, wave0, wblk, nb, groups, wpb);
+    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_FULL, false>), g, b, 0, s, tab, wave0, wblk, nb, groups, wpb);
   } else {
-    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, true>), g, b, 0, s, tab, wave0, nb, groups, wpb);
-    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, false>), g, b, 0, s, tab, wave0, nb, groups, wpb);
+    if (per) hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, true>), g, b, 0, s, tab, wave0, wblk, nb, groups, wpb);
+    else hipLaunchKernelGGL((k_step_batch<Q, S, PAR, FAM_BGK, false>), g, b, 0, s, tab, wave0, wblk, nb, groups, wpb);
   }
   return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_batch(int q, int parity, int family, bool per, const KArgs* tab,
-                        const int64_t* wave0, int nb, int64_t groups, int wpb, int grid,
-                        hipStream_t s) {
+                        const int64_t* wave0, const int* wblk, int nb, int64_t groups, int wpb,
+                        int grid, hipStream_t s) {
   if (q == 19)
-    return parity ? batch_q<19, S, 1>(family, per, tab, wave0, nb, groups, wpb, grid, s)
-                  : batch_q<19, S, 0>(family, per, tab, wave0, nb, groups, wpb, grid, s);
+    return parity ? batch_q<19, S, 1>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s)
+                  : batch_q<19, S, 0>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
   if (q == 9)
-    return parity ? batch_q<9, S, 1>(family, per, tab, wave0, nb, groups, wpb, grid, s)
-                  : batch_q<9, S, 0>(family, per, tab, wave0, nb, groups, wpb, grid, s);
+    return parity ? batch_q<9, S, 1>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s)
+                  : batch_q<9, S, 0>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
   return hipErrorInvalidValue;
 }
 
@@ -1086,9 +1059,9 @@ hipError_t launch_reduce(int q, int parity, const KArgs& a, uint8_t mask, double
     return kern::launch_reduce<STORE>(q, parity, a, mask, partials, out, s);                  \
   }                                                                                            \
   hipError_t launch_batch_##SUFFIX(int q, int parity, int family, bool per, const KArgs* tab, \
-                                   const int64_t* wave0, int nb, int64_t groups, int wpb,     \
-                                   int grid, hipStream_t s) {                                 \
-    return kern::launch_batch<STORE>(q, parity, family, per, tab, wave0, nb, groups, wpb,     \
-                                     grid, s);                                                \
+                                   const int64_t* wave0, const int* wblk, int nb,             \
+                                   int64_t groups, int wpb, int grid, hipStream_t s) {        \
+    return kern::launch_batch<STORE>(q, parity, family, per, tab, wave0, wblk, nb, groups,    \
+                                     wpb, grid, s);                                           \
   }                                                                                            \
   }

@@ -294,6 +294,21 @@ int flb_adjoint_state(flb_adjoint h, double* pops);
 /* Device time of the last forward and backward pass (ms, HIP events). */
 int flb_adjoint_timing(flb_adjoint h, double* forward_ms, double* backward_ms);
 
+/* Many boxes of any sizes stepped together: one launch per time step (and
+ * per adjoint step) covers the cells of every member. Members keep their own
+ * fields and buffers; all must use the same storage kind and device. After a
+ * batch forward run each member holds its own result exactly as after
+ * flb_adjoint_forward (its objective, final state and recorded run, so
+ * flb_adjoint_backward on one member works too). No reference counterpart. */
+typedef struct flb_adjoint_batch_s* flb_adjoint_batch;
+int flb_adjoint_batch_create(const flb_adjoint* members, int n, void* stream, flb_adjoint_batch* out);
+int flb_adjoint_batch_destroy(flb_adjoint_batch b);  /* the members stay */
+/* J[n]: each member's objective. */
+int flb_adjoint_batch_forward(flb_adjoint_batch b, int steps, int checkpoint_every, double* J);
+/* grad: every member's dJ/ds one after another (sum of the members' N). */
+int flb_adjoint_batch_backward(flb_adjoint_batch b, double* grad);
+int flb_adjoint_batch_timing(flb_adjoint_batch b, double* forward_ms, double* backward_ms);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* Device copy bandwidth: a float4 copy kernel over `bytes` (rounded down to a
  * multiple of 16) between two fresh buffers, `reps` timed launches; *gbs gets

@@ -111,3 +111,21 @@ def test_batch_null_arguments():
     assert L.flb_batch_set_launch(None, None) == abi.FLB_ERR_ARG
     assert L.flb_batch_autotune(None, 2, None, None) == abi.FLB_ERR_ARG
     assert "null batch" in L.flb_last_error().decode()
+
+
+def test_adjoint_batch_null_arguments():
+    L = abi.lib()
+    h = C.c_void_p()
+    assert L.flb_adjoint_batch_create(None, 1, None, C.byref(h)) == abi.FLB_ERR_ARG
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.flb_adjoint_batch_create(arr, 0, None, C.byref(h)) == abi.FLB_ERR_ARG
+    assert "at least one member" in L.flb_last_error().decode()
+    assert L.flb_adjoint_batch_create(arr, 2, None, C.byref(h)) == abi.FLB_ERR_ARG
+    assert "member 0 is null" in L.flb_last_error().decode()
+    assert not h.value
+    assert L.flb_adjoint_batch_destroy(None) == abi.FLB_OK
+    J = C.c_double()
+    assert L.flb_adjoint_batch_forward(None, 1, -1, C.byref(J)) == abi.FLB_ERR_ARG
+    assert L.flb_adjoint_batch_backward(None, C.byref(J)) == abi.FLB_ERR_ARG
+    assert L.flb_adjoint_batch_timing(None, None, None) == abi.FLB_ERR_ARG
+    assert "null batch" in L.flb_last_error().decode()

@@ -9,7 +9,7 @@ import torch
 
 from adjoint_model import Model
 from adjoint_torch import AdjointObjective
-from freelb_amd import Adjoint
+from freelb_amd import Adjoint, AdjointBatch
 from freelb_amd.cases import channel_adjoint
 
 pytestmark = pytest.mark.gpu
@@ -133,3 +133,103 @@ def test_channel_gradient_against_fd():
     print(f"channel: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, max rel {rel.max():.3e}, "
           f"median rel {np.median(rel):.3e}, |fd| in [{np.abs(fd).min():.3e}, {np.abs(fd).max():.3e}]")
     assert rel.max() <= 1e-3
+
+
+def _mixed_members(storage):
+    """Three boxes of different sizes and boundary setups."""
+    specs = [((10, 6, 5), (True, False, False), [33, 34, 140, 201]),
+             ((7, 9, 4), (False, True, False), [5, 77, 150]),
+             ((13, 5, 6), (True, True, True), [])]
+    out = []
+    for k, (dims, per, sol) in enumerate(specs):
+        N = int(np.prod(dims))
+        rng = np.random.default_rng(40 + k)
+        solid = np.zeros(N, dtype=bool)
+        solid[sol] = True
+        a = Adjoint(dims, omega=1 / (0.6 + 0.1 * k), force=(1e-3, -2e-4 * k, 1e-4), brinkman_alpha=0.2 + 0.1 * k,
+                    outlet_x=dims[0] - 1 - k, periodic=per, storage=storage)
+        a.set_solid(solid)
+        a.set_fraction(rng.uniform(0, 1, N))
+        a.init_equilibrium(1 + 0.01 * rng.standard_normal(N), 0.02 * rng.standard_normal((3, N)))
+        out.append(a)
+    return out
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("K", [0, 4, 6])
+def test_batch_matches_members_bitwise(storage, K):
+    """The batched forward and adjoint give each member exactly what its own
+    run gives: objective, final state and gradient, bit for bit."""
+    T = 17
+    solo = _mixed_members(storage)
+    mem = _mixed_members(storage)
+    try:
+        J1, S1, G1 = [], [], []
+        for a in solo:
+            J1.append(a.forward(T, K))
+            S1.append(a.state())
+            G1.append(a.backward())
+        with AdjointBatch(mem) as b:
+            J2 = b.forward(T, K)
+            S2 = [a.state() for a in mem]
+            G2 = b.backward()
+            # a member's own backward after the batch forward run
+            G3 = mem[1].backward()
+        assert np.array_equal(np.array(J1), J2)
+        for s1, s2 in zip(S1, S2):
+            assert np.array_equal(s1, s2)
+        for g1, g2 in zip(G1, G2):
+            assert np.array_equal(g1, g2)
+        assert np.array_equal(G3, G1[1])
+    finally:
+        for a in solo + mem:
+            a.close()
+
+
+def test_batch_forward_only_and_rerun():
+    mem = _mixed_members("f32")
+    try:
+        with AdjointBatch(mem) as b:
+            J = b.forward(9, -1)
+            with pytest.raises(Exception):
+                b.backward()
+            assert np.array_equal(J, b.forward(9, -1))
+            b.forward(12, 5)
+            g = b.backward()
+            assert len(g) == 3 and all(x.size == m.N for x, m in zip(g, mem))
+    finally:
+        for a in mem:
+            a.close()
+
+
+def test_batch_rejects_bad_members():
+    a32 = _mixed_members("f32")
+    a64 = _mixed_members("f64")
+    try:
+        with pytest.raises(Exception):
+            AdjointBatch([a32[0], a32[0]])
+        with pytest.raises(Exception):
+            AdjointBatch([a32[0], a64[1]])
+    finally:
+        for a in a32 + a64:
+            a.close()
+
+
+def test_channel_batch_runs_and_matches_one_member():
+    """Eight channels of seeded extents in [24,12,12]..[40,20,20]: the batch
+    gradient of one member equals that member's own run."""
+    from freelb_amd.cases import channel_batch, channel_dims
+    dims = channel_dims(8, lo=(24, 12, 12), hi=(40, 20, 20), seed=3)
+    b, mem, fr = channel_batch(dims, seed=3)
+    try:
+        J = b.forward(40, 7)
+        G = b.backward()
+        assert np.all(np.isfinite(J)) and np.all(J > 0)
+        solo, _ = channel_adjoint(dims[5], seed=1000 * 3 + 5)
+        with solo:
+            assert solo.forward(40, 7) == J[5]
+            assert np.array_equal(solo.backward(), G[5])
+    finally:
+        b.close()
+        for a in mem:
+            a.close()
