@@ -20,7 +20,7 @@ D2Q9, D3Q19 = 9, 19
 STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64, STORE_F16C = range(5)
 STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64, "f16c": STORE_F16C}
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK,
- DYN_SMAG_MRT, DYN_BGK_FORCE) = range(9)
+ DYN_SMAG_MRT, DYN_BGK_FORCE, DYN_BGK_WRITE) = range(10)
 HWBC_NORMAL, HWBC_MOVING = 0, 1
 
 # Every symbol include/flb_abi.h declares (checked by the CPU test suite).
@@ -34,7 +34,9 @@ EXPORTS = (
     "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
     "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
     "flb_batch_synchronize", "flb_batch_autotune", "flb_batch_set_halo", "flb_set_ghost_overlap",
-    "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange", "flb_adjoint_create", "flb_adjoint_destroy",
+    "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange",
+    "flb_refine_create", "flb_refine_destroy", "flb_refine_info", "flb_refine_communicate", "flb_refine_run",
+    "flb_download_fields", "flb_adjoint_create", "flb_adjoint_destroy",
     "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
     "flb_adjoint_timing", "flb_adjoint_batch_create", "flb_adjoint_batch_destroy", "flb_adjoint_batch_forward",
@@ -139,6 +141,12 @@ def lib() -> C.CDLL:
         "flb_halo_destroy": [P],
         "flb_halo_info": [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)],
         "flb_halo_exchange": [P],
+        "flb_refine_create": [P, P, C.POINTER(C.c_int), P, C.POINTER(P)],
+        "flb_refine_destroy": [P],
+        "flb_refine_info": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "flb_refine_communicate": [P, C.c_int64],
+        "flb_refine_run": [P, C.c_int64, C.c_int],
+        "flb_download_fields": [P, C.POINTER(C.c_double)],
         "flb_adjoint_create": [C.POINTER(AdjointDesc), C.POINTER(P)],
         "flb_adjoint_destroy": [P],
         "flb_adjoint_set_solid": [P, C.POINTER(C.c_uint8)],

@@ -229,7 +229,8 @@ void to_slots(const flb_block_s* b, const U* dense, T* slots, T fill) {
 int family_of(const flb_block_s* b) {
   for (int t = 0; t < b->ntasks; ++t) {
     const int d = b->tasks[t].dynamics;
-    if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT || d == FLB_DYN_BGK_FORCE)
+    if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT || d == FLB_DYN_BGK_FORCE ||
+        d == FLB_DYN_BGK_WRITE)
       return flb::FAM_FULL;
   }
   return flb::FAM_BGK;
@@ -480,7 +481,7 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
   if (n < 0 || n > flb::kMaxTasks) return fail(FLB_ERR_ARG, "task count %d outside [0, %d]", n, flb::kMaxTasks);
   if (n > 0 && !tasks) return fail(FLB_ERR_ARG, "null task table");
   for (int t = 0; t < n; ++t) {
-    if (tasks[t].dynamics > FLB_DYN_BGK_FORCE)
+    if (tasks[t].dynamics > FLB_DYN_BGK_WRITE)
       return fail(FLB_ERR_ARG, "task %d: unknown dynamics %d", t, tasks[t].dynamics);
     if (tasks[t].mask == 0) return fail(FLB_ERR_ARG, "task %d: zero mask never matches", t);
   }
@@ -489,12 +490,28 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
   std::memcpy(saved, b->tasks, sizeof saved);
   for (int t = 0; t < n; ++t) b->tasks[t] = tasks[t];
   b->ntasks = n;
-  const int rc = check_layout(b);
+  int rc = check_layout(b);
   if (rc != FLB_OK) {
     std::memcpy(b->tasks, saved, sizeof saved);
     b->ntasks = saved_n;
+    return rc;
   }
-  return rc;
+  // rhoU<CELL, true> writes the RHO / VELOCITY fields: give the block fields
+  // at FreeLB's initial values (rho 1, u 0) when it has none yet
+  bool writes = false;
+  for (int t = 0; t < n; ++t) writes = writes || tasks[t].dynamics == FLB_DYN_BGK_WRITE;
+  if (writes && (!b->d_rho || !b->d_u)) {
+    DeviceGuard g(b->device);
+    if (!b->d_rho) {
+      const std::vector<double> one(size_t(b->N), 1.0);
+      if ((rc = upload_field(b, one.data(), 1, &b->d_rho))) return rc;
+    }
+    if (!b->d_u) {
+      const std::vector<double> zero(size_t(b->N) * size_t(b->d), 0.0);
+      if ((rc = upload_field(b, zero.data(), b->d, &b->d_u))) return rc;
+    }
+  }
+  return FLB_OK;
 }
 
 int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
@@ -591,6 +608,36 @@ int flb_set_field_u(flb_block b, const double* u) {
   if (!u) return fail(FLB_ERR_ARG, "null u");
   DeviceGuard g(b->device);
   return upload_field(b, u, b->d, &b->d_u);
+}
+
+int flb_download_fields(flb_block b, double* rhou) {
+  CHECK_BLOCK(b);
+  if (!rhou) return fail(FLB_ERR_ARG, "null output");
+  if (b->pending) return fail(FLB_ERR_STATE, "fields read between ApplyCellDynamics and Stream");
+  DeviceGuard g(b->device);
+  const int nx = b->desc.nx, ny = b->desc.ny, nz = b->desc.nz;
+  for (int c = 0; c <= b->d; ++c) {
+    void* src = c == 0 ? b->d_rho : b->d_u;
+    double* out = rhou + size_t(c) * size_t(b->N);
+    if (!src) {
+      for (int64_t k = 0; k < b->N; ++k) out[k] = c == 0 ? 1.0 : 0.0;
+      continue;
+    }
+    const size_t off = c == 0 ? 0 : size_t(c - 1) * size_t(b->stride);
+    std::vector<char> tmp(size_t(b->stride) * b->real);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), static_cast<char*>(src) + off * b->real, tmp.size(), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t sl = flb::kFrontPad + slot_of(b, x, y, z);
+          const double v = b->real == 8 ? reinterpret_cast<const double*>(tmp.data())[sl]
+                                        : double(reinterpret_cast<const float*>(tmp.data())[sl]);
+          out[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * z)] = v;
+        }
+  }
+  return FLB_OK;
 }
 
 static int upload_pops(flb_block b, const void* host, size_t host_real) {

@@ -168,15 +168,16 @@ __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restr
 }
 
 // ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
-// one workgroup: the plane sum into out[0]
+// Workgroup z sums the row (x = outlet, z) of the plane into out[z] (a fixed
+// lane order and a fixed LDS tree, so every run gives the same bits);
+// k_obj_final adds the rows of each step in z order.
 template <class R>
-__device__ __forceinline__ void obj_plane(const AdjArgs<R>& a, const R* __restrict__ f,
-                                          double* __restrict__ out) {
+__device__ __forceinline__ void obj_row(const AdjArgs<R>& a, const R* __restrict__ f, int z,
+                                        double* __restrict__ out) {
   __shared__ double part[kThreads];
   double acc = 0.0;
-  const int64_t plane = int64_t(a.ny) * a.nz;
-  for (int64_t k = threadIdx.x; k < plane; k += kThreads) {
-    const int64_t id = a.outlet + int64_t(a.nx) * k;
+  for (int y = threadIdx.x; y < a.ny; y += kThreads) {
+    const int64_t id = a.outlet + int64_t(a.nx) * (y + int64_t(a.ny) * z);
     if (a.solid[id]) continue;
     double m = 0.0;
 #pragma unroll
@@ -190,13 +191,28 @@ __device__ __forceinline__ void obj_plane(const AdjArgs<R>& a, const R* __restri
     if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = part[0];
+  if (threadIdx.x == 0) out[z] = part[0];
 }
 
+// one workgroup per z
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restrict__ f,
-                                                  double* __restrict__ out) {
-  obj_plane(a, f, out);
+                                                  double* __restrict__ rows) {
+  obj_row(a, f, int(blockIdx.x), rows);
+}
+
+// jt[t] = sum over z of rows[t * nz + z], t = 1 .. T
+__device__ __forceinline__ void obj_final(const double* __restrict__ rows, int nz, int T, double* __restrict__ jt,
+                                          int t) {
+  if (t < 1 || t > T) return;
+  double s = 0.0;
+  for (int z = 0; z < nz; ++z) s += rows[int64_t(t) * nz + z];
+  jt[t] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_obj_final(const double* __restrict__ rows, int nz, int T,
+                                                         double* __restrict__ jt) {
+  obj_final(rows, nz, T, jt, int(blockIdx.x) * kThreads + int(threadIdx.x));
 }
 
 // ---- equilibrium initial state ----------------------------------------------
@@ -315,6 +331,7 @@ struct Entry {
   R* const* slots;
   R* grad;
   double* jt;
+  double* rows;  // per-step plane row sums, [T + 1][nz]
   int64_t cell0;
 };
 
@@ -338,10 +355,19 @@ __global__ __launch_bounds__(kThreads) void k_fwd_b(const Entry<R>* __restrict__
   fwd_cell(E.a, E.slots[src], E.slots[dst], k - E.cell0);
 }
 
+// grid (max nz, boxes)
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_obj_b(const Entry<R>* __restrict__ tab, int f, int t) {
-  const Entry<R>& E = tab[blockIdx.x];
-  obj_plane(E.a, E.slots[f], E.jt + t);
+  const Entry<R>& E = tab[blockIdx.y];
+  if (int(blockIdx.x) >= E.a.nz) return;
+  obj_row(E.a, E.slots[f], int(blockIdx.x), E.rows + int64_t(t) * E.a.nz);
+}
+
+// grid (ceil(T / kThreads), boxes)
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_obj_final_b(const Entry<R>* __restrict__ tab, int T) {
+  const Entry<R>& E = tab[blockIdx.y];
+  obj_final(E.rows, E.a.nz, T, E.jt, int(blockIdx.x) * kThreads + int(threadIdx.x));
 }
 
 template <class R>
@@ -383,6 +409,8 @@ struct flb_adjoint_s {
   void* d_grad = nullptr;
   double* d_jt = nullptr;
   int jt_cap = 0;
+  double* d_rows = nullptr;  // [T + 1][nz] plane row sums
+  int rows_cap = 0;          // in steps
   std::vector<void*> ckpt;  // ckpt[k] = state at step k*K (ckpt[0] aliases d_f0)
   std::vector<void*> tape;  // recompute slots, or every state when K == 0
   int T = 0;
@@ -427,10 +455,11 @@ hipError_t fwd_step(const flb_adjoint_s* h, const void* src, void* dst) {
   return hipGetLastError();
 }
 
+// plane row sums of state t into rows[t]
 template <class R>
-hipError_t obj_of(const flb_adjoint_s* h, const void* f, double* out) {
-  hipLaunchKernelGGL(A::k_obj<R>, dim3(1), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
-                     static_cast<const R*>(f), out);
+hipError_t obj_of(const flb_adjoint_s* h, const void* f, int t) {
+  hipLaunchKernelGGL(A::k_obj<R>, dim3(unsigned(h->desc.nz)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<const R*>(f), h->d_rows + int64_t(t) * h->desc.nz);
   return hipGetLastError();
 }
 
@@ -491,6 +520,15 @@ int prepare_forward(flb_adjoint_s* h, int T, int K) {
       return fail(FLB_ERR_NOMEM, "objective buffer allocation failed");
     h->jt_cap = T + 1;
   }
+  if (h->rows_cap < T + 1) {
+    if (h->d_rows) (void)hipFree(h->d_rows);
+    h->d_rows = nullptr;
+    h->rows_cap = 0;
+    if (hipMalloc(reinterpret_cast<void**>(&h->d_rows), size_t(T + 1) * size_t(h->desc.nz) * sizeof(double)) !=
+        hipSuccess)
+      return fail(FLB_ERR_NOMEM, "objective row buffer allocation failed");
+    h->rows_cap = T + 1;
+  }
   return FLB_OK;
 }
 
@@ -521,9 +559,12 @@ int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
   for (int t = 0; t < T; ++t) {
     void* dst = forward_target(h, t, src);
     HIP_TRY(fwd_step<R>(h, src, dst));
-    HIP_TRY(obj_of<R>(h, dst, h->d_jt + t + 1));
+    HIP_TRY(obj_of<R>(h, dst, t + 1));
     src = dst;
   }
+  hipLaunchKernelGGL(A::k_obj_final, dim3(unsigned((T + A::kThreads) / A::kThreads)), dim3(A::kThreads), 0,
+                     h->stream, static_cast<const double*>(h->d_rows), h->desc.nz, T, h->d_jt);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->e1, h->stream));
   std::vector<double> jt(size_t(T + 1), 0.0);
   if (T > 0)
@@ -614,6 +655,7 @@ struct flb_adjoint_batch_s {
   std::vector<int64_t> cell0;
   int64_t total = 0;
   int64_t cells = 0;  // sum of member cell counts
+  int max_nz = 1;
   size_t real = 4;
   void* d_tab = nullptr;
   size_t tab_cap = 0;
@@ -682,6 +724,7 @@ int upload_table(flb_adjoint_batch_s* b) {
     tab[m].slots = dslots + m * size_t(nslots);
     tab[m].grad = static_cast<R*>(h->d_grad);
     tab[m].jt = h->d_jt;
+    tab[m].rows = h->d_rows;
     tab[m].cell0 = b->cell0[m];
   }
   HIP_TRY(hipMemcpyAsync(b->d_slots, slots.data(), bytes_slots, hipMemcpyHostToDevice, b->stream));
@@ -729,10 +772,14 @@ int batch_forward_impl(flb_adjoint_batch_s* b, int T, int K, double* J) {
   for (int t = 0; t < T; ++t) {
     const int dst = forward_slot(T, K, t, src);
     HIP_TRY(fwd_batch<R>(b, src, dst));
-    hipLaunchKernelGGL(A::k_obj_b<R>, dim3(nm), dim3(A::kThreads), 0, b->stream, tab, dst, t + 1);
+    hipLaunchKernelGGL(A::k_obj_b<R>, dim3(unsigned(b->max_nz), nm), dim3(A::kThreads), 0, b->stream, tab, dst,
+                       t + 1);
     HIP_TRY(hipGetLastError());
     src = dst;
   }
+  hipLaunchKernelGGL(A::k_obj_final_b<R>, dim3(unsigned((T + A::kThreads) / A::kThreads), nm), dim3(A::kThreads), 0,
+                     b->stream, tab, T);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(b->e1, b->stream));
   std::vector<double> jt(size_t(nm) * size_t(T + 1), 0.0);
   for (unsigned m = 0; m < nm; ++m)
@@ -857,7 +904,7 @@ int flb_adjoint_destroy(flb_adjoint h) {
   free_vec(h->ckpt, h->d_f0);
   free_vec(h->tape, h->d_f0);
   void* ptrs[] = {h->d_solid, h->d_s, h->d_f0, h->d_work[0], h->d_work[1], h->d_lam[0], h->d_lam[1],
-                  h->d_grad, h->d_jt};
+                  h->d_grad, h->d_jt, h->d_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->e0) (void)hipEventDestroy(h->e0);
@@ -1015,6 +1062,7 @@ int flb_adjoint_batch_create(const flb_adjoint* members, int n, void* stream, fl
     b->cell0.push_back(b->total);
     b->total += (members[m]->N + 63) / 64 * 64;
     b->cells += members[m]->N;
+    b->max_nz = std::max(b->max_nz, members[m]->desc.nz);
   }
   DeviceGuard g(b->device);
   if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {

@@ -54,7 +54,8 @@ enum : int {
   DYN_MRT = 5,
   DYN_SMAG_BGK = 6,
   DYN_SMAG_MRT = 7,
-  DYN_BGK_FORCE = 8
+  DYN_BGK_FORCE = 8,
+  DYN_BGK_WRITE = 9
 };
 enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1 };
 
@@ -160,6 +161,18 @@ __device__ __forceinline__ void load_fields(const KArgs& a, int slot, R& rho_f, 
   for (int d = 0; d < 3; ++d) u_f[d] = (u && d < Lat<Q>::D) ? u[d * a.fstride + slot] : R(0);
 }
 
+// moment::rhoU<CELL, true> (moment.h:359-377): the moments go to the RHO /
+// VELOCITY fields of the cell.
+template <int Q, class R>
+__device__ __forceinline__ void store_fields(const KArgs& a, int slot, R rho, const R (&u)[3]) {
+  if (a.rho_f) static_cast<R*>(const_cast<void*>(a.rho_f))[slot] = rho;
+  R* uf = static_cast<R*>(const_cast<void*>(a.u_f));
+  if (uf) {
+#pragma unroll
+    for (int d = 0; d < Lat<Q>::D; ++d) uf[d * a.fstride + slot] = u[d];
+  }
+}
+
 // One cell's collision. The RHO / VELOCITY fields of the cell (slot) are read
 // here, by the dynamics that use them, so they hold no registers elsewhere.
 template <int Q, class R, int FAM>
@@ -190,6 +203,11 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
           break;
         }
         rhou(f, rho, u);
+        if (dyn == DYN_BGK_WRITE) {
+          store_fields<Q>(a, slot, rho, u);
+          bgk_with<Q>(f, rho, u, om, _om);
+          break;
+        }
         R eo = om;
         if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
           load_fields<Q>(a, slot, rho_f, u_f);

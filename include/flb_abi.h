@@ -87,9 +87,12 @@ enum {
                                 D3Q19 on the d'Humieres 2002 basis, no reference form) */
   FLB_DYN_SMAG_BGK = 6,      /* collision::SmagorinskyBGK                                   lbm/collisionLES.h:29-67 */
   FLB_DYN_SMAG_MRT = 7,      /* MRT whose shear rates take the Smagorinsky omega (no reference form) */
-  FLB_DYN_BGK_FORCE = 8      /* collision::BGKForce<moment::forcerhoU, SecondOrder, force::ConstForce>
+  FLB_DYN_BGK_FORCE = 8,     /* collision::BGKForce<moment::forcerhoU, SecondOrder, force::ConstForce>
                                 lbm/collision.h:61-90, lbm/force.h:29-36, lbm/moment.h:401-438;
                                 the force is the block's constant (flb_set_force) */
+  FLB_DYN_BGK_WRITE = 9      /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: BGK that also
+                                stores rho and u in the RHO / VELOCITY fields (lbm/moment.h:359-377),
+                                as the refined drivers do (examples/cavref2d/cavref2d.cpp:178-179) */
 };
 
 /* Halfway bounce-back kinds: bounceback::normal / bounceback::movingwall
@@ -160,6 +163,10 @@ int flb_set_force(flb_block b, const double* force, int d);
 int flb_upload_flags(flb_block b, const uint8_t* flags);           /* N bytes */
 int flb_set_field_rho(flb_block b, const double* rho);             /* N values  */
 int flb_set_field_u(flb_block b, const double* u);                 /* d*N values, SoA */
+/* RHO and VELOCITY fields back to the host, rhou[(d+1)][N] (rho 1, u 0 where
+ * the block has none): the fields FLB_DYN_BGK_WRITE cells wrote
+ * (cell.get<RHO>() / get<VELOCITY>() of the reference). */
+int flb_download_fields(flb_block b, double* rhou);
 
 /* ---- populations --------------------------------------------------------- */
 /* Natural-layout (post-stream) populations [q][N]. _f32 serves the F32, F16S,
@@ -251,6 +258,36 @@ int flb_halo_create(const flb_block* blocks, int n, const int* origins, const in
 int flb_halo_destroy(flb_halo h);
 int flb_halo_info(flb_halo h, int* n_items, int64_t* n_ghost_cells);
 int flb_halo_exchange(flb_halo h);
+
+/* ---- two-level refinement -------------------------------------------------------
+ * A coarse block (level 0) and a fine block at twice its resolution (level 1)
+ * whose interior covers the coarse cells [origin, origin + m) per axis; the
+ * fine extent is 2m + 4 (two fine ghost layers, one coarse cell, as FreeLB's
+ * level-1 blocks built with useHelperOlap = false, geometry/block_geometry3d.hh:174-190)
+ * and fine local cell 2 + 2 (c - origin) + b, b in {0, 1}, lies in coarse cell c.
+ * The fine block has overlap 1 (its outer ghost layer; the inner one is updated),
+ * the coarse cells inside the patch beyond its outer coarse layer match no task,
+ * and both blocks keep RHO / VELOCITY fields written by FLB_DYN_BGK_WRITE cells.
+ * Replaces FreeLB's BlockLatticeManager::FullCommunicate(count) for the pair
+ * (data_struct/block_lattice.hh:1579-1591): at even counts the outer coarse layer
+ * of the patch takes the average of its 2^d fine cells converted by
+ * RefineConverter::getPopC (averCommunicate, block_lattice.hh:1724-1755,
+ * lbm/unit_converter.h:657-659); at every count the inner fine ghost layer takes
+ * the 2^d-point interpolation of the coarse cells around it (weights
+ * parallel/communicator.h:575-600) converted by getPopF (intpCommunicate,
+ * block_lattice.hh:1757-1811, unit_converter.h:646-648); feq is
+ * Equilibrium::SecondOrder of the averaged / interpolated fields. Both blocks
+ * run on the refinement's stream. */
+typedef struct flb_refine_s* flb_refine;
+int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void* stream, flb_refine* out);
+int flb_refine_destroy(flb_refine r);  /* the blocks stay */
+int flb_refine_info(flb_refine r, int64_t* n_average, int64_t* n_interp);
+/* FullCommunicate(count) of the pair: average at even counts, then interpolation. */
+int flb_refine_communicate(flb_refine r, int64_t count);
+/* n counts of the refined loop from count0 (examples/cavref2d/cavref2d.cpp:221-228):
+ * the fine block steps every count, the coarse block on even counts, then
+ * flb_refine_communicate(count). */
+int flb_refine_run(flb_refine r, int64_t count0, int n);
 
 /* ---- differentiation -------------------------------------------------------
  * A D3Q19 box (nx*ny*nz cells, FreeLB index order, any axis periodic; cells

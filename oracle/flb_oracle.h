@@ -36,8 +36,10 @@ enum {
   FLBO_DYN_MRT = 5,            /* collision::MRT_Feq_RhoU                         collisionMRT.h:132-190 */
   FLBO_DYN_SMAG_BGK = 6,       /* collision::SmagorinskyBGK                       collisionLES.h:29-67 */
   FLBO_DYN_SMAG_MRT = 7,       /* MRT with the Smagorinsky effective shear rate (no reference form) */
-  FLBO_DYN_BGK_FORCE = 8       /* collision::BGKForce<forcerhoU, SecondOrder, ConstForce>
+  FLBO_DYN_BGK_FORCE = 8,      /* collision::BGKForce<forcerhoU, SecondOrder, ConstForce>
                                   collision.h:61-90, moment.ur.h:564-574/639-663, force.ur.h */
+  FLBO_DYN_BGK_WRITE = 9       /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: the moments
+                                  go to the RHO / VELOCITY fields (moment.h:359-377) */
 };
 
 enum { FLBO_HWBC_NORMAL = 0, FLBO_HWBC_MOVING = 1 };
@@ -66,8 +68,9 @@ typedef struct {
   int nbc;
   flbo_hwbc bcs[4];
   double omega;       /* 1/tau; the FP32 path rounds it to float first */
-  const double* rho_field;  /* N values or NULL (then 1): RHO field of moving walls / useFieldrhoU */
-  const double* u_field;    /* d*N values (SoA) or NULL (then 0): VELOCITY field */
+  double* rho_field;  /* N values or NULL (then 1): RHO field of moving walls / useFieldrhoU;
+                         FLBO_DYN_BGK_WRITE cells store their rho here */
+  double* u_field;    /* d*N values (SoA) or NULL (then 0): VELOCITY field (likewise) */
   const double* mrt_rates;  /* q relaxation rates for the MRT kinds, or NULL for the defaults;
                                a NaN entry marks the shear rates that take omega */
   int nthreads;       /* OpenMP threads, <= 0 for the default */
@@ -88,6 +91,31 @@ void flbo_rhou_f64(const flbo_lattice* L, const double* pops, uint8_t mask, doub
  * (equilibrium.ur.h:65-94, 165-214); rho/u are (d+1)*N SoA. */
 void flbo_init_feq_f32(const flbo_lattice* L, const float* rhou, float* pops);
 void flbo_init_feq_f64(const flbo_lattice* L, const double* rhou, double* pops);
+
+/* Two-level refinement, FreeLB's FullCommunicate(count) for one coarse and one
+ * fine lattice (block_lattice.hh:1579-1591). The fine lattice covers the
+ * coarse cells [origin, origin + m) per axis with two fine ghost layers
+ * (extent 2m + 4); fine cell 2 + 2 (c - origin) + b lies in coarse cell c.
+ * average: averCommunicate (block_lattice.hh:1724-1755) into the outer
+ *   coarse layer of the patch, sources in getRefinedCellIdx order
+ *   (basic_geometry.hh:1211-1239), getAverage (communicator.h:608-620),
+ *   getPopC (unit_converter.h:657-659) with the fine omega;
+ * interpolate: intpCommunicate / popIntpCommunicate (block_lattice.hh:1757-1811)
+ *   into both fine ghost layers, stencils as AddtoSharedIntpComm
+ *   (block_geometry2d.hh:988-1059), weights getIntpWeight (communicator.h:575-600),
+ *   getInterpolation (communicator.h:622-645), getPopF (unit_converter.h:646-648)
+ *   with the coarse omega.
+ * feq is Equilibrium::SecondOrder (equilibrium.h:49-75) of the averaged or
+ * interpolated RHO / VELOCITY fields. Populations are the natural post-stream
+ * arrays [q][N] of flbo_run. */
+void flbo_refine_average_f32(const flbo_lattice* C, float* popsC, const flbo_lattice* F, const float* popsF,
+                             const int* origin);
+void flbo_refine_average_f64(const flbo_lattice* C, double* popsC, const flbo_lattice* F, const double* popsF,
+                             const int* origin);
+void flbo_refine_interpolate_f32(const flbo_lattice* C, const float* popsC, const flbo_lattice* F, float* popsF,
+                                 const int* origin);
+void flbo_refine_interpolate_f64(const flbo_lattice* C, const double* popsC, const flbo_lattice* F,
+                                 double* popsF, const int* origin);
 
 /* Flag layout of the lid-driven cavity drivers: BlockGeometry3D/2D with one
  * block and overlap 1, FlagFM.forEach(cavity, AABB), SetupBoundary(cavity, BB),

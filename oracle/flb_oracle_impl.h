@@ -205,11 +205,18 @@ static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const RE
   switch (t->dynamics) {
     case FLBO_DYN_BGK:
     case FLBO_DYN_BGK_FIELD:
+    case FLBO_DYN_BGK_WRITE:
       /* collision::BGK (collision.h:39-56) */
-      if (t->dynamics == FLBO_DYN_BGK) {
-        FN(rhou_cell)(q, c, &rho, u);
-      } else {
+      if (t->dynamics == FLBO_DYN_BGK_FIELD) {
         FN(fields)(L, id, N, &rho, u);
+      } else {
+        FN(rhou_cell)(q, c, &rho, u);
+        if (t->dynamics == FLBO_DYN_BGK_WRITE) {
+          /* rhoUImpl<CELL, true> (moment.h:373-376) */
+          if (L->rho_field) L->rho_field[id] = (double)rho;
+          if (L->u_field)
+            for (int a = 0; a < d; ++a) L->u_field[(size_t)a * N + id] = (double)u[a];
+        }
       }
       FN(feq_cell)(q, w, rho, u, feq);
       for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i];
@@ -412,6 +419,130 @@ void FN(init_feq_impl)(const flbo_lattice* L, const REAL* rhou, REAL* pops) {
     FN(feq_cell)(q, w, rhou[id], u, feq);
     for (int i = 0; i < q; ++i) pops[(size_t)i * N + id] = feq[i];
   }
+}
+
+
+/* ---- two-level refinement (see flb_oracle.h) ---- */
+
+/* Equilibrium<T, LatSet>::Order2 / SecondOrder (equilibrium.h:49-53, 69-75):
+ * uc = u . c_k and u2 = |u|^2 summed from 0 in component order,
+ * w rho (1 + 3 uc + uc uc 0.5 9 - 3 u2 0.5) evaluated left to right. */
+static void FN(feq_order2)(int q, const REAL* w, REAL rho, const REAL* u, REAL* e) {
+  const int d = (q == 19) ? 3 : 2;
+  REAL u2 = 0;
+  for (int a = 0; a < d; ++a) u2 += u[a] * u[a];
+  for (int k = 0; k < q; ++k) {
+    const int* ck = (q == 19) ? &C19[k][0] : &C9[k][0];
+    REAL uc = 0;
+    for (int a = 0; a < d; ++a) uc += u[a] * (REAL)ck[a];
+    e[k] = w[k] * rho * ((REAL)1 + (REAL)3 * uc + uc * uc * (REAL)0.5 * (REAL)9 - (REAL)3 * u2 * (REAL)0.5);
+  }
+}
+
+static REAL FN(field_rho)(const flbo_lattice* L, size_t id) {
+  return L->rho_field ? (REAL)L->rho_field[id] : (REAL)1;
+}
+static REAL FN(field_u)(const flbo_lattice* L, int a, size_t id, size_t N) {
+  return L->u_field ? (REAL)L->u_field[(size_t)a * N + id] : (REAL)0;
+}
+
+void CAT(flbo_refine_average, SFX)(const flbo_lattice* C, REAL* popsC, const flbo_lattice* F, const REAL* popsF,
+                                   const int* origin) {
+  const int q = C->q, d = (q == 19) ? 3 : 2, ns = 1 << d;
+  const size_t NC = (size_t)C->nx * C->ny * C->nz, NF = (size_t)F->nx * F->ny * F->nz;
+  const int m[3] = {(F->nx - 4) / 2, (F->ny - 4) / 2, d == 3 ? (F->nz - 4) / 2 : 1};
+  const REAL omega_f = (REAL)F->omega, scale = d == 3 ? (REAL)0.125 : (REAL)0.25;
+  REAL w[19];
+  FN(weights)(q, w);
+  for (int z = 0; z < m[2]; ++z)
+    for (int y = 0; y < m[1]; ++y)
+      for (int x = 0; x < m[0]; ++x) {
+        if (!(x == 0 || x == m[0] - 1 || y == 0 || y == m[1] - 1 || (d == 3 && (z == 0 || z == m[2] - 1))))
+          continue;
+        size_t src[8];
+        for (int j = 0; j < ns; ++j) {
+          const int fx = 2 + 2 * x + (j & 1), fy = 2 + 2 * y + ((j >> 1) & 1);
+          const int fz = d == 3 ? 2 + 2 * z + ((j >> 2) & 1) : 0;
+          src[j] = (size_t)fx + (size_t)F->nx * ((size_t)fy + (size_t)F->ny * fz);
+        }
+        /* getAverage: (a0 + a1 + ... + a(n-1)) * 1/n */
+        REAL rho = FN(field_rho)(F, src[0]), u[3] = {0, 0, 0};
+        for (int j = 1; j < ns; ++j) rho = rho + FN(field_rho)(F, src[j]);
+        rho = rho * scale;
+        for (int a = 0; a < d; ++a) {
+          REAL s = FN(field_u)(F, a, src[0], NF);
+          for (int j = 1; j < ns; ++j) s = s + FN(field_u)(F, a, src[j], NF);
+          u[a] = s * scale;
+        }
+        REAL e[19];
+        FN(feq_order2)(q, w, rho, u, e);
+        const size_t cid = (size_t)(origin[0] + x) +
+                           (size_t)C->nx * ((size_t)(origin[1] + y) + (size_t)C->ny * (d == 3 ? origin[2] + z : 0));
+        for (int k = 0; k < q; ++k) {
+          const REAL* pk = popsF + (size_t)k * NF;
+          REAL s = pk[src[0]];
+          for (int j = 1; j < ns; ++j) s = s + pk[src[j]];
+          const REAL p = s * scale;
+          popsC[(size_t)k * NC + cid] = p + (REAL)0.5 * omega_f * (p - e[k]);
+        }
+      }
+}
+
+void CAT(flbo_refine_interpolate, SFX)(const flbo_lattice* C, const REAL* popsC, const flbo_lattice* F,
+                                       REAL* popsF, const int* origin) {
+  const int q = C->q, d = (q == 19) ? 3 : 2, ns = 1 << d;
+  const size_t NC = (size_t)C->nx * C->ny * C->nz, NF = (size_t)F->nx * F->ny * F->nz;
+  const int fn[3] = {F->nx, F->ny, d == 3 ? F->nz : 1};
+  const REAL omega_c = (REAL)C->omega;
+  REAL w[19];
+  FN(weights)(q, w);
+  for (int z = 0; z < fn[2]; ++z)
+    for (int y = 0; y < fn[1]; ++y)
+      for (int x = 0; x < fn[0]; ++x) {
+        const int f3[3] = {x, y, z};
+        int ghost = 0;
+        for (int a = 0; a < d; ++a) ghost |= f3[a] < 2 || f3[a] >= fn[a] - 2;
+        if (!ghost) continue;
+        /* coarse unit u = origin + floor((f - 2) / 2); sub-cell bit (f - 2) mod 2;
+           stencil corner u - 1 + bit (AddtoSharedIntpComm's Cid0 and its shifts) */
+        int corner[3] = {0, 0, 0}, sub = 0;
+        for (int a = 0; a < d; ++a) {
+          const int rel = f3[a] - 2;
+          const int fl = rel >= 0 ? rel / 2 : -((1 - rel) / 2);
+          const int bit = rel - 2 * fl;
+          corner[a] = origin[a] + fl - 1 + bit;
+          sub |= bit << a;
+        }
+        size_t src[8];
+        REAL wt[8];
+        for (int j = 0; j < ns; ++j) {
+          const int cx = corner[0] + (j & 1), cy = corner[1] + ((j >> 1) & 1);
+          const int cz = d == 3 ? corner[2] + ((j >> 2) & 1) : 0;
+          src[j] = (size_t)cx + (size_t)C->nx * ((size_t)cy + (size_t)C->ny * cz);
+          /* getIntpWeight: 3/4 along each axis where the stencil cell is the
+             nearer one (its bit differs from the sub-cell's), else 1/4 */
+          REAL wj = 1;
+          for (int a = 0; a < d; ++a) wj *= (((sub >> a) & 1) != ((j >> a) & 1)) ? (REAL)0.75 : (REAL)0.25;
+          wt[j] = wj;
+        }
+        /* getInterpolation: a0 w0 + a1 w1 + ... left to right */
+        REAL rho = FN(field_rho)(C, src[0]) * wt[0], u[3] = {0, 0, 0};
+        for (int j = 1; j < ns; ++j) rho = rho + FN(field_rho)(C, src[j]) * wt[j];
+        for (int a = 0; a < d; ++a) {
+          REAL s = FN(field_u)(C, a, src[0], NC) * wt[0];
+          for (int j = 1; j < ns; ++j) s = s + FN(field_u)(C, a, src[j], NC) * wt[j];
+          u[a] = s;
+        }
+        REAL e[19];
+        FN(feq_order2)(q, w, rho, u, e);
+        const size_t fid = (size_t)x + (size_t)F->nx * ((size_t)y + (size_t)F->ny * z);
+        for (int k = 0; k < q; ++k) {
+          const REAL* pk = popsC + (size_t)k * NC;
+          REAL s = pk[src[0]] * wt[0];
+          for (int j = 1; j < ns; ++j) s = s + pk[src[j]] * wt[j];
+          popsF[(size_t)k * NF + fid] = s - (REAL)0.25 * omega_c * (s - e[k]);
+        }
+      }
 }
 
 #undef FN

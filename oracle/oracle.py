@@ -15,11 +15,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK, DYN_SMAG_MRT,
- DYN_BGK_FORCE) = range(9)
+ DYN_BGK_FORCE, DYN_BGK_WRITE) = range(10)
 HWBC_NORMAL, HWBC_MOVING = 0, 1
 
 # Cavity flags of the reference drivers (benchmarks/cavity3d/cavity3d.cpp:121-124).
 VOID, AABB, BB, BBMW = 1, 2, 4, 8
+# Coarse cells covered by a finer patch beyond its outer coarse layer (no task).
+COVERED = 16
 # Body-force density of the 'force' cavity (float values, as FreeLB's CONSTFORCE<float>);
 # the reference harness takes the same decimal strings.
 FORCE_STR = ("1e-4", "-5e-5", "2e-5")
@@ -64,6 +66,11 @@ def lib():
             getattr(_lib, "flbo_run_" + name).restype = C.c_int
             getattr(_lib, "flbo_rhou_" + name).argtypes = [C.POINTER(_Lattice), P, C.c_uint8, P]
             getattr(_lib, "flbo_init_feq_" + name).argtypes = [C.POINTER(_Lattice), P, P]
+            P2 = C.POINTER(C.c_int)
+            getattr(_lib, "flbo_refine_average_" + name).argtypes = [C.POINTER(_Lattice), P, C.POINTER(_Lattice),
+                                                                     P, P2]
+            getattr(_lib, "flbo_refine_interpolate_" + name).argtypes = [C.POINTER(_Lattice), P,
+                                                                         C.POINTER(_Lattice), P, P2]
         _lib.flbo_cavity_flags.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)]
         _lib.flbo_convert_from_rt.argtypes = [C.c_float, C.c_float, C.c_float,
                                               C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -105,10 +112,13 @@ class Lattice:
         for k, (cm, vm, kind) in enumerate(bcs):
             L.bcs[k].cell_mask, L.bcs[k].void_mask, L.bcs[k].kind = cm, vm, kind
         L.omega = float(omega)
+        # the arrays the C side reads (and FLBO_DYN_BGK_WRITE cells write)
+        self.buffers = {}
         for name, arr in (("rho_field", rho_field), ("u_field", u_field), ("mrt_rates", mrt_rates)):
             if arr is not None:
-                a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+                a = np.array(arr, dtype=np.float64).reshape(-1)
                 self._keep.append(a)
+                self.buffers[name] = a
                 setattr(L, name, _ptr(a, C.c_double))
         L.nthreads = nthreads
         self.force = tuple(float(v) for v in force) + (0.0,) * (3 - len(force))
@@ -145,6 +155,94 @@ class Lattice:
         pops = np.empty((self.q, self.N), dtype=rhou.dtype)
         getattr(lib(), "flbo_init_feq_" + name)(C.byref(self.L), _ptr(rhou, T), _ptr(pops, T))
         return pops
+
+
+    def fields(self):
+        """Current RHO and VELOCITY fields, [(d+1)][N] (written by BGK_WRITE cells)."""
+        rho = self.buffers.get("rho_field", np.ones(self.N))
+        u = self.buffers.get("u_field", np.zeros(self.d * self.N)).reshape(self.d, self.N)
+        return np.concatenate([rho.reshape(1, -1), u], axis=0)
+
+
+def refine_average(coarse, popsC, fine, popsF, origin):
+    """averCommunicate into the outer coarse layer of the patch (in place on popsC)."""
+    T = C.c_float if popsC.dtype == np.float32 else C.c_double
+    name = "f32" if popsC.dtype == np.float32 else "f64"
+    o = (C.c_int * 3)(*[int(v) for v in origin])
+    getattr(lib(), "flbo_refine_average_" + name)(C.byref(coarse.L), _ptr(popsC, T), C.byref(fine.L),
+                                                  _ptr(popsF, T), o)
+
+
+def refine_interpolate(coarse, popsC, fine, popsF, origin):
+    """intpCommunicate into both fine ghost layers (in place on popsF)."""
+    T = C.c_float if popsC.dtype == np.float32 else C.c_double
+    name = "f32" if popsC.dtype == np.float32 else "f64"
+    o = (C.c_int * 3)(*[int(v) for v in origin])
+    getattr(lib(), "flbo_refine_interpolate_" + name)(C.byref(coarse.L), _ptr(popsC, T), C.byref(fine.L),
+                                                      _ptr(popsF, T), o)
+
+
+class RefinedCavity:
+    """The two-level lid-driven cavity of examples/cavref2d/cavref2d.cpp with a
+    centred patch: FreeLB's block cells of `cell` cells per axis whose index
+    lies in [p0, p1) on every axis are refined once. Here the coarse level is
+    one block over the whole cavity (the reference's coarse blocks tile it and
+    mirror each other through their overlap cells, which one block reproduces)
+    whose cells inside the patch beyond its outer layer are COVERED, and the
+    fine level one block of 2m + 4 cells per axis (two fine ghost layers).
+    Fluid cells collide with BGK<rhoU<CELL, true>>, wall cells with
+    BGK<useFieldrhoU>, halfway normal / moving walls (cavref2d.cpp:155-179).
+    The fine omega is RefineConverter::getOmegaF (unit_converter.h:627)."""
+
+    def __init__(self, lattice, Ni, Nj, Nk, cell, p0, p1, RT=0.6, visc=0.01, u_phys=0.05, dtype=np.float32):
+        q = 19 if lattice == "d3q19" else 9
+        d = 3 if q == 19 else 2
+        nk = Nk if d == 3 else 0
+        self.q, self.d, self.dtype = q, d, dtype
+        flags = cavity_flags(Ni, Nj, nk)
+        self.cdims = (Ni + 2, Nj + 2, (Nk + 2) if d == 3 else 1)
+        self.origin = [1 + p0 * cell] * d + [0] * (3 - d)
+        self.m = (p1 - p0) * cell
+        f3 = flags.reshape(self.cdims[::-1])
+        o, m = self.origin[0], self.m
+        inner = (slice(o + 1, o + m - 1),) * d
+        if d == 3:
+            f3[inner[::-1]] = COVERED
+        else:
+            f3[0][inner[::-1]] = COVERED
+        flags = f3.reshape(-1)
+        omega, ulat = convert_from_rt(np.float32(RT), np.float32(visc), np.float32(u_phys))
+        omega_f = np.float32(1) / (np.float32(2) / omega - np.float32(0.5))
+        if dtype == np.float64:
+            omega, omega_f = np.float64(1.0 / RT), 1.0 / (2.0 * RT - 0.5)
+        self.omega_c, self.omega_f, self.ulat = omega, omega_f, ulat
+        N = flags.size
+        rho = np.ones(N)
+        u = np.zeros((d, N))
+        u[0, flags == BBMW] = float(ulat)
+        tasks = [(AABB, DYN_BGK_WRITE), (BB | BBMW, DYN_BGK_FIELD)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+        self.coarse = Lattice(q, self.cdims, flags, tasks, bcs, omega=float(omega), rho_field=rho, u_field=u)
+        fn = 2 * m + 4
+        self.fdims = (fn, fn, fn if d == 3 else 1)
+        NF = int(np.prod(self.fdims))
+        self.fine = Lattice(q, self.fdims, np.full(NF, AABB, dtype=np.uint8), tasks, [], omega=float(omega_f),
+                            rho_field=np.ones(NF), u_field=np.zeros((d, NF)))
+        w = weights(q, dtype)
+        self.popsC = np.ascontiguousarray(np.repeat(w[:, None], N, axis=1).astype(dtype))
+        self.popsF = np.ascontiguousarray(np.repeat(w[:, None], NF, axis=1).astype(dtype))
+        self.count = 0
+
+    def run(self, n):
+        """n counts: fine step, coarse step on even counts, FullCommunicate(count)."""
+        for _ in range(n):
+            c = self.count
+            self.fine.run(self.popsF, 1)
+            if c % 2 == 0:
+                self.coarse.run(self.popsC, 1)
+                refine_average(self.coarse, self.popsC, self.fine, self.popsF, self.origin)
+            refine_interpolate(self.coarse, self.popsC, self.fine, self.popsF, self.origin)
+            self.count += 1
 
 
 def cavity_flags(Ni, Nj, Nk=0):
