@@ -291,6 +291,41 @@ def adjoint_batch_bench(n=64, T=400, seed=11, fp64_check=False):
     return out
 
 
+def refined_bench(timer, n=128, patch=64, counts=200, storage="f32"):
+    """The refined cavity: n^3 coarse D3Q19 cells with a centred patch^3 fine
+    patch at twice the resolution, the loop of examples/cavref2d/cavref2d.cpp
+    (fine step every count, coarse step on even counts, average and
+    interpolation exchange) for `counts` counts after 2 untimed ones. Cell
+    updates are the coarse cells stepped (every cell with a task: the cavity
+    minus the covered part of the patch) plus the fine cells stepped (the
+    patch and its inner ghost layer), each update reading and writing 19
+    populations (152 B in FP32) and writing its moments to the fields (16 B)."""
+    from freelb_amd.cases import refined_cavity
+    coarse, fine, ref = refined_cavity(n, patch, storage)
+    m = patch // 2
+    try:
+        ref.run(2)
+        timer.sync()
+        timer.start()
+        ref.run(counts)
+        ms = timer.stop()
+        finite = bool(np.isfinite(coarse.mass_momentum()[0]) and np.isfinite(fine.mass_momentum()[0]))
+        info = ref.info()
+    finally:
+        ref.close()
+        coarse.close()
+        fine.close()
+    c_cells = n ** 3 - (m - 2) ** 3
+    f_cells = (patch + 2) ** 3
+    updates = c_cells * (counts // 2) + f_cells * counts
+    bpu = (152 if storage == "f32" else 76) + 16
+    return {"workload": f"refined cavity: {n}^3 coarse, {patch}^3 fine patch (2:1), {counts} fine steps, "
+                        f"{storage}", "coarse_cells": c_cells, "fine_cells": f_cells,
+            "exchange_items": info, "ms": round(ms, 3), "ms_per_coarse_step": round(2 * ms / counts, 4),
+            "mlups": round(updates / (ms / 1e3) / 1e6, 1),
+            "achieved_gbs": round(updates * bpu / (ms / 1e3) / 1e9, 1), "finite": finite}
+
+
 def profiled_traffic(storage):
     """HBM bytes per launch of the step kernel for this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
@@ -364,6 +399,7 @@ def main():
                     help="512^3 MRT+LES cavity (Re 5000, lid 0.05), 5000 steps, f32/f16s/f16c vs FP64")
     ap.add_argument("--no-cavity", action="store_true", help="with --config4: only the batched adjoint")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
+    ap.add_argument("--no-refined", action="store_true", help="skip the refined-cavity run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -429,6 +465,9 @@ def main():
     if not args.no_batch:
         line["batch"] = batch_bench(timer)
         print(f"# batch: {line['batch']}", file=sys.stderr, flush=True)
+    if not args.no_refined:
+        line["refined"] = refined_bench(timer)
+        print(f"# refined: {line['refined']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)

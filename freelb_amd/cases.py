@@ -133,3 +133,57 @@ def channel_batch(dims_list, storage: str = "f32", tau: float = 0.6, g: float = 
             m.close()
         raise
     return batch, members, fields
+
+
+COVERED = 16  # coarse cells under a finer patch beyond its outer coarse layer
+
+
+def refined_cavity(n: int = 128, patch: int = 64, storage: str = "f32", tau: float = 0.6, u_lid: float = 0.05,
+                   stream=None):
+    """The two-level lid-driven cavity of examples/cavref2d/cavref2d.cpp in 3D:
+    an n^3 D3Q19 coarse cavity (halfway walls, moving lid) with a centred
+    patch of patch^3 fine cells (patch / 2 coarse cells per axis) at twice the
+    resolution. Fluid cells collide with BGK writing their moments to the
+    fields (rhoU<CELL, true>), wall cells with BGK<useFieldrhoU>; the fine
+    omega is RefineConverter::getOmegaF(omega) = 1 / (2 / omega - 1/2) in
+    float (lbm/unit_converter.h:627). Returns (coarse, fine, Refinement)."""
+    from .lattice import Refinement
+    if patch % 2 or patch < 4 or patch // 2 + 4 > n:
+        raise ValueError("patch must be even, >= 4 and leave room for the coarse stencils")
+    m = patch // 2
+    origin = 1 + (n - m) // 2
+    omega_c = np.float32(1.0) / np.float32(tau)
+    omega_f = np.float32(1) / (np.float32(2) / omega_c - np.float32(0.5))
+    flags = cavity_flags(n).reshape(n + 2, n + 2, n + 2)
+    flags[origin + 1:origin + m - 1, origin + 1:origin + m - 1, origin + 1:origin + m - 1] = COVERED
+    flags = flags.reshape(-1)
+    tasks = [Key_TypePair(FLUID, abi.DYN_BGK_WRITE), Key_TypePair(WALL | LID, abi.DYN_BGK_FIELD)]
+    coarse = BlockLattice(19, (n + 2,) * 3, omega=float(omega_c), storage=storage, stream=stream)
+    fine = None
+    try:
+        N = flags.size
+        coarse.upload_flags(flags)
+        coarse.set_tasks(tasks)
+        coarse.set_halfway_bc([HalfwayBB(WALL, VOID, False), HalfwayBB(LID, VOID, True)])
+        u = np.zeros((3, N))
+        u[0, flags == LID] = float(np.float32(u_lid))
+        coarse.set_field_rho(np.ones(N))
+        coarse.set_field_u(u)
+        coarse.init_equilibrium(np.ones(N), np.zeros((3, N)))
+        fn = 2 * m + 4
+        fine = BlockLattice(19, (fn,) * 3, omega=float(omega_f), storage=storage, stream=stream)
+        ff = np.full((fn, fn, fn), COVERED, dtype=np.uint8)
+        ff[1:-1, 1:-1, 1:-1] = FLUID
+        NF = ff.size
+        fine.upload_flags(ff.reshape(-1))
+        fine.set_tasks(tasks)
+        fine.set_field_rho(np.ones(NF))
+        fine.set_field_u(np.zeros((3, NF)))
+        fine.init_equilibrium(np.ones(NF), np.zeros((3, NF)))
+        ref = Refinement(coarse, fine, (origin,) * 3, stream=stream)
+    except Exception:
+        coarse.close()
+        if fine is not None:
+            fine.close()
+        raise
+    return coarse, fine, ref

@@ -156,6 +156,12 @@ class BlockLattice:
             raise ValueError("velocity field size mismatch (expects d*N SoA)")
         abi.check(abi.lib().flb_set_field_u(self._h, abi.ptr(v, C.c_double)))
 
+    def download_fields(self) -> np.ndarray:
+        """RHO and VELOCITY fields, [(d+1)][N] (what BGK_WRITE cells stored)."""
+        out = np.empty((self.d + 1, self.N), dtype=np.float64)
+        abi.check(abi.lib().flb_download_fields(self._h, abi.ptr(out, C.c_double)))
+        return out
+
     # ---- populations -------------------------------------------------------
     def upload_pops(self, pops: np.ndarray) -> None:
         p = np.ascontiguousarray(pops, dtype=self._real).reshape(-1)
@@ -255,6 +261,54 @@ class BlockHalo:
             self.close()
         except Exception:
             pass
+
+
+class Refinement:
+    """A coarse block and a fine block at twice its resolution whose interior
+    covers the coarse cells [origin, origin + m) per axis (fine extent 2m + 4,
+    overlap 1), exchanged as FreeLB's FullCommunicate(count) does for a
+    refined pair (averCommunicate / intpCommunicate,
+    data_struct/block_lattice.hh:1724-1811). run(n) is n counts of
+    examples/cavref2d/cavref2d.cpp's loop: fine step, coarse step on even
+    counts, communicate(count)."""
+
+    def __init__(self, coarse: BlockLattice, fine: BlockLattice, origin, stream=None):
+        L = abi.lib()
+        self.coarse, self.fine = coarse, fine
+        o = (C.c_int * 3)(*[int(v) for v in (list(origin) + [0, 0])[:3]])
+        h = C.c_void_p()
+        abi.check(L.flb_refine_create(coarse.handle, fine.handle, o, stream, C.byref(h)))
+        self._h = h
+        self.count = 0
+
+    def info(self):
+        a, b = C.c_int64(), C.c_int64()
+        abi.check(abi.lib().flb_refine_info(self._h, C.byref(a), C.byref(b)))
+        return {"average": a.value, "interp": b.value}
+
+    def communicate(self, count: int) -> None:
+        abi.check(abi.lib().flb_refine_communicate(self._h, int(count)))
+
+    def run(self, n: int) -> None:
+        abi.check(abi.lib().flb_refine_run(self._h, int(self.count), int(n)))
+        self.count += int(n)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            abi.lib().flb_refine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class BlockLatticeManager:

@@ -92,6 +92,10 @@ struct BGK<moment::rhoU<CELL>, EQ> {
   static constexpr uint8_t kind = FLB_DYN_BGK;
 };
 template <typename CELL, typename EQ>
+struct BGK<moment::rhoU<CELL, true>, EQ> {  // the moments also go to the RHO / VELOCITY fields
+  static constexpr uint8_t kind = FLB_DYN_BGK_WRITE;
+};
+template <typename CELL, typename EQ>
 struct BGK<moment::useFieldrhoU<CELL>, EQ> {
   static constexpr uint8_t kind = FLB_DYN_BGK_FIELD;
 };
@@ -251,6 +255,14 @@ class BlockLattice {
     return out;
   }
 
+  // RHO / VELOCITY fields as the cells left them ((d+1)*N SoA; rho 1, u 0
+  // where none exist): what moment::rhoU<CELL, true> cells wrote.
+  std::vector<double> fields() const {
+    std::vector<double> out(n_ * (LatSet::d + 1));
+    check(flb_download_fields(h_, out.data()));
+    return out;
+  }
+
   // called by the halfway managers
   void addHalfway(const flb_hwbc& bc) {
     bcs_.push_back(bc);
@@ -272,6 +284,54 @@ class BlockLattice {
   std::vector<flb_task> tasks_;
   std::vector<flb_hwbc> bcs_;
   float smagorinsky_ = 0.0f;
+};
+
+// ---- two refinement levels (flb_refine_*) ---------------------------------------------
+// A coarse block and a fine block at twice its resolution covering the coarse
+// cells [origin, origin + m) (fine extent 2m + 4, overlap 1). FreeLB's refined
+// loop (examples/cavref2d/cavref2d.cpp:221-228) maps onto it as
+//   pair.ApplyCellDynamics<Sel>(count, coarse_flags, fine_flags);
+//   pair.Stream(count);                 // BM.Apply(count) is folded into the step
+//   pair.FullCommunicate(count);        // averCommunicate / intpCommunicate
+// The fine level steps every count, the coarse level on even counts
+// (block_lattice.hh:736-764). Run(count0, n) does n counts in one call.
+template <typename T, typename LatSet>
+class RefinedPair {
+ public:
+  RefinedPair(BlockLattice<T, LatSet>& coarse, BlockLattice<T, LatSet>& fine, const int origin[3],
+              void* stream = nullptr)
+      : coarse_(coarse), fine_(fine), stream_(stream) {
+    for (int a = 0; a < 3; ++a) origin_[a] = origin[a];
+  }
+  ~RefinedPair() { flb_refine_destroy(r_); }
+  RefinedPair(const RefinedPair&) = delete;
+  RefinedPair& operator=(const RefinedPair&) = delete;
+
+  template <typename Selector>
+  void ApplyCellDynamics(int64_t count, const std::vector<uint8_t>& coarse_flags,
+                         const std::vector<uint8_t>& fine_flags) {
+    fine_.template ApplyCellDynamics<Selector>(fine_flags);
+    if (count % 2 == 0) coarse_.template ApplyCellDynamics<Selector>(coarse_flags);
+  }
+  void Stream(int64_t count) {
+    fine_.Stream();
+    if (count % 2 == 0) coarse_.Stream();
+  }
+  void FullCommunicate(int64_t count) { check(flb_refine_communicate(handle(), count)); }
+  void Run(int64_t count0, int n) { check(flb_refine_run(handle(), count0, n)); }
+
+  // created on first use: both blocks need their flags and tasks first
+  flb_refine handle() {
+    if (!r_) check(flb_refine_create(coarse_.handle(), fine_.handle(), origin_, stream_, &r_));
+    return r_;
+  }
+
+ private:
+  BlockLattice<T, LatSet>& coarse_;
+  BlockLattice<T, LatSet>& fine_;
+  void* stream_ = nullptr;
+  int origin_[3] = {0, 0, 0};
+  flb_refine r_ = nullptr;
 };
 
 // ---- many blocks: BlockLatticeManager (data_struct/block_lattice.hh:744-778) -------

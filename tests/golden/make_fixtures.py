@@ -8,7 +8,8 @@ the populations, the flag field and FreeLB's own moment::rhoU read-out.
 This script only runs that binary and stores its outputs; it needs the
 reference sources, so it runs in the build container, not on the GPU box.
 
-    python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures)
+    python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures,
+                                                          or "refined" for the refined-cavity ones)
 """
 from __future__ import annotations
 
@@ -38,6 +39,12 @@ CS = "0.17"  # Smagorinsky constant of the smag fixtures
 # the 100^3 cavity, 1000 steps: hashes and a mid-plane slice
 LARGE = [("d3q19", "fw", 100, 100, 100, 1000), ("d3q19", "hw", 100, 100, 100, 1000)]
 RT, VISC, U = "0.53", "0.01", "0.1"
+# two-level refined cavity (oracle/_ref/refine_ref, examples/cavref2d/cavref2d.cpp's loop):
+# (lattice, Ni=Nj(=Nk), blockcell, p0, p1, counts); FreeLB's 3D refinement builder
+# writes past its fine block, so only D2Q9 is pinned this way
+REFINED = [("d2q9", 16, 4, 1, 3, 41), ("d2q9", 24, 4, 2, 4, 34)]
+REF_RT, REF_VISC, REF_U = "0.6", "0.01", "0.05"
+EXE_REF = os.path.join(ROOT, "oracle", "_ref", "refine_ref")
 
 
 def read_dump(path):
@@ -56,6 +63,40 @@ def read_dump(path):
                 uw=np.float32(uw), flags=flags, pops=pops, rhou=rhou)
 
 
+def read_refined(path):
+    """FLBREF02: every block of the refined run (level, overlap, mesh, omega,
+    voxel size, min corner, flags, pops, RHO/VELOCITY fields)."""
+    b = open(path, "rb").read()
+    assert b[:8] == b"FLBREF02"
+    d, q, nb, steps = (int(v) for v in np.frombuffer(b[8:24], dtype=np.int32))
+    uw = np.frombuffer(b[24:28], dtype=np.float32)[0]
+    off = 28
+    blocks = []
+    for _ in range(nb):
+        lev, ov, nx, ny, nz = (int(v) for v in np.frombuffer(b[off:off + 20], dtype=np.int32))
+        off += 20
+        om, vox, mx, my, mz = np.frombuffer(b[off:off + 20], dtype=np.float32)
+        off += 20
+        n = nx * ny * nz
+        flags = np.frombuffer(b[off:off + n], dtype=np.uint8).copy()
+        off += n
+        pops = np.frombuffer(b[off:off + 4 * q * n], dtype=np.float32).reshape(q, n).copy()
+        off += 4 * q * n
+        fields = np.frombuffer(b[off:off + 4 * (d + 1) * n], dtype=np.float32).reshape(d + 1, n).copy()
+        off += 4 * (d + 1) * n
+        blocks.append(dict(level=lev, overlap=ov, dims=(nx, ny, nz), omega=om, vox=vox,
+                           min=np.array([mx, my, mz], dtype=np.float32), flags=flags, pops=pops, fields=fields))
+    return dict(d=d, q=q, steps=steps, uw=uw, blocks=blocks)
+
+
+def run_refined(lat, n, cell, p0, p1, counts, tmp):
+    out = os.path.join(tmp, f"refine_{lat}_{n}_{cell}_{p0}_{p1}_{counts}.bin")
+    nk = n if lat == "d3q19" else 1
+    subprocess.run([EXE_REF, lat, str(n), str(n), str(nk), str(cell), str(p0), str(p1), str(counts), REF_RT,
+                    REF_VISC, REF_U, out], check=True, capture_output=True)
+    return read_refined(out)
+
+
 def run(lat, mode, ni, nj, nk, steps, tmp):
     out = os.path.join(tmp, f"{lat}_{mode}_{ni}_{nj}_{nk}_{steps}.bin")
     env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
@@ -71,6 +112,21 @@ def main():
     if not os.path.exists(EXE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        if only in (None, "refined"):
+            if not os.path.exists(EXE_REF):
+                subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "_ref/refine_ref"], check=True)
+            for lat, n, cell, p0, p1, counts in REFINED:
+                r = run_refined(lat, n, cell, p0, p1, counts, tmp)
+                arrays = {"case": np.array([n, cell, p0, p1, counts]), "uw": r["uw"]}
+                for k, blk in enumerate(r["blocks"]):
+                    for key in ("level", "overlap", "dims", "omega", "min", "flags", "pops", "fields"):
+                        arrays[f"b{k}_{key}"] = np.asarray(blk[key])
+                arrays["nblocks"] = np.array(len(r["blocks"]))
+                name = f"refined_{lat}_{n}_cell{cell}_p{p0}{p1}_{counts}.npz"
+                np.savez_compressed(os.path.join(HERE, name), **arrays)
+                print("wrote", name, flush=True)
+            if only:
+                return
         for lat, mode, ni, nj, nk, steps in SMALL:
             if only and mode != only:
                 continue

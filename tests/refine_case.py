@@ -1,0 +1,89 @@
+"""Helpers for the two-level refined cavity: the FreeLB fixture reader, the
+mapping of FreeLB's blocks onto the oracle's (one coarse block over the whole
+cavity, one fine block), and the same case handed to the GPU path."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+
+
+def load_fixture(path):
+    z = np.load(path)
+    n, cell, p0, p1, counts = (int(v) for v in z["case"])
+    blocks = []
+    for k in range(int(z["nblocks"])):
+        blocks.append({key: z[f"b{k}_{key}"] for key in ("level", "overlap", "dims", "omega", "min", "flags",
+                                                          "pops", "fields")})
+    return dict(n=n, cell=cell, p0=p0, p1=p1, counts=counts, blocks=blocks)
+
+
+def fixture_mismatches(rc, fx, fine_mask=None):
+    """Cells where the oracle's state differs from FreeLB's: every non-void
+    cell of each level-0 block (base and overlap, which mirror the coarse
+    block's cells, the patch's outer coarse layer included) and every cell of
+    the level-1 block (or those of fine_mask), populations and RHO / VELOCITY
+    fields."""
+    d, q = rc.d, rc.q
+    cd = rc.cdims
+    popsC = rc.popsC.reshape((q,) + cd[::-1])
+    fldC = rc.coarse.fields().astype(np.float32).reshape((d + 1,) + cd[::-1])
+    bad, checked = 0, 0
+    for b in fx["blocks"]:
+        nx, ny, nz = (int(v) for v in b["dims"])
+        P = b["pops"].reshape((q, nz, ny, nx))
+        Fd = b["fields"].reshape((d + 1, nz, ny, nx))
+        fl = b["flags"].reshape((nz, ny, nx))
+        if int(b["level"]) == 0:
+            g0 = [int(round(float(b["min"][a]))) + 1 for a in range(d)] + [0] * (3 - d)
+            live = (fl & O.VOID) == 0
+            sl = (slice(g0[2], g0[2] + nz), slice(g0[1], g0[1] + ny), slice(g0[0], g0[0] + nx))
+            mine = popsC[(slice(None),) + sl]
+            mf = fldC[(slice(None),) + sl]
+            diff = np.any(mine != P, axis=0) | (np.any(mf != Fd, axis=0) & (fl == O.AABB))
+            bad += int((diff & live).sum())
+            checked += int(live.sum())
+        else:
+            assert tuple(int(v) for v in b["dims"]) == rc.fdims
+            assert np.float32(b["omega"]) == np.float32(rc.omega_f)
+            mine = rc.popsF.reshape(P.shape)
+            mf = rc.fine.fields().astype(np.float32).reshape(Fd.shape)
+            sel = np.ones((nz, ny, nx), dtype=bool) if fine_mask is None else fine_mask.reshape((nz, ny, nx))
+            bad += int(((np.any(mine != P, axis=0) | np.any(mf != Fd, axis=0)) & sel).sum())
+            checked += int(sel.sum())
+    return bad, checked
+
+
+def gpu_refined(rc, storage="f32"):
+    """The oracle's coarse and fine lattices as GPU blocks plus their
+    Refinement, at the oracle's current state. The fine block's outer ghost
+    layer is its overlap (not updated); its inner ghost layer is updated."""
+    from freelb_amd import BlockLattice, Refinement
+    from gpu_case import gpu_lattice
+    coarse = gpu_lattice(rc.coarse, storage, rc.popsC)
+    fl = rc.fine.flags.reshape(rc.fdims[::-1]).copy()
+    if rc.d == 3:
+        fl[0, :, :] = fl[-1, :, :] = O.COVERED
+        fl[:, 0, :] = fl[:, -1, :] = O.COVERED
+        fl[:, :, 0] = fl[:, :, -1] = O.COVERED
+    else:
+        fl[0, 0, :] = fl[0, -1, :] = O.COVERED
+        fl[0, :, 0] = fl[0, :, -1] = O.COVERED
+    fine_case = O.Lattice(rc.q, rc.fdims, fl.reshape(-1), rc.fine.tasks, [], omega=rc.fine.omega,
+                          rho_field=rc.fine.buffers["rho_field"], u_field=rc.fine.buffers["u_field"])
+    fine = gpu_lattice(fine_case, storage, rc.popsF)
+    ref = Refinement(coarse, fine, rc.origin)
+    return coarse, fine, ref
+
+
+def compare_masks(rc):
+    """Cells whose state the GPU must reproduce: coarse cells that are updated
+    and not covered (live cells and the patch's outer coarse layer), and the
+    fine interior with its inner ghost layer."""
+    cm = (rc.coarse.flags != O.VOID) & (rc.coarse.flags != O.COVERED)
+    fm = np.zeros(rc.fdims[::-1], dtype=bool)
+    if rc.d == 3:
+        fm[1:-1, 1:-1, 1:-1] = True
+    else:
+        fm[0, 1:-1, 1:-1] = True
+    return cm, fm.reshape(-1)

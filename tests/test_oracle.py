@@ -190,3 +190,92 @@ def test_d3q19_mrt_basis_is_orthogonal():
                   (y * y - z * z) * x, (z * z - x * x) * y, (x * x - y * y) * z])
     G = M @ M.T
     assert np.allclose(G - np.diag(np.diag(G)), 0)
+
+
+# ---- two-level refinement --------------------------------------------------------
+
+REFINED_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "refined_*.npz")))
+
+
+@pytest.mark.parametrize("path", REFINED_FIXTURES, ids=os.path.basename)
+def test_refined_cavity_reproduces_reference_bitwise(path):
+    """FreeLB's refined cavity (examples/cavref2d/cavref2d.cpp's loop, several
+    coarse blocks around one fine block) against the oracle's one coarse and
+    one fine block: every population and field of every cell, bit for bit."""
+    from refine_case import fixture_mismatches, load_fixture
+    fx = load_fixture(path)
+    rc = O.RefinedCavity("d2q9", fx["n"], fx["n"], 1, fx["cell"], fx["p0"], fx["p1"])
+    rc.run(fx["counts"])
+    bad, checked = fixture_mismatches(rc, fx)
+    assert checked > 0 and bad == 0, f"{bad} of {checked} cells differ"
+
+
+@pytest.mark.parametrize("lattice", ["d2q9", "d3q19"])
+def test_refinement_keeps_rest_state(lattice):
+    """No lid motion: the rest state f = w is a fixed point of both levels and
+    of both conversions (to FP64 rounding)."""
+    rc = O.RefinedCavity(lattice, 12, 12, 12, 4, 1, 2, u_phys=0.0, dtype=np.float64)
+    w = O.weights(rc.q, np.float64)
+    rc.run(7)
+    cm = (rc.coarse.flags != O.VOID) & (rc.coarse.flags != O.COVERED)
+    np.testing.assert_allclose(rc.popsC[:, cm], np.repeat(w[:, None], int(cm.sum()), axis=1), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(rc.popsF, np.repeat(w[:, None], rc.popsF.shape[1], axis=1), rtol=0, atol=1e-15)
+
+
+def _linear(q, pts, a, b):
+    return a[:, None] + b @ pts.T
+
+
+@pytest.mark.parametrize("lattice", ["d2q9", "d3q19"])
+def test_refinement_reproduces_linear_fields(lattice):
+    """With rho = 1 and u = 0 in the fields feq = w, so both conversions are
+    affine maps of the exchanged value; a population field linear in space
+    must come out as the same linear field at the fine (coarse) cell centres,
+    which fixes the stencil placement, the weights and which sub-cell each
+    fine cell is (a swapped sub-cell or a transposed weight breaks it)."""
+    rc = O.RefinedCavity(lattice, 12, 12, 12, 4, 1, 2, u_phys=0.0, dtype=np.float64)
+    d, q = rc.d, rc.q
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0.5, 1.0, q)
+    b = rng.uniform(-0.01, 0.01, (q, d))
+    w = O.weights(q, np.float64)
+
+    def centres(dims, origin_shift, h):
+        idx = np.indices(dims[::-1]).reshape(len(dims), -1)[::-1][:d].T  # (N, d), x first
+        return origin_shift + (idx + 0.5) * h
+
+    # coarse cell k spans [k, k + 1); fine cell f of the patch at origin o spans
+    # o - 1 + (f + [0, 1]) / 2
+    cpts = centres(rc.cdims, 0.0, 1.0)
+    rc.popsC[:] = _linear(q, cpts, a, b)
+    O.refine_interpolate(rc.coarse, rc.popsC, rc.fine, rc.popsF, rc.origin)
+    fpts = centres(rc.fdims, np.array(rc.origin[:d], dtype=float) - 1.0, 0.5)
+    want = _linear(q, fpts, a, b)
+    want = want - 0.25 * rc.omega_c * (want - w[:, None])
+    fn = rc.fdims
+    ghost = np.zeros(fn[::-1], dtype=bool)
+    if d == 3:
+        ghost[...] = True
+        ghost[2:-2, 2:-2, 2:-2] = False
+    else:
+        ghost[0] = True
+        ghost[0, 2:-2, 2:-2] = False
+    ghost = ghost.reshape(-1)
+    np.testing.assert_allclose(rc.popsF[:, ghost], want[:, ghost], rtol=0, atol=1e-13)
+
+    rc.popsF[:] = _linear(q, fpts, a, b)
+    O.refine_average(rc.coarse, rc.popsC, rc.fine, rc.popsF, rc.origin)
+    want = _linear(q, cpts, a, b)
+    want = want + 0.5 * rc.omega_f * (want - w[:, None])
+    o, m = rc.origin[0], rc.m
+    ring = np.zeros(rc.cdims[::-1], dtype=bool)
+    box = (slice(o, o + m),) * d
+    inner = (slice(o + 1, o + m - 1),) * d
+    if d == 3:
+        ring[box] = True
+        ring[inner] = False
+    else:
+        ring[(0,) + box] = True
+        ring[(0,) + inner] = False
+    ring = ring.reshape(-1)
+    np.testing.assert_allclose(rc.popsC[:, ring], want[:, ring], rtol=0, atol=1e-13)
