@@ -150,13 +150,15 @@ def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
     return out
 
 
-def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", loop_steps=20):
+def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", loop_steps=20, eager=False):
     """512 independent D3Q19 BGK halfway cavities, edges drawn per axis from
     U{16..96}, tau from U[0.51, 0.9], lid speed from U[0.02, 0.1] (numpy
     default_rng(seed)), stepped together: one launch per step over the block
     table (autotuned shape), recorded as a hipGraph of graph_steps steps. The
     same blocks stepped with one launch per block and step (FreeLB's
-    per-block manager loop) are timed beside it."""
+    per-block manager loop) are timed beside it. eager: time graph_steps
+    batch launches issued one by one instead of the graph (for kernel
+    profilers that do not follow graph launches)."""
     from freelb_amd import BlockLatticeManager
     from freelb_amd.cases import cavity_block
     rng = np.random.default_rng(seed)
@@ -180,20 +182,28 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
         loop_ms = timer.stop() / loop_steps
         with BlockLatticeManager(blocks) as m:
             tuned = m.autotune(10)
-            m.capture(graph_steps)
-            m.replay()
-            m.synchronize()
-            timer.start()
-            m.replay()
-            graph_ms = timer.stop() / graph_steps
+            if eager:
+                m.step(2)
+                m.synchronize()
+                timer.start()
+                m.step(graph_steps)
+                graph_ms = timer.stop() / graph_steps
+            else:
+                m.capture(graph_steps)
+                m.replay()
+                m.synchronize()
+                timer.start()
+                m.replay()
+                graph_ms = timer.stop() / graph_steps
             m.synchronize()
             mass = [b.mass_momentum()[0] for b in blocks[:4]]
     finally:
         for b in blocks:
             b.close()
     bytes_per = 152 if storage == "f32" else 76
+    how = f"{graph_steps} eager batch launches" if eager else f"hipGraph of {graph_steps} steps"
     return {"workload": f"{nblocks} cavities, edges U{{16..96}}^3 (seed {seed}), tau U[0.51,0.9], "
-                        f"lid U[0.02,0.1], hipGraph of {graph_steps} steps",
+                        f"lid U[0.02,0.1], {how}",
             "blocks": nblocks, "cells": cells, "storage": storage,
             "mlups": round(cells / (graph_ms / 1e3) / 1e6, 1),
             "ms_per_step": round(graph_ms, 5),
@@ -400,6 +410,8 @@ def main():
     ap.add_argument("--no-cavity", action="store_true", help="with --config4: only the batched adjoint")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
     ap.add_argument("--no-refined", action="store_true", help="skip the refined-cavity run")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="time the batch with launches issued one by one (no hipGraph), e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -463,7 +475,7 @@ def main():
         line["device_copy_gbs"] = round(gbs.value, 1)
         line["roofline"]["frac_of_copy"] = round(f32["achieved_gbs"] / gbs.value, 4)
     if not args.no_batch:
-        line["batch"] = batch_bench(timer)
+        line["batch"] = batch_bench(timer, eager=args.batch_eager, graph_steps=50 if args.batch_eager else 500)
         print(f"# batch: {line['batch']}", file=sys.stderr, flush=True)
     if not args.no_refined:
         line["refined"] = refined_bench(timer)

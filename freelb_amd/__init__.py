@@ -9,6 +9,6 @@ FreeLB's BlockLattice interface.
 from .abi import FLBError  # noqa: F401
 from .adjoint import Adjoint, AdjointBatch  # noqa: F401
 from .lattice import (BlockHalo, BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair,  # noqa: F401
-                      Refinement, weights)
+                      Refinement, weights, write_vtm)
 
-__all__ = ["Adjoint", "AdjointBatch", "BlockHalo", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "Refinement", "weights", "FLBError"]
+__all__ = ["Adjoint", "AdjointBatch", "BlockHalo", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "Refinement", "weights", "write_vtm", "FLBError"]

@@ -36,7 +36,7 @@ EXPORTS = (
     "flb_batch_synchronize", "flb_batch_autotune", "flb_batch_set_halo", "flb_set_ghost_overlap",
     "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange",
     "flb_refine_create", "flb_refine_destroy", "flb_refine_info", "flb_refine_communicate", "flb_refine_run",
-    "flb_download_fields", "flb_adjoint_create", "flb_adjoint_destroy",
+    "flb_download_fields", "flb_enable_tolerance", "flb_tolerance", "flb_store_moments", "flb_write_vtm", "flb_adjoint_create", "flb_adjoint_destroy",
     "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
     "flb_adjoint_timing", "flb_adjoint_batch_create", "flb_adjoint_batch_destroy", "flb_adjoint_batch_forward",
@@ -147,6 +147,12 @@ def lib() -> C.CDLL:
         "flb_refine_communicate": [P, C.c_int64],
         "flb_refine_run": [P, C.c_int64, C.c_int],
         "flb_download_fields": [P, C.POINTER(C.c_double)],
+        "flb_enable_tolerance": [P],
+        "flb_tolerance": [P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "flb_store_moments": [P, C.c_uint8],
+        "flb_write_vtm": [C.POINTER(P), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_int), C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_char_p,
+                          C.c_char_p],
         "flb_adjoint_create": [C.POINTER(AdjointDesc), C.POINTER(P)],
         "flb_adjoint_destroy": [P],
         "flb_adjoint_set_solid": [P, C.POINTER(C.c_uint8)],

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflb_hip.so")
 SOURCES = ["flb_inst_f32.hip", "flb_inst_f16s.hip", "flb_inst_bf16s.hip", "flb_inst_f64.hip", "flb_inst_f16c.hip",
            "flb_probe.hip", "flb_links.hip", "flb_abi.hip", "flb_batch.hip", "flb_halo.hip", "flb_adjoint.hip",
-           "flb_refine.hip"]
+           "flb_refine.hip", "flb_tol.hip", "flb_vtk.hip"]
 HEADERS = ["flb_cell.hpp", "flb_storage.hpp", "flb_internal.hpp", "flb_kernels.hpp", "flb_host.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

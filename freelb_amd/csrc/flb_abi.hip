@@ -353,11 +353,14 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
   // groups (the widest vector row a workgroup covers), plus one cache line of
   // padding that takes the odd step's whole-line stores past the row ends
   const int ex_x = desc->periodic[0] ? desc->nx : desc->nx - 2 * desc->overlap;
-  // Short rows (at most 222 updated cells) take a compact pitch instead, a
-  // multiple of 32 slots with room for the two overlap cells, so that
-  // neighbouring rows share cache lines rather than each filling a mostly
-  // empty 320-slot row.
-  const int pitch = ex_x + 2 <= 224 ? (ex_x + 2 + 31) / 32 * 32
+  // Short rows (at most 222 updated cells) take a compact pitch instead:
+  // room for the two overlap cells, rounded up to FLB_PITCH_ROUND slots
+  // (default 8), so that neighbouring rows share cache lines rather than
+  // each filling a mostly empty 320-slot row (512 cavities of 16..96 cells
+  // per edge: 8 slots +6 % over 32, 1 slot +5 %).
+  int round = 8;
+  if (const char* e = std::getenv("FLB_PITCH_ROUND")) round = std::max(1, std::min(256, std::atoi(e)));
+  const int pitch = ex_x + 2 <= 224 ? (ex_x + 2 + round - 1) / round * round
                                     : std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
   const int64_t plane = int64_t(pitch) * desc->ny;
   // tail: one extra row plus the widest vector run past a row end
@@ -441,7 +444,8 @@ int flb_block_destroy(flb_block b) {
   DeviceGuard g(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   else (void)hipDeviceSynchronize();
-  void* ptrs[] = {b->d_pops, b->d_flags, b->d_rho, b->d_u, b->d_partials, b->d_red, b->d_stage, b->d_links};
+  void* ptrs[] = {b->d_pops,  b->d_flags,   b->d_rho,   b->d_u,       b->d_partials, b->d_red,
+                  b->d_stage, b->d_links, b->d_rho_old, b->d_u_old, b->d_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -608,6 +612,47 @@ int flb_set_field_u(flb_block b, const double* u) {
   if (!u) return fail(FLB_ERR_ARG, "null u");
   DeviceGuard g(b->device);
   return upload_field(b, u, b->d, &b->d_u);
+}
+
+int flb_enable_tolerance(flb_block b) {
+  CHECK_BLOCK(b);
+  if (!b->d_rho || !b->d_u)
+    return fail(FLB_ERR_STATE, "the block has no RHO / VELOCITY fields (FLB_DYN_BGK_WRITE tasks create them)");
+  DeviceGuard g(b->device);
+  const size_t one = size_t(b->stride) * b->real;
+  if (!b->d_rho_old && hipMalloc(&b->d_rho_old, one) != hipSuccess)
+    return fail(FLB_ERR_NOMEM, "residual field allocation failed");
+  if (!b->d_u_old && hipMalloc(&b->d_u_old, one * size_t(b->d)) != hipSuccess)
+    return fail(FLB_ERR_NOMEM, "residual field allocation failed");
+  if (!b->d_res && hipMalloc(reinterpret_cast<void**>(&b->d_res), sizeof(unsigned long long)) != hipSuccess)
+    return fail(FLB_ERR_NOMEM, "residual allocation failed");
+  HIP_TRY(hipMemcpyAsync(b->d_rho_old, b->d_rho, one, hipMemcpyDeviceToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_u_old, b->d_u, one * size_t(b->d), hipMemcpyDeviceToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_tolerance(flb_block b, int which, int shift, double* res) {
+  CHECK_BLOCK(b);
+  if (!res) return fail(FLB_ERR_ARG, "null output");
+  if (which != 0 && which != 1) return fail(FLB_ERR_ARG, "which must be 0 (RHO) or 1 (VELOCITY)");
+  if (shift < 0) return fail(FLB_ERR_ARG, "negative shift");
+  if (!b->d_res) return fail(FLB_ERR_STATE, "flb_enable_tolerance first");
+  if (b->pending) return fail(FLB_ERR_STATE, "residual between ApplyCellDynamics and Stream");
+  DeviceGuard g(b->device);
+  const size_t pad = size_t(flb::kFrontPad) * b->real;
+  const void* field = static_cast<const char*>(which == 0 ? b->d_rho : b->d_u) + pad;
+  void* old = static_cast<char*>(which == 0 ? b->d_rho_old : b->d_u_old) + pad;
+  HIP_TRY(hipMemsetAsync(b->d_res, 0, sizeof(unsigned long long), b->stream));
+  HIP_TRY(flb::launch_tolerance(b->real == 8, field, old, b->stride, which == 0 ? 1 : b->d, b->desc.nx, b->desc.ny,
+                                b->desc.nz, int(axis_lo(b, 0)), b->pitch, b->plane, shift, b->d_res, b->stream));
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, b->d_res, sizeof bits, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof v);
+  *res = v;
+  return FLB_OK;
 }
 
 int flb_download_fields(flb_block b, double* rhou) {

@@ -170,7 +170,9 @@ __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restr
 // ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
 // Workgroup z sums the row (x = outlet, z) of the plane into out[z] (a fixed
 // lane order and a fixed LDS tree, so every run gives the same bits);
-// k_obj_final adds the rows of each step in z order.
+// k_obj_final adds the rows of each step in z order. The plane's cells lie
+// nx apart, so every value is a line fetch of its own: the kernel is bound
+// by those scattered reads, not by its parallelism.
 template <class R>
 __device__ __forceinline__ void obj_row(const AdjArgs<R>& a, const R* __restrict__ f, int z,
                                         double* __restrict__ out) {

@@ -51,6 +51,9 @@ struct flb_block_s {
   uint64_t* d_links = nullptr;  // precomputed halfway wall links (slot layout)
   bool links_dirty = true;      // flags or halfway managers changed since d_links was built
   bool ghost_overlap = false;   // the overlap layer mirrors neighbouring blocks (flb_halo)
+  void* d_rho_old = nullptr;    // RhoOld / UOld of the convergence residual (field slot layout)
+  void* d_u_old = nullptr;
+  unsigned long long* d_res = nullptr;  // residual as the bits of a non-negative double
 };
 
 struct flb_halo_s {

@@ -105,6 +105,14 @@ hipError_t launch_links(int q, const KArgs& a, uint64_t* links, hipStream_t s);
 // float4 copy of n4 16-byte elements, one element per thread (bandwidth calibration).
 hipError_t launch_copy_probe(void* dst, const void* src, size_t n4, hipStream_t s);
 
+// Convergence residual of one field (ncomp components at fstride, field and
+// old in slot layout from slot 0) over the host cells [shift, n - shift) per
+// axis; folds the maximum into *res (bits of a non-negative double) and
+// copies the field into old (flb_tol.hip).
+hipError_t launch_tolerance(bool dbl, const void* field, void* old, int64_t fstride, int ncomp, int nx, int ny,
+                            int nz, int xlo, int pitch, int64_t plane, int shift, unsigned long long* res,
+                            hipStream_t s);
+
 // Workgroups of the per-cell (non-step) kernels for a block of n_cells host cells.
 inline unsigned cell_blocks(int64_t n_cells) {
   return unsigned((n_cells + kMaxThreads - 1) / kMaxThreads);

@@ -63,24 +63,20 @@ using kern::slot_of;
 
 constexpr int kThreads = 256;
 
-// Equilibrium::Order2 with u2 = |u|^2 (lbm/equilibrium.h:49-53, 69-75):
-// w rho (1 + 3 uc + uc uc 0.5 * 9 - 3 u2 0.5), uc and u2 summed from 0 in
-// component order.
+// Equilibrium::Order2 of direction k with u2 = |u|^2 (lbm/equilibrium.h:49-53,
+// 69-75): w rho (1 + 3 uc + uc uc 0.5 * 9 - 3 u2 0.5), uc and u2 summed from 0
+// in component order.
 template <int Q, class R>
-__device__ __forceinline__ void feq_order2(R (&e)[Q], R rho, const R (&u)[3]) {
+__device__ __forceinline__ R feq_order2(int k, R rho, const R (&u)[3]) {
   constexpr int D = Lat<Q>::D;
   R u2 = R(0);
 #pragma unroll
   for (int d = 0; d < D; ++d) u2 += u[d] * u[d];
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    R uc = R(0);
-    uc += u[0] * R(Lat<Q>::cx(k));
-    uc += u[1] * R(Lat<Q>::cy(k));
-    if (D == 3) uc += u[2] * R(Lat<Q>::cz(k));
-    e[k] = Lat<Q>::template w<R>(k) * rho *
-           (R(1) + R(3) * uc + uc * uc * R(0.5) * R(9) - R(3) * u2 * R(0.5));
-  }
+  R uc = R(0);
+  uc += u[0] * R(Lat<Q>::cx(k));
+  uc += u[1] * R(Lat<Q>::cy(k));
+  if (D == 3) uc += u[2] * R(Lat<Q>::cz(k));
+  return Lat<Q>::template w<R>(k) * rho * (R(1) + R(3) * uc + uc * uc * R(0.5) * R(9) - R(3) * u2 * R(0.5));
 }
 
 // getIntpWeight2D / 3D (parallel/communicator.h:575-600): w[D][j] =
@@ -103,19 +99,40 @@ __device__ __forceinline__ void load_rhou(const KArgs& a, int slot, R& rho, R (&
   for (int d = 0; d < 3; ++d) u[d] = d < Lat<Q>::D ? uf[d * a.fstride + slot] : R(0);
 }
 
-// Post-stream populations of (x, y, z) stored where the block's next step
-// reads them (the target cells carry no halfway links).
+// Post-stream population i of cell (x, y, z) at the block's parity, halfway
+// links included (the single-direction form of read_logical).
 template <int Q, class S, int PAR>
-__device__ __forceinline__ void write_post(const KArgs& a, int x, int y, int z, const typename S::R (&f)[Q]) {
-  const int64_t ps = pslot_of(a, x, y, z);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    if (PAR == 0 || i == 0) {
-      arr<S>(a, i)[ps] = S::template to<Q>(f[i], i);
-    } else {
-      const int o = opp(i);
-      arr<S>(a, o)[ps + nb_off<Q, false, true>(a, o, x, y, z)] = S::template to<Q>(f[i], i);
+__device__ __forceinline__ typename S::R read_post(const KArgs& a, int x, int y, int z, int slot, int64_t ps,
+                                                   int i) {
+  using R = typename S::R;
+  if (PAR == 0 || i == 0) return S::template from<Q>(arr<S>(a, i)[ps], i);
+  const int o = opp(i);
+  if (a.bcany && (a.flags[slot] & a.bcany)) {
+    const uint64_t lk = a.links[slot];
+    const unsigned ln = unsigned(lk), lm = unsigned(lk >> 32);
+    if (((ln | lm) >> o) & 1u) {
+      R v = S::template from<Q>(arr<S>(a, i)[ps], i);
+      if ((lm >> o) & 1u) {
+        R rho_f, u_f[3];
+        kern::load_fields<Q>(a, slot, rho_f, u_f);
+        v = v + movingwall_term<Q>(i, rho_f, u_f);
+      }
+      return v;
     }
+  }
+  return S::template from<Q>(arr<S>(a, o)[ps + nb_off<Q, false, true>(a, o, x, y, z)], i);
+}
+
+// Post-stream population i of (x, y, z) stored where the block's next step
+// reads it (target cells carry no halfway links).
+template <int Q, class S, int PAR>
+__device__ __forceinline__ void write_post(const KArgs& a, int x, int y, int z, int i, typename S::R v) {
+  const int64_t ps = pslot_of(a, x, y, z);
+  if (PAR == 0 || i == 0) {
+    arr<S>(a, i)[ps] = S::template to<Q>(v, i);
+  } else {
+    const int o = opp(i);
+    arr<S>(a, o)[ps + nb_off<Q, false, true>(a, o, x, y, z)] = S::template to<Q>(v, i);
   }
 }
 
@@ -131,50 +148,45 @@ struct IntpItem {
   int sub;
 };
 
+// One thread per (item, direction): the 2^d sources' fields are read by all
+// q threads of an item (cached), each thread sums its own direction.
 template <int Q, class S, int PC, int PF>
 __global__ __launch_bounds__(kThreads) void k_average(KArgs c, KArgs f, const AvgItem* __restrict__ items,
                                                       int n, typename S::R omega_f) {
   using R = typename S::R;
   constexpr int D = Lat<Q>::D;
   constexpr int NS = 1 << D;
-  const int k = int(blockIdx.x) * kThreads + int(threadIdx.x);
-  if (k >= n) return;
+  const int t = int(blockIdx.x) * kThreads + int(threadIdx.x);
+  if (t >= n * Q) return;
+  const int k = t / Q, i = t - k * Q;
   const AvgItem it = items[k];
-  R acc[Q], rho = R(0), u[3] = {R(0), R(0), R(0)};
+  R acc = R(0), rho = R(0), u[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int x = it.fx + (j & 1), y = it.fy + ((j >> 1) & 1), z = it.fz + ((j >> 2) & 1);
     const int sl = slot_of(f, x, y, z);
-    R g[Q], r, v[3];
-    read_logical<Q, S, PF>(f, x, y, z, sl, pslot_of(f, x, y, z), g);
+    const R g = read_post<Q, S, PF>(f, x, y, z, sl, pslot_of(f, x, y, z), i);
+    R r, v[3];
     load_rhou<Q>(f, sl, r, v);
     if (j == 0) {
+      acc = g;
       rho = r;
 #pragma unroll
       for (int d = 0; d < 3; ++d) u[d] = v[d];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) acc[i] = g[i];
     } else {
+      acc = acc + g;
       rho = rho + r;
 #pragma unroll
       for (int d = 0; d < 3; ++d) u[d] = u[d] + v[d];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) acc[i] = acc[i] + g[i];
     }
   }
   const R scale = D == 3 ? R(0.125) : R(0.25);
   rho = rho * scale;
 #pragma unroll
   for (int d = 0; d < 3; ++d) u[d] = u[d] * scale;
-  R e[Q];
-  feq_order2<Q>(e, rho, u);
-  const R h = R(0.5) * omega_f;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const R p = acc[i] * scale;
-    acc[i] = p + h * (p - e[i]);
-  }
-  write_post<Q, S, PC>(c, it.cx, it.cy, it.cz, acc);
+  const R e = feq_order2<Q>(i, rho, u);
+  const R p = acc * scale;
+  write_post<Q, S, PC>(c, it.cx, it.cy, it.cz, i, p + R(0.5) * omega_f * (p - e));
 }
 
 template <int Q, class S, int PC, int PF>
@@ -183,45 +195,40 @@ __global__ __launch_bounds__(kThreads) void k_interp(KArgs c, KArgs f, const Int
   using R = typename S::R;
   constexpr int D = Lat<Q>::D;
   constexpr int NS = 1 << D;
-  const int k = int(blockIdx.x) * kThreads + int(threadIdx.x);
-  if (k >= n) return;
+  const int t = int(blockIdx.x) * kThreads + int(threadIdx.x);
+  if (t >= n * Q) return;
+  const int k = t / Q, i = t - k * Q;
   const IntpItem it = items[k];
-  R acc[Q], rho = R(0), u[3] = {R(0), R(0), R(0)};
+  R acc = R(0), rho = R(0), u[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int x = it.cx + (j & 1), y = it.cy + ((j >> 1) & 1), z = it.cz + ((j >> 2) & 1);
     const int sl = slot_of(c, x, y, z);
-    R g[Q], r, v[3];
-    read_logical<Q, S, PC>(c, x, y, z, sl, pslot_of(c, x, y, z), g);
+    const R g = read_post<Q, S, PC>(c, x, y, z, sl, pslot_of(c, x, y, z), i);
+    R r, v[3];
     load_rhou<Q>(c, sl, r, v);
     const R w = intp_weight<D, R>(it.sub, j);
     if (j == 0) {
+      acc = g * w;
       rho = r * w;
 #pragma unroll
       for (int d = 0; d < 3; ++d) u[d] = v[d] * w;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) acc[i] = g[i] * w;
     } else {
+      acc = acc + g * w;
       rho = rho + r * w;
 #pragma unroll
       for (int d = 0; d < 3; ++d) u[d] = u[d] + v[d] * w;
-#pragma unroll
-      for (int i = 0; i < Q; ++i) acc[i] = acc[i] + g[i] * w;
     }
   }
-  R e[Q];
-  feq_order2<Q>(e, rho, u);
-  const R h = R(0.25) * omega_c;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) acc[i] = acc[i] - h * (acc[i] - e[i]);
-  write_post<Q, S, PF>(f, it.fx, it.fy, it.fz, acc);
+  const R e = feq_order2<Q>(i, rho, u);
+  write_post<Q, S, PF>(f, it.fx, it.fy, it.fz, i, acc - R(0.25) * omega_c * (acc - e));
 }
 
 template <int Q, class S>
 hipError_t launch_average_q(const KArgs& c, const KArgs& f, int pc, int pf, const AvgItem* items, int n,
                             double omega_f, hipStream_t s) {
   using R = typename S::R;
-  const dim3 g(unsigned((n + kThreads - 1) / kThreads)), b(kThreads);
+  const dim3 g(unsigned((int64_t(n) * Q + kThreads - 1) / kThreads)), b(kThreads);
   const R om = R(omega_f);
   if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 0>), g, b, 0, s, c, f, items, n, om);
   else if (pc == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 1>), g, b, 0, s, c, f, items, n, om);
@@ -234,7 +241,7 @@ template <int Q, class S>
 hipError_t launch_interp_q(const KArgs& c, const KArgs& f, int pc, int pf, const IntpItem* items, int n,
                            double omega_c, hipStream_t s) {
   using R = typename S::R;
-  const dim3 g(unsigned((n + kThreads - 1) / kThreads)), b(kThreads);
+  const dim3 g(unsigned((int64_t(n) * Q + kThreads - 1) / kThreads)), b(kThreads);
   const R om = R(omega_c);
   if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 0>), g, b, 0, s, c, f, items, n, om);
   else if (pc == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 1>), g, b, 0, s, c, f, items, n, om);
@@ -316,6 +323,9 @@ int check_target(const flb_block_s* b, int x, int y, int z, const char* what) {
 int launch_comm(flb_refine_s* r, bool average) {
   flb_block_s* c = r->coarse;
   flb_block_s* f = r->fine;
+  // sources read their halfway links from the precomputed link words
+  int rc = FLB_OK;
+  if ((rc = ensure_links(c)) || (rc = ensure_links(f))) return rc;
   const KArgs ca = make_args(c), fa = make_args(f);
   const int n = average ? r->navg : r->nintp;
   const void* items = average ? static_cast<const void*>(r->d_avg) : static_cast<const void*>(r->d_intp);

@@ -162,6 +162,21 @@ class BlockLattice:
         abi.check(abi.lib().flb_download_fields(self._h, abi.ptr(out, C.c_double)))
         return out
 
+    def store_moments(self, mask: int = 0xFE) -> None:
+        """rhoU<CELL, true> read-out: rho, u of cells with flag & mask go to the fields."""
+        abi.check(abi.lib().flb_store_moments(self._h, int(mask) & 0xFF))
+
+    def enable_tolerance(self) -> None:
+        """EnableToleranceRho / EnableToleranceU: the current fields become the old ones."""
+        abi.check(abi.lib().flb_enable_tolerance(self._h))
+
+    def tolerance(self, which: str = "u", shift: int = 0) -> float:
+        """getToleranceU / getTolU(shift) (which="u") or the RHO form (which="rho"):
+        max |field - old| over the cells [shift, n - shift), then old = field."""
+        r = C.c_double()
+        abi.check(abi.lib().flb_tolerance(self._h, 0 if which == "rho" else 1, int(shift), C.byref(r)))
+        return r.value
+
     # ---- populations -------------------------------------------------------
     def upload_pops(self, pops: np.ndarray) -> None:
         p = np.ascontiguousarray(pops, dtype=self._real).reshape(-1)
@@ -261,6 +276,24 @@ class BlockHalo:
             self.close()
         except Exception:
             pass
+
+
+def write_vtm(blocks, min_centres, voxels, directory: str, name: str, step: int = -1, strips=None,
+              rho_name: str | None = "Rho", u_name: str | None = "Velocity", flag_name: str | None = None) -> None:
+    """FreeLB's vtmo::vtmWriter::WriteBinary(step) for these blocks: one .vti
+    per block (its RHO / VELOCITY fields and flags) and a .vtm listing them.
+    min_centres[k] is the centre of block k's cell (0, 0, 0), voxels[k] its
+    cell size, strips[k] the layers cut off each side."""
+    blocks = list(blocks)
+    n = len(blocks)
+    arr = (C.c_void_p * n)(*[b.handle for b in blocks])
+    mc = np.ascontiguousarray(np.asarray(min_centres, dtype=np.float64).reshape(n, 3))
+    vx = np.ascontiguousarray(np.asarray(voxels, dtype=np.float64).reshape(n))
+    st = None if strips is None else np.ascontiguousarray(np.asarray(strips, dtype=np.int32).reshape(n))
+    enc = (lambda v: None if v is None else v.encode())
+    abi.check(abi.lib().flb_write_vtm(arr, n, abi.ptr(mc, C.c_double), abi.ptr(vx, C.c_double),
+                                      None if st is None else abi.ptr(st, C.c_int), directory.encode(),
+                                      name.encode(), int(step), enc(rho_name), enc(u_name), enc(flag_name)))
 
 
 class Refinement:

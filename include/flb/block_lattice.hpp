@@ -263,6 +263,19 @@ class BlockLattice {
     return out;
   }
 
+  // BlockLattice::EnableToleranceU / getToleranceU (getTolU(shift)) and the RHO forms
+  void EnableTolerance() { check(flb_enable_tolerance(h_)); }
+  double getToleranceU(int shift = 0) {
+    double r = 0.0;
+    check(flb_tolerance(h_, 1, shift, &r));
+    return r;
+  }
+  double getToleranceRho(int shift = 0) {
+    double r = 0.0;
+    check(flb_tolerance(h_, 0, shift, &r));
+    return r;
+  }
+
   // called by the halfway managers
   void addHalfway(const flb_hwbc& bc) {
     bcs_.push_back(bc);

@@ -167,6 +167,28 @@ int flb_set_field_u(flb_block b, const double* u);                 /* d*N values
  * the block has none): the fields FLB_DYN_BGK_WRITE cells wrote
  * (cell.get<RHO>() / get<VELOCITY>() of the reference). */
 int flb_download_fields(flb_block b, double* rhou);
+/* Convergence residual (BlockLattice::EnableToleranceRho/U, getToleranceRho/U,
+ * getTolRho/getTolU(shift), data_struct/block_lattice.hh:239-367):
+ * flb_enable_tolerance keeps the current RHO and VELOCITY fields as the old
+ * ones; flb_tolerance(which 0 = RHO, 1 = VELOCITY) returns the maximum over
+ * the cells [shift, n - shift) per axis of |rho - rho_old| (of
+ * max_a |u_a - u_old_a|) and makes the current field the old one. */
+int flb_enable_tolerance(flb_block b);
+int flb_tolerance(flb_block b, int which, int shift, double* res);
+/* Read-out moment::rhoU<CELL, true> on the cells with (flag & mask) != 0
+ * (lbm/moment.h:359-377, run before output as examples/cavity3d/cavity3d.cpp:192-194
+ * does): their rho and u go to the RHO / VELOCITY fields. */
+int flb_store_moments(flb_block b, uint8_t mask);
+/* FreeLB's VTK output, vtmo::vtmWriter::WriteBinary(step) (io/vtm_writer.h:288-430,
+ * io/vti_writer.h:424-987): per block <dir>/<name>_T<step>_B<k>.vti (ImageData,
+ * Origin = min_centres[3k..] + strips[k] * voxels[k], strips[k] layers cut off
+ * each side; point data as base64 binary arrays: the RHO field named rho_name,
+ * the VELOCITY field named u_name, the flags named flag_name, any of them NULL
+ * to leave out) and <dir>/<name><step>.vtm listing them; step < 0 drops the
+ * step from the names (WriteBinary()). strips may be NULL (no cut). */
+int flb_write_vtm(const flb_block* blocks, int n, const double* min_centres, const double* voxels,
+                  const int* strips, const char* dir, const char* name, int step, const char* rho_name,
+                  const char* u_name, const char* flag_name);
 
 /* ---- populations --------------------------------------------------------- */
 /* Natural-layout (post-stream) populations [q][N]. _f32 serves the F32, F16S,

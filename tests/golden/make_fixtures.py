@@ -9,7 +9,8 @@ This script only runs that binary and stores its outputs; it needs the
 reference sources, so it runs in the build container, not on the GPU box.
 
     python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures,
-                                                          or "refined" for the refined-cavity ones)
+                                                          "refined" for the refined-cavity ones,
+                                                          "vtk" for FreeLB's VTK files)
 """
 from __future__ import annotations
 
@@ -45,6 +46,10 @@ RT, VISC, U = "0.53", "0.01", "0.1"
 REFINED = [("d2q9", 16, 4, 1, 3, 41), ("d2q9", 24, 4, 2, 4, 34)]
 REF_RT, REF_VISC, REF_U = "0.6", "0.01", "0.05"
 EXE_REF = os.path.join(ROOT, "oracle", "_ref", "refine_ref")
+# FreeLB's own VTK output (oracle/_ref/vtk_ref, vtmo::vtmWriter) of the halfway cavity after the
+# rhoU<CELL, true> read-out: (lattice, Ni, Nj, Nk, steps, overlap threshold, name)
+VTK = [("d3q19", 8, 7, 6, 40, 0, "cav3"), ("d2q9", 16, 13, 1, 100, -1, "cav2")]
+EXE_VTK = os.path.join(ROOT, "oracle", "_ref", "vtk_ref")
 
 
 def read_dump(path):
@@ -112,6 +117,23 @@ def main():
     if not os.path.exists(EXE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        if only in (None, "vtk"):
+            if not os.path.exists(EXE_VTK):
+                subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "_ref/vtk_ref"], check=True)
+            outdir = os.path.join(HERE, "vtk")
+            os.makedirs(outdir, exist_ok=True)
+            for lat, ni, nj, nk, steps, olth, name in VTK:
+                work = os.path.join(tmp, name)
+                os.makedirs(work)
+                subprocess.run([EXE_VTK, lat, str(ni), str(nj), str(nk), str(steps), str(olth), name], cwd=work,
+                               check=True, capture_output=True)
+                src = os.path.join(work, "vtkoutput", "vtidata")
+                for fn in sorted(os.listdir(src)):
+                    with open(os.path.join(src, fn), "rb") as a, open(os.path.join(outdir, fn), "wb") as b:
+                        b.write(a.read())
+                    print("wrote vtk/" + fn, flush=True)
+            if only:
+                return
         if only in (None, "refined"):
             if not os.path.exists(EXE_REF):
                 subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "_ref/refine_ref"], check=True)

@@ -129,3 +129,12 @@ def test_adjoint_batch_null_arguments():
     assert L.flb_adjoint_batch_backward(None, C.byref(J)) == abi.FLB_ERR_ARG
     assert L.flb_adjoint_batch_timing(None, None, None) == abi.FLB_ERR_ARG
     assert "null batch" in L.flb_last_error().decode()
+
+
+def test_output_null_arguments():
+    L = abi.lib()
+    assert L.flb_write_vtm(None, 1, None, None, None, b".", b"x", 0, None, None, None) == abi.FLB_ERR_ARG
+    assert L.flb_store_moments(None, 0xFE) == abi.FLB_ERR_ARG
+    r = C.c_double()
+    assert L.flb_tolerance(None, 1, 0, C.byref(r)) == abi.FLB_ERR_ARG
+    assert L.flb_enable_tolerance(None) == abi.FLB_ERR_ARG
