@@ -126,11 +126,15 @@ __device__ __forceinline__ R geq(int i, R drho, R rho, const R (&u)[3], R usq) {
   return Lat<Q>::template w<R>(i) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq));
 }
 
+// Boxes hold fewer than 2^31 cells (checked at creation), so the index
+// splits with 32-bit divisions (64-bit ones cost several times the
+// instructions).
 __device__ __forceinline__ void cell_xyz(int64_t id, int nx, int ny, int& x, int& y, int& z) {
-  const int64_t yz = id / nx;
-  x = int(id - yz * nx);
-  y = int(yz % ny);
-  z = int(yz / ny);
+  const unsigned i = unsigned(id), yz = i / unsigned(nx);
+  x = int(i - yz * unsigned(nx));
+  const unsigned zz = yz / unsigned(ny);
+  y = int(yz - zz * unsigned(ny));
+  z = int(zz);
 }
 
 // ---- forward step: state src -> dst (push, halfway bounce-back) -------------
@@ -873,7 +877,7 @@ int flb_adjoint_create(const flb_adjoint_desc* desc, flb_adjoint* out) {
   if (!(desc->brinkman_alpha >= 0.0)) return fail(FLB_ERR_ARG, "Brinkman coefficient must be >= 0");
   if (desc->outlet_x < 0 || desc->outlet_x >= desc->nx) return fail(FLB_ERR_ARG, "outlet plane outside the box");
   const int64_t N = int64_t(desc->nx) * desc->ny * desc->nz;
-  if (N * 19 >= (int64_t(1) << 40)) return fail(FLB_ERR_ARG, "box too large");
+  if (N >= (int64_t(1) << 31)) return fail(FLB_ERR_ARG, "box too large (2^31 cells or more)");
   flb_adjoint_s* h = new flb_adjoint_s();
   h->desc = *desc;
   h->N = N;
