@@ -300,10 +300,13 @@ __device__ __forceinline__ void store_row(typename S::E* p, int i, const typenam
 }
 
 // Population loads and stores of the step kernels. Each population is read
-// and written once per step; at the even step no other wavefront touches a
-// cell's lines, so non-temporal accesses (FLB_NT_EVEN) keep them from
-// displacing lines in the caches. The odd step re-reads neighbour lines from
-// L2 and keeps its loads cached (FLB_NT_ODD_STORE marks only its stores).
+// and written once per step; at the even step of a block with wide rows no
+// other wavefront touches a cell's lines, so non-temporal accesses
+// (FLB_NT_EVEN, PAR 0) keep them from displacing lines in the caches. Blocks
+// with compact rows share lines between neighbouring rows and wavefronts, and
+// batches mix blocks: their even step (PAR 2) keeps the lines cached. The odd
+// step re-reads neighbour lines from L2 and keeps its loads cached
+// (FLB_NT_ODD_STORE marks only its stores).
 #ifndef FLB_NT_EVEN
 #define FLB_NT_EVEN 1
 #endif
@@ -375,6 +378,7 @@ template <int Q, class S, int PAR, int FAM, bool PER>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z) {
   using R = typename S::R;
   using E = typename S::E;
+  constexpr bool EVEN = PAR != 1;  // PAR 0: even step with non-temporal accesses, 2: without
   const int x = a.xs + c0;
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
@@ -383,7 +387,7 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   const int dyn = match_task(a, fl, cs);
   if (dyn == DYN_NONE) return;
   unsigned ln = 0u, lm = 0u;
-  if (PAR == 1 && (fl & a.bcany)) {
+  if (!EVEN && (fl & a.bcany)) {
     // halfway links of a boundary cell, precomputed from the flags (k_links)
     const uint64_t m = a.links[s0];
     ln = uint32_t(m);
@@ -393,7 +397,7 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   R f[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    if (PAR == 0) {
+    if (EVEN) {
       f[i] = S::template from<Q>(ld_pop<PAR>((P + int64_t(i) * a.dpop) + c0), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
@@ -404,7 +408,7 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
       f[i] = S::template from<Q>(ld_pop<PAR>(p), i);
     }
   }
-  if (PAR == 1 && lm) {
+  if (!EVEN && lm) {
     R rho_f, u_f[3];
     load_fields<Q>(a, s0, rho_f, u_f);
 #pragma unroll
@@ -414,10 +418,10 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   const R om = R(a.omega);
   collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
   R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
-  if (PAR == 1 && lm) load_fields<Q>(a, s0, rho_f, u_f);
+  if (!EVEN && lm) load_fields<Q>(a, s0, rho_f, u_f);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    if (PAR == 0) {
+    if (EVEN) {
       st_pop<PAR>((P + int64_t(opp(i)) * a.dpop) + c0, S::template to<Q>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
@@ -922,28 +926,38 @@ __global__ __launch_bounds__(kMaxThreads) void k_links(KArgs a, uint64_t* __rest
 // Launch helpers for one storage kind.
 // ---------------------------------------------------------------------------
 template <int Q, class S, int PAR>
+hipError_t step_cell_q(int family, bool per, const KArgs& a, hipStream_t s) {
+  const dim3 grid(a.nblocks), block(kWave * a.wpb);
+  if (family == FAM_FULL) {
+    if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, false>), grid, block, 0, s, a);
+  } else {
+    if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, false>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <int Q, class S, int PAR>
 hipError_t step_q(int family, const KArgs& a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(kWave * a.wpb);
   const bool per = a.px || a.py || a.pz;
-  if (per || a.vec <= 0) {
-    if (family == FAM_FULL) {
-      if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, false>), grid, block, 0, s, a);
-    } else {
-      if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_BGK, false>), grid, block, 0, s, a);
-    }
-  } else if (family == FAM_FULL) {
-    if (a.vec == 1) hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 1>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 2>), grid, block, 0, s, a);
-  } else if (a.vec == 1) {
-    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 1>), grid, block, 0, s, a);
-  } else if (a.vec == 2) {
-    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 2>), grid, block, 0, s, a);
+  if constexpr (PAR == 2) {
+    return step_cell_q<Q, S, 2>(family, per, a, s);
   } else {
-    hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 4>), grid, block, 0, s, a);
+    if (per || a.vec <= 0) return step_cell_q<Q, S, PAR>(family, per, a, s);
+    if (family == FAM_FULL) {
+      if (a.vec == 1) hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 1>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_FULL, 2>), grid, block, 0, s, a);
+    } else if (a.vec == 1) {
+      hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 1>), grid, block, 0, s, a);
+    } else if (a.vec == 2) {
+      hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 2>), grid, block, 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_step_row<Q, S, PAR, FAM_BGK, 4>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
 }
 
 template <int Q, class S, int PAR>
@@ -960,23 +974,32 @@ hipError_t batch_q(int family, bool per, const KArgs* tab, const int64_t* wave0,
   return hipGetLastError();
 }
 
+// batches: the even step without non-temporal accesses (PAR 2)
 template <class S>
 hipError_t launch_batch(int q, int parity, int family, bool per, const KArgs* tab,
                         const int64_t* wave0, const int* wblk, int nb, int64_t groups, int wpb,
                         int grid, hipStream_t s) {
   if (q == 19)
     return parity ? batch_q<19, S, 1>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s)
-                  : batch_q<19, S, 0>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
+                  : batch_q<19, S, 2>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
   if (q == 9)
     return parity ? batch_q<9, S, 1>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s)
-                  : batch_q<9, S, 0>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
+                  : batch_q<9, S, 2>(family, per, tab, wave0, wblk, nb, groups, wpb, grid, s);
   return hipErrorInvalidValue;
 }
 
+// Non-temporal even steps for blocks with wide rows (pitch of whole
+// 256-slot groups) and for the row form; compact rows take PAR 2.
 template <class S>
 hipError_t launch_step(int q, int parity, int family, const KArgs& a, hipStream_t s) {
-  if (q == 19) return parity ? step_q<19, S, 1>(family, a, s) : step_q<19, S, 0>(family, a, s);
-  if (q == 9) return parity ? step_q<9, S, 1>(family, a, s) : step_q<9, S, 0>(family, a, s);
+  const bool per = a.px || a.py || a.pz;
+  const bool nt = a.pitch >= 256 || !(per || a.vec <= 0);
+  if (q == 19)
+    return parity ? step_q<19, S, 1>(family, a, s)
+                  : (nt ? step_q<19, S, 0>(family, a, s) : step_q<19, S, 2>(family, a, s));
+  if (q == 9)
+    return parity ? step_q<9, S, 1>(family, a, s)
+                  : (nt ? step_q<9, S, 0>(family, a, s) : step_q<9, S, 2>(family, a, s));
   return hipErrorInvalidValue;
 }
 

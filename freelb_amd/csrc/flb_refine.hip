@@ -56,10 +56,6 @@ namespace flb {
 namespace refine {
 
 using kern::arr;
-using kern::nb_off;
-using kern::pslot_of;
-using kern::read_logical;
-using kern::slot_of;
 
 constexpr int kThreads = 256;
 
@@ -99,15 +95,23 @@ __device__ __forceinline__ void load_rhou(const KArgs& a, int slot, R& rho, R (&
   for (int d = 0; d < 3; ++d) u[d] = d < Lat<Q>::D ? uf[d * a.fstride + slot] : R(0);
 }
 
-// Post-stream population i of cell (x, y, z) at the block's parity, halfway
-// links included (the single-direction form of read_logical).
-template <int Q, class S, int PAR>
-__device__ __forceinline__ typename S::R read_post(const KArgs& a, int x, int y, int z, int slot, int64_t ps,
-                                                   int i) {
+// Slot offset of the neighbour x + c_o in the population arrays (the
+// refined blocks are not periodic).
+template <int Q>
+__device__ __forceinline__ int64_t pop_off(const KArgs& a, int o) {
+  return int64_t(Lat<Q>::cx(o)) + int64_t(Lat<Q>::cy(o)) * a.ypop + int64_t(Lat<Q>::cz(o)) * a.zpop;
+}
+
+// Post-stream population i of the cell at population slot ps / field slot
+// slot for the block's parity (the single-direction form of read_logical).
+// LINKS: some source cell may carry halfway links (read from the
+// precomputed link words); without them no flag is read.
+template <int Q, class S, int PAR, bool LINKS>
+__device__ __forceinline__ typename S::R read_post(const KArgs& a, int64_t ps, int slot, int i) {
   using R = typename S::R;
   if (PAR == 0 || i == 0) return S::template from<Q>(arr<S>(a, i)[ps], i);
   const int o = opp(i);
-  if (a.bcany && (a.flags[slot] & a.bcany)) {
+  if (LINKS && (a.flags[slot] & a.bcany)) {
     const uint64_t lk = a.links[slot];
     const unsigned ln = unsigned(lk), lm = unsigned(lk >> 32);
     if (((ln | lm) >> o) & 1u) {
@@ -120,37 +124,47 @@ __device__ __forceinline__ typename S::R read_post(const KArgs& a, int x, int y,
       return v;
     }
   }
-  return S::template from<Q>(arr<S>(a, o)[ps + nb_off<Q, false, true>(a, o, x, y, z)], i);
+  return S::template from<Q>(arr<S>(a, o)[ps + pop_off<Q>(a, o)], i);
 }
 
-// Post-stream population i of (x, y, z) stored where the block's next step
-// reads it (target cells carry no halfway links).
+// Post-stream population i of the cell at population slot ps stored where
+// the block's next step reads it (target cells carry no halfway links).
 template <int Q, class S, int PAR>
-__device__ __forceinline__ void write_post(const KArgs& a, int x, int y, int z, int i, typename S::R v) {
-  const int64_t ps = pslot_of(a, x, y, z);
+__device__ __forceinline__ void write_post(const KArgs& a, int64_t ps, int i, typename S::R v) {
   if (PAR == 0 || i == 0) {
     arr<S>(a, i)[ps] = S::template to<Q>(v, i);
   } else {
     const int o = opp(i);
-    arr<S>(a, o)[ps + nb_off<Q, false, true>(a, o, x, y, z)] = S::template to<Q>(v, i);
+    arr<S>(a, o)[ps + pop_off<Q>(a, o)] = S::template to<Q>(v, i);
   }
 }
 
-// One coarse target (cx, cy, cz) and the lower corner (fx, fy, fz) of its fine cells.
+// One coarse target and the lower corner of its fine cells, as population
+// and field slots (cps / csl coarse, fps / fsl fine).
 struct AvgItem {
-  int cx, cy, cz;
-  int fx, fy, fz;
+  int64_t cps, fps;
+  int csl, fsl;
 };
-// One fine target, the lower corner of its coarse stencil and its sub-cell.
+// One fine target, the lower corner of its coarse stencil (slots as above)
+// and its sub-cell.
 struct IntpItem {
-  int fx, fy, fz;
-  int cx, cy, cz;
-  int sub;
+  int64_t fps, cps;
+  int fsl, csl;
+  int sub, pad;
 };
+
+// The 2^d cube cells of a corner, x fastest: population and field slots.
+template <int Q>
+__device__ __forceinline__ int64_t cube_ps(const KArgs& a, int64_t ps, int j) {
+  return ps + (j & 1) + ((j >> 1) & 1) * int64_t(a.ypop) + ((j >> 2) & 1) * a.zpop;
+}
+__device__ __forceinline__ int cube_sl(const KArgs& a, int sl, int j) {
+  return sl + (j & 1) + ((j >> 1) & 1) * a.pitch + ((j >> 2) & 1) * int(a.plane);
+}
 
 // One thread per (item, direction): the 2^d sources' fields are read by all
 // q threads of an item (cached), each thread sums its own direction.
-template <int Q, class S, int PC, int PF>
+template <int Q, class S, int PC, int PF, bool LINKS>
 __global__ __launch_bounds__(kThreads) void k_average(KArgs c, KArgs f, const AvgItem* __restrict__ items,
                                                       int n, typename S::R omega_f) {
   using R = typename S::R;
@@ -163,9 +177,8 @@ __global__ __launch_bounds__(kThreads) void k_average(KArgs c, KArgs f, const Av
   R acc = R(0), rho = R(0), u[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const int x = it.fx + (j & 1), y = it.fy + ((j >> 1) & 1), z = it.fz + ((j >> 2) & 1);
-    const int sl = slot_of(f, x, y, z);
-    const R g = read_post<Q, S, PF>(f, x, y, z, sl, pslot_of(f, x, y, z), i);
+    const int sl = cube_sl(f, it.fsl, j);
+    const R g = read_post<Q, S, PF, LINKS>(f, cube_ps<Q>(f, it.fps, j), sl, i);
     R r, v[3];
     load_rhou<Q>(f, sl, r, v);
     if (j == 0) {
@@ -186,10 +199,10 @@ __global__ __launch_bounds__(kThreads) void k_average(KArgs c, KArgs f, const Av
   for (int d = 0; d < 3; ++d) u[d] = u[d] * scale;
   const R e = feq_order2<Q>(i, rho, u);
   const R p = acc * scale;
-  write_post<Q, S, PC>(c, it.cx, it.cy, it.cz, i, p + R(0.5) * omega_f * (p - e));
+  write_post<Q, S, PC>(c, it.cps, i, p + R(0.5) * omega_f * (p - e));
 }
 
-template <int Q, class S, int PC, int PF>
+template <int Q, class S, int PC, int PF, bool LINKS>
 __global__ __launch_bounds__(kThreads) void k_interp(KArgs c, KArgs f, const IntpItem* __restrict__ items,
                                                      int n, typename S::R omega_c) {
   using R = typename S::R;
@@ -202,9 +215,8 @@ __global__ __launch_bounds__(kThreads) void k_interp(KArgs c, KArgs f, const Int
   R acc = R(0), rho = R(0), u[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const int x = it.cx + (j & 1), y = it.cy + ((j >> 1) & 1), z = it.cz + ((j >> 2) & 1);
-    const int sl = slot_of(c, x, y, z);
-    const R g = read_post<Q, S, PC>(c, x, y, z, sl, pslot_of(c, x, y, z), i);
+    const int sl = cube_sl(c, it.csl, j);
+    const R g = read_post<Q, S, PC, LINKS>(c, cube_ps<Q>(c, it.cps, j), sl, i);
     R r, v[3];
     load_rhou<Q>(c, sl, r, v);
     const R w = intp_weight<D, R>(it.sub, j);
@@ -221,57 +233,64 @@ __global__ __launch_bounds__(kThreads) void k_interp(KArgs c, KArgs f, const Int
     }
   }
   const R e = feq_order2<Q>(i, rho, u);
-  write_post<Q, S, PF>(f, it.fx, it.fy, it.fz, i, acc - R(0.25) * omega_c * (acc - e));
+  write_post<Q, S, PF>(f, it.fps, i, acc - R(0.25) * omega_c * (acc - e));
 }
 
-template <int Q, class S>
+template <int Q, class S, bool LINKS>
 hipError_t launch_average_q(const KArgs& c, const KArgs& f, int pc, int pf, const AvgItem* items, int n,
                             double omega_f, hipStream_t s) {
   using R = typename S::R;
   const dim3 g(unsigned((int64_t(n) * Q + kThreads - 1) / kThreads)), b(kThreads);
   const R om = R(omega_f);
-  if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 0>), g, b, 0, s, c, f, items, n, om);
-  else if (pc == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 1>), g, b, 0, s, c, f, items, n, om);
-  else if (pf == 0) hipLaunchKernelGGL((k_average<Q, S, 1, 0>), g, b, 0, s, c, f, items, n, om);
-  else hipLaunchKernelGGL((k_average<Q, S, 1, 1>), g, b, 0, s, c, f, items, n, om);
+  if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 0, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else if (pc == 0) hipLaunchKernelGGL((k_average<Q, S, 0, 1, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else if (pf == 0) hipLaunchKernelGGL((k_average<Q, S, 1, 0, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else hipLaunchKernelGGL((k_average<Q, S, 1, 1, LINKS>), g, b, 0, s, c, f, items, n, om);
   return hipGetLastError();
 }
 
-template <int Q, class S>
+template <int Q, class S, bool LINKS>
 hipError_t launch_interp_q(const KArgs& c, const KArgs& f, int pc, int pf, const IntpItem* items, int n,
                            double omega_c, hipStream_t s) {
   using R = typename S::R;
   const dim3 g(unsigned((int64_t(n) * Q + kThreads - 1) / kThreads)), b(kThreads);
   const R om = R(omega_c);
-  if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 0>), g, b, 0, s, c, f, items, n, om);
-  else if (pc == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 1>), g, b, 0, s, c, f, items, n, om);
-  else if (pf == 0) hipLaunchKernelGGL((k_interp<Q, S, 1, 0>), g, b, 0, s, c, f, items, n, om);
-  else hipLaunchKernelGGL((k_interp<Q, S, 1, 1>), g, b, 0, s, c, f, items, n, om);
+  if (pc == 0 && pf == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 0, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else if (pc == 0) hipLaunchKernelGGL((k_interp<Q, S, 0, 1, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else if (pf == 0) hipLaunchKernelGGL((k_interp<Q, S, 1, 0, LINKS>), g, b, 0, s, c, f, items, n, om);
+  else hipLaunchKernelGGL((k_interp<Q, S, 1, 1, LINKS>), g, b, 0, s, c, f, items, n, om);
   return hipGetLastError();
 }
 
-template <class S>
+template <class S, bool LINKS>
 hipError_t launch_pair(int q, bool average, const KArgs& c, const KArgs& f, int pc, int pf, const void* items,
                        int n, double omega, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (average) {
     const auto* it = static_cast<const AvgItem*>(items);
-    return q == 19 ? launch_average_q<19, S>(c, f, pc, pf, it, n, omega, s)
-                   : launch_average_q<9, S>(c, f, pc, pf, it, n, omega, s);
+    return q == 19 ? launch_average_q<19, S, LINKS>(c, f, pc, pf, it, n, omega, s)
+                   : launch_average_q<9, S, LINKS>(c, f, pc, pf, it, n, omega, s);
   }
   const auto* it = static_cast<const IntpItem*>(items);
-  return q == 19 ? launch_interp_q<19, S>(c, f, pc, pf, it, n, omega, s)
-                 : launch_interp_q<9, S>(c, f, pc, pf, it, n, omega, s);
+  return q == 19 ? launch_interp_q<19, S, LINKS>(c, f, pc, pf, it, n, omega, s)
+                 : launch_interp_q<9, S, LINKS>(c, f, pc, pf, it, n, omega, s);
 }
 
-hipError_t launch(int storage, int q, bool average, const KArgs& c, const KArgs& f, int pc, int pf,
+template <class S>
+hipError_t launch_kind(int q, bool average, bool links, const KArgs& c, const KArgs& f, int pc, int pf,
+                       const void* items, int n, double omega, hipStream_t s) {
+  return links ? launch_pair<S, true>(q, average, c, f, pc, pf, items, n, omega, s)
+               : launch_pair<S, false>(q, average, c, f, pc, pf, items, n, omega, s);
+}
+
+hipError_t launch(int storage, int q, bool average, bool links, const KArgs& c, const KArgs& f, int pc, int pf,
                   const void* items, int n, double omega, hipStream_t s) {
   switch (storage) {
-    case FLB_STORE_F32: return launch_pair<StoreF32>(q, average, c, f, pc, pf, items, n, omega, s);
-    case FLB_STORE_F16S: return launch_pair<StoreF16S>(q, average, c, f, pc, pf, items, n, omega, s);
-    case FLB_STORE_BF16S: return launch_pair<StoreBF16S>(q, average, c, f, pc, pf, items, n, omega, s);
-    case FLB_STORE_F64: return launch_pair<StoreF64>(q, average, c, f, pc, pf, items, n, omega, s);
-    case FLB_STORE_F16C: return launch_pair<StoreF16C>(q, average, c, f, pc, pf, items, n, omega, s);
+    case FLB_STORE_F32: return launch_kind<StoreF32>(q, average, links, c, f, pc, pf, items, n, omega, s);
+    case FLB_STORE_F16S: return launch_kind<StoreF16S>(q, average, links, c, f, pc, pf, items, n, omega, s);
+    case FLB_STORE_BF16S: return launch_kind<StoreBF16S>(q, average, links, c, f, pc, pf, items, n, omega, s);
+    case FLB_STORE_F64: return launch_kind<StoreF64>(q, average, links, c, f, pc, pf, items, n, omega, s);
+    case FLB_STORE_F16C: return launch_kind<StoreF16C>(q, average, links, c, f, pc, pf, items, n, omega, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -290,6 +309,7 @@ struct flb_refine_s {
   int origin[3] = {0, 0, 0};
   int m[3] = {1, 1, 1};
   int navg = 0, nintp = 0;
+  bool avg_links = false, intp_links = false;  // some source cell is a halfway boundary cell
   RF::AvgItem* d_avg = nullptr;
   RF::IntpItem* d_intp = nullptr;
 };
@@ -306,6 +326,24 @@ bool inside_update(const flb_block_s* b, int x, int y, int z) {
 
 uint8_t flag_at(const flb_block_s* b, int x, int y, int z) {
   return b->h_flags[size_t(x) + size_t(b->desc.nx) * (size_t(y) + size_t(b->desc.ny) * z)];
+}
+
+// Population and field slots of host cell (x, y, z), as the kernels' pslot_of / slot_of.
+int64_t pop_slot(const flb_block_s* b, int x, int y, int z) {
+  return int64_t(x - b->desc.overlap) + int64_t(b->ypop) * y + b->zpop * z;
+}
+int field_slot(const flb_block_s* b, int x, int y, int z) {
+  return int(int64_t(x - b->desc.overlap) + int64_t(b->pitch) * y + b->plane * z);
+}
+
+// Whether any cell of the 2^d cube at (x, y, z) is a halfway boundary cell.
+bool cube_links(const flb_block_s* b, int x, int y, int z) {
+  uint8_t any = 0;
+  for (int k = 0; k < b->nbc; ++k) any |= b->bcs[k].cell_mask;
+  if (!any) return false;
+  for (int j = 0; j < (b->d == 3 ? 8 : 4); ++j)
+    if (flag_at(b, x + (j & 1), y + ((j >> 1) & 1), z + ((j >> 2) & 1)) & any) return true;
+  return false;
 }
 
 // A target cell must be updated (so its next step reads what is written) and
@@ -330,7 +368,9 @@ int launch_comm(flb_refine_s* r, bool average) {
   const int n = average ? r->navg : r->nintp;
   const void* items = average ? static_cast<const void*>(r->d_avg) : static_cast<const void*>(r->d_intp);
   const double om = average ? f->desc.omega : c->desc.omega;
-  HIP_TRY(RF::launch(c->desc.storage, c->desc.q, average, ca, fa, c->parity, f->parity, items, n, om, r->stream));
+  const bool links = average ? r->avg_links : r->intp_links;
+  HIP_TRY(RF::launch(c->desc.storage, c->desc.q, average, links, ca, fa, c->parity, f->parity, items, n, om,
+                     r->stream));
   return FLB_OK;
 }
 
@@ -378,6 +418,7 @@ int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void*
   }
   std::vector<RF::AvgItem> avg;
   std::vector<RF::IntpItem> intp;
+  bool avg_links = false, intp_links = false;
   int rc = FLB_OK;
   // outer coarse layer of the patch
   for (int z = 0; z < (d == 3 ? m[2] : 1); ++z)
@@ -386,9 +427,12 @@ int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void*
         const bool shell = x == 0 || x == m[0] - 1 || y == 0 || y == m[1] - 1 ||
                            (d == 3 && (z == 0 || z == m[2] - 1));
         if (!shell) continue;
-        RF::AvgItem it{o[0] + x, o[1] + y, o[2] + z, 2 + 2 * x, 2 + 2 * y, d == 3 ? 2 + 2 * z : 0};
-        if ((rc = check_target(coarse, it.cx, it.cy, it.cz, "average target"))) return rc;
-        avg.push_back(it);
+        const int cx = o[0] + x, cy = o[1] + y, cz = o[2] + z;
+        const int fx = 2 + 2 * x, fy = 2 + 2 * y, fz = d == 3 ? 2 + 2 * z : 0;
+        if ((rc = check_target(coarse, cx, cy, cz, "average target"))) return rc;
+        avg_links = avg_links || cube_links(fine, fx, fy, fz);
+        avg.push_back(RF::AvgItem{pop_slot(coarse, cx, cy, cz), pop_slot(fine, fx, fy, fz),
+                                  field_slot(coarse, cx, cy, cz), field_slot(fine, fx, fy, fz)});
       }
   // inner fine ghost layer: fine cells 1 .. 2m + 2 with one coordinate on 1 or 2m + 2
   auto unit = [](int fcell, int oo) { return oo + ((fcell - 2) >= 0 ? (fcell - 2) / 2 : -1); };
@@ -405,9 +449,11 @@ int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void*
           corner[a] = unit(fc[a], o[a]) - 1 + bbit;
           sub |= bbit << a;
         }
-        RF::IntpItem it{x, y, z, corner[0], corner[1], corner[2], sub};
         if ((rc = check_target(fine, x, y, z, "interpolation target"))) return rc;
-        intp.push_back(it);
+        intp_links = intp_links || cube_links(coarse, corner[0], corner[1], corner[2]);
+        intp.push_back(RF::IntpItem{pop_slot(fine, x, y, z), pop_slot(coarse, corner[0], corner[1], corner[2]),
+                                    field_slot(fine, x, y, z), field_slot(coarse, corner[0], corner[1], corner[2]),
+                                    sub, 0});
       }
   flb_refine_s* r = new flb_refine_s();
   r->device = coarse->device;
@@ -420,6 +466,8 @@ int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void*
   }
   r->navg = int(avg.size());
   r->nintp = int(intp.size());
+  r->avg_links = avg_links;
+  r->intp_links = intp_links;
   DeviceGuard g(r->device);
   if (hipMalloc(reinterpret_cast<void**>(&r->d_avg), std::max<size_t>(1, avg.size()) * sizeof(RF::AvgItem)) !=
           hipSuccess ||
