@@ -336,6 +336,39 @@ def refined_bench(timer, n=128, patch=64, counts=200, storage="f32"):
             "achieved_gbs": round(updates * bpu / (ms / 1e3) / 1e9, 1), "finite": finite}
 
 
+def coupled_bench(timer, n=256, steps=100, warmup=4):
+    """The differentially heated cavity (freelb_amd.cases.heated_cavity): a
+    D3Q19 flow block with the buoyancy of a D3Q7 temperature block advected by
+    its VELOCITY field, FreeLB's coupled-lattice loop (flow step, then
+    temperature step) for `steps` iterations, FP32. One iteration updates
+    every non-void cell of both lattices: per cell the flow step moves 152 B
+    of populations, reads the temperature (4 B) and writes rho and u (16 B);
+    the temperature step moves 56 B of populations, reads u (12 B) and writes
+    the temperature (4 B): 244 B."""
+    from freelb_amd.cases import heated_cavity
+    flow, temp, cpl = heated_cavity(n, 3)
+    try:
+        cpl.step(warmup)
+        timer.sync()
+        timer.start()
+        cpl.step(steps)
+        ms = timer.stop()
+        T = temp.download_fields()[0]
+        u = flow.download_fields()[1:]
+        finite = bool(np.isfinite(T).all() and np.isfinite(u).all())
+        umax = float(np.abs(u).max())
+    finally:
+        temp.close()
+        flow.close()
+    cells = n ** 3
+    bpu = 244
+    return {"workload": f"heated cavity {n}^3: D3Q19 flow + D3Q7 temperature, buoyancy coupling, {steps} "
+                        "iterations, f32", "cells": cells, "ms_per_iteration": round(ms / steps, 4),
+            "mlups": round(cells * steps / (ms / 1e3) / 1e6, 1), "bytes_per_cell": bpu,
+            "achieved_gbs": round(cells * steps * bpu / (ms / 1e3) / 1e9, 1), "u_max": round(umax, 6),
+            "finite": finite}
+
+
 def profiled_traffic(storage):
     """HBM bytes per launch of the step kernel for this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
@@ -410,6 +443,7 @@ def main():
     ap.add_argument("--no-cavity", action="store_true", help="with --config4: only the batched adjoint")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
     ap.add_argument("--no-refined", action="store_true", help="skip the refined-cavity run")
+    ap.add_argument("--no-coupled", action="store_true", help="skip the coupled flow + temperature run")
     ap.add_argument("--batch-eager", action="store_true",
                     help="time the batch with launches issued one by one (no hipGraph), e.g. under a profiler")
     args = ap.parse_args()
@@ -480,6 +514,9 @@ def main():
     if not args.no_refined:
         line["refined"] = refined_bench(timer)
         print(f"# refined: {line['refined']}", file=sys.stderr, flush=True)
+    if not args.no_coupled:
+        line["coupled"] = coupled_bench(timer)
+        print(f"# coupled: {line['coupled']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)

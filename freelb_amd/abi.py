@@ -16,18 +16,19 @@ LIB_PATH = os.environ.get("FLB_LIB_PATH") or os.path.join(HERE, "libflb_hip.so")
 
 # ---- enums (include/flb_abi.h) ---------------------------------------------
 FLB_OK, FLB_ERR_ARG, FLB_ERR_STATE, FLB_ERR_HIP, FLB_ERR_UNSUPPORTED, FLB_ERR_LAYOUT, FLB_ERR_NOMEM = range(7)
-D2Q9, D3Q19 = 9, 19
+D2Q9, D3Q19, D2Q5, D3Q7 = 9, 19, 5, 7
 STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64, STORE_F16C = range(5)
 STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64, "f16c": STORE_F16C}
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK,
- DYN_SMAG_MRT, DYN_BGK_FORCE, DYN_BGK_WRITE) = range(10)
-HWBC_NORMAL, HWBC_MOVING = 0, 1
+ DYN_SMAG_MRT, DYN_BGK_FORCE, DYN_BGK_WRITE, DYN_BGK_SCALAR_FORCE, DYN_SCALAR_BGK,
+ DYN_SCALAR_BGK_FIELD) = range(13)
+HWBC_NORMAL, HWBC_MOVING, HWBC_ANTI = 0, 1, 2
 
 # Every symbol include/flb_abi.h declares (checked by the CPU test suite).
 EXPORTS = (
     "flb_last_error", "flb_abi_version", "flb_block_create", "flb_block_destroy", "flb_get_info",
     "flb_synchronize", "flb_set_tasks", "flb_set_halfway_bc", "flb_set_mrt_rates", "flb_set_omega",
-    "flb_set_force",
+    "flb_set_force", "flb_share_velocity", "flb_add_buoyancy", "flb_clear_buoyancy",
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
     "flb_upload_pops_f64", "flb_download_pops_f32", "flb_download_pops_f64", "flb_init_equilibrium",
     "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
@@ -111,6 +112,9 @@ def lib() -> C.CDLL:
         "flb_set_mrt_rates": [P, C.POINTER(C.c_double), C.c_int],
         "flb_set_omega": [P, C.c_double],
         "flb_set_force": [P, C.POINTER(C.c_double), C.c_int],
+        "flb_share_velocity": [P, P],
+        "flb_add_buoyancy": [P, P, C.c_uint8, C.c_double, C.c_double],
+        "flb_clear_buoyancy": [P],
         "flb_upload_flags": [P, C.POINTER(C.c_uint8)],
         "flb_set_field_rho": [P, C.POINTER(C.c_double)],
         "flb_set_field_u": [P, C.POINTER(C.c_double)],

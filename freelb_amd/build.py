@@ -74,7 +74,7 @@ def build(verbose: bool = False, jobs: int = 0, extra_flags=(), lib: str = LIB,
     return lib
 
 
-EXAMPLES = {"cavity3d": "cavity3d.cpp", "cavref3d": "cavref3d.cpp"}
+EXAMPLES = {"cavity3d": "cavity3d.cpp", "cavref3d": "cavref3d.cpp", "natconv3d": "natconv3d.cpp"}
 
 
 def build_examples(verbose: bool = False) -> list:

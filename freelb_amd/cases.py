@@ -187,3 +187,58 @@ def refined_cavity(n: int = 128, patch: int = 64, storage: str = "f32", tau: flo
             fine.close()
         raise
     return coarse, fine, ref
+
+
+# Flags of the differentially heated cavity: walls (4) adiabatic, the x = 1
+# layer hot (8) and the x = n layer cold (16), edges included.
+HOT, COLD = 8, 16
+
+
+def heated_cavity_flags(n: int, d: int = 3) -> np.ndarray:
+    if d == 3:
+        f = np.full((n + 2, n + 2, n + 2), VOID, dtype=np.uint8)
+        f[1:-1, 1:-1, 1:-1] = WALL
+        f[2:-2, 2:-2, 2:-2] = FLUID
+        f[1:-1, 1:-1, 1] = HOT
+        f[1:-1, 1:-1, n] = COLD
+    else:
+        f = np.full((1, n + 2, n + 2), VOID, dtype=np.uint8)
+        f[0, 1:-1, 1:-1] = WALL
+        f[0, 2:-2, 2:-2] = FLUID
+        f[0, 1:-1, 1] = HOT
+        f[0, 1:-1, n] = COLD
+    return f.reshape(-1)
+
+
+def heated_cavity(n: int = 256, d: int = 3, storage: str = "f32", tau_flow: float = 0.6, tau_temp: float = 0.7,
+                  gbeta: float = 1e-4, t_hot: float = 1.5, t_cold: float = 0.5, t_init: float = 1.0):
+    """A differentially heated cavity of n^d cells on the GPU: a flow block
+    (D3Q19 / D2Q9, BGKForce with the scalar buoyancy force, halfway walls) and
+    a temperature block (D3Q7 / D2Q5, advected by the flow's VELOCITY field,
+    adiabatic walls and fixed-temperature hot / cold walls), coupled by
+    force::Buoyancy; FreeLB's coupled-lattice loop of
+    examples/cazsblock2d/cazsblock2d.cpp:385-409 without the cellular
+    automaton. Returns (flow, temperature, coupling)."""
+    from .lattice import BlockLatManagerCoupling, weights
+    qf, qt = (19, 7) if d == 3 else (9, 5)
+    dims = (n + 2, n + 2, n + 2) if d == 3 else (n + 2, n + 2, 1)
+    flags = heated_cavity_flags(n, d)
+    alls = FLUID | WALL | HOT | COLD
+    flow = BlockLattice(qf, dims, 1.0 / tau_flow, storage=storage)
+    temp = BlockLattice(qt, dims, 1.0 / tau_temp, storage=storage)
+    for b in (flow, temp):
+        b.upload_flags(flags)
+    flow.set_tasks([Key_TypePair(alls, abi.DYN_BGK_SCALAR_FORCE)])
+    flow.set_halfway_bc([HalfwayBB(WALL | HOT | COLD, VOID)])
+    temp.set_tasks([Key_TypePair(FLUID | WALL, abi.DYN_SCALAR_BGK),
+                    Key_TypePair(HOT | COLD, abi.DYN_SCALAR_BGK_FIELD)])
+    temp.set_halfway_bc([HalfwayBB(WALL, VOID), HalfwayBB(HOT | COLD, VOID, anti=True)])
+    T0 = np.where(flags == HOT, t_hot, np.where(flags == COLD, t_cold, t_init)).astype(np.float64)
+    temp.set_field_rho(T0)
+    temp.share_velocity(flow)
+    cpl = BlockLatManagerCoupling(flow, temp)
+    cpl.add_buoyancy(alls, gbeta, t_init)
+    flow.init_uniform(1.0)
+    real = np.float64 if storage == "f64" else np.float32
+    temp.upload_pops((weights(qt, real)[:, None] * T0.astype(real)[None, :]).astype(real))
+    return flow, temp, cpl

@@ -126,7 +126,8 @@ int family_of(const flb_block_s* b);
 // launch) and with the wide row pitch its stores past the row end need.
 int vec_of(const flb_block_s* b) {
   const bool per = b->desc.periodic[0] || b->desc.periodic[1] || b->desc.periodic[2];
-  if (per || b->nbc > 0) return 0;
+  // scalar (D2Q5 / D3Q7) blocks step in the cell form only
+  if (per || b->nbc > 0 || b->desc.q == FLB_D2Q5 || b->desc.q == FLB_D3Q7) return 0;
   int v = b->vec;
   if (v <= 0) return 0;
   if (family_of(b) == flb::FAM_FULL && v > 2) v = 2;
@@ -206,6 +207,17 @@ KArgs make_args(const flb_block_s* b) {
   a.omega = b->desc.omega;
   for (int j = 0; j < 19; ++j) a.rates[j] = b->rates[j];
   for (int d = 0; d < 3; ++d) a.force[d] = b->force[d];
+  const flb_block_s* v = b->vel_src;
+  a.u_ext = v && v->d_u ? static_cast<char*>(v->d_u) + flb::kFrontPad * v->real : nullptr;
+  a.ncpl = 0;
+  for (const auto& c : b->buoy) {
+    if (!c.src->d_rho || a.ncpl == flb::kMaxCouplings) continue;
+    a.cpl_mask[a.ncpl] = c.mask;
+    a.cpl_rho[a.ncpl] = static_cast<char*>(c.src->d_rho) + flb::kFrontPad * c.src->real;
+    a.cpl_gbeta[a.ncpl] = c.gbeta;
+    a.cpl_rho0[a.ncpl] = c.rho0;
+    ++a.ncpl;
+  }
   return a;
 }
 
@@ -226,14 +238,93 @@ void to_slots(const flb_block_s* b, const U* dense, T* slots, T fill) {
             T(dense[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * z)]);
 }
 
+bool scalar_lattice(const flb_block_s* b) { return b->desc.q == FLB_D2Q5 || b->desc.q == FLB_D3Q7; }
+
 int family_of(const flb_block_s* b) {
+  if (scalar_lattice(b)) return flb::FAM_SCALAR;
   for (int t = 0; t < b->ntasks; ++t) {
     const int d = b->tasks[t].dynamics;
     if (d == FLB_DYN_MRT || d == FLB_DYN_SMAG_BGK || d == FLB_DYN_SMAG_MRT || d == FLB_DYN_BGK_FORCE ||
-        d == FLB_DYN_BGK_WRITE)
+        d == FLB_DYN_BGK_WRITE || d == FLB_DYN_BGK_SCALAR_FORCE)
       return flb::FAM_FULL;
   }
   return flb::FAM_BGK;
+}
+
+int upload_field(flb_block b, const double* host, int ncomp, void** dst);
+
+// RHO / VELOCITY fields at FreeLB's initial values (rho 1, u 0) for a block
+// that has none yet.
+int ensure_fields(flb_block b) {
+  int rc = FLB_OK;
+  DeviceGuard g(b->device);
+  if (!b->d_rho) {
+    const std::vector<double> one(size_t(b->N), 1.0);
+    if ((rc = upload_field(b, one.data(), 1, &b->d_rho))) return rc;
+  }
+  if (!b->d_u) {
+    const std::vector<double> zero(size_t(b->N) * size_t(b->d), 0.0);
+    if ((rc = upload_field(b, zero.data(), b->d, &b->d_u))) return rc;
+  }
+  return FLB_OK;
+}
+
+// Drops every coupling reference between b and other blocks (both directions).
+void detach_couplings(flb_block b) {
+  auto drop_user = [](flb_block from, flb_block user) {
+    auto& u = from->users;
+    for (size_t k = 0; k < u.size(); ++k)
+      if (u[k] == user) {
+        u.erase(u.begin() + std::ptrdiff_t(k));
+        return;
+      }
+  };
+  if (b->vel_src) drop_user(b->vel_src, b);
+  for (const auto& c : b->buoy) drop_user(c.src, b);
+  for (flb_block u : b->users) {
+    if (u->vel_src == b) {
+      u->vel_src = nullptr;
+      u->version++;
+    }
+    for (size_t k = u->buoy.size(); k-- > 0;)
+      if (u->buoy[k].src == b) {
+        u->buoy.erase(u->buoy.begin() + std::ptrdiff_t(k));
+        u->version++;
+      }
+  }
+  b->vel_src = nullptr;
+  b->buoy.clear();
+  b->users.clear();
+}
+
+// Drops the velocity source of a scalar block.
+void detach_couplings_one(flb_block b) {
+  if (!b->vel_src) return;
+  auto& u = b->vel_src->users;
+  for (size_t k = 0; k < u.size(); ++k)
+    if (u[k] == b) {
+      u.erase(u.begin() + std::ptrdiff_t(k));
+      break;
+    }
+  b->vel_src = nullptr;
+  b->version++;
+}
+
+// Two blocks of one coupled pair: same cells, same slot layout, same compute
+// type and device, and lattices of one dimension (flow D2Q9 / D3Q19 with
+// scalar D2Q5 / D3Q7).
+int check_pair(flb_block flow, flb_block scal) {
+  if (scalar_lattice(flow) || !scalar_lattice(scal))
+    return fail(FLB_ERR_ARG, "expected a flow block (D2Q9 / D3Q19) and a scalar block (D2Q5 / D3Q7)");
+  if (flow->d != scal->d) return fail(FLB_ERR_ARG, "flow and scalar blocks of different dimension");
+  const flb_block_desc &a = flow->desc, &c = scal->desc;
+  if (a.nx != c.nx || a.ny != c.ny || a.nz != c.nz || a.overlap != c.overlap || a.periodic[0] != c.periodic[0] ||
+      a.periodic[1] != c.periodic[1] || (flow->d == 3 && a.periodic[2] != c.periodic[2]) ||
+      flow->pitch != scal->pitch || flow->stride != scal->stride)
+    return fail(FLB_ERR_ARG, "flow and scalar blocks cover different cells");
+  if (flow->real != scal->real) return fail(FLB_ERR_ARG, "flow and scalar blocks compute in different types");
+  if (flow->device != scal->device) return fail(FLB_ERR_ARG, "flow and scalar blocks on different devices");
+  return FLB_OK;
 }
 
 // A non-periodic axis's overlap layer is not visited by the kernels; a cell
@@ -335,14 +426,18 @@ int flb_abi_version(void) { return FLB_ABI_VERSION; }
 int flb_block_create(const flb_block_desc* desc, flb_block* out) {
   if (!desc || !out) return fail(FLB_ERR_ARG, "null descriptor or output");
   *out = nullptr;
-  if (desc->q != FLB_D2Q9 && desc->q != FLB_D3Q19)
-    return fail(FLB_ERR_ARG, "q must be 9 (D2Q9) or 19 (D3Q19), got %d", desc->q);
+  if (desc->q != FLB_D2Q9 && desc->q != FLB_D3Q19 && desc->q != FLB_D2Q5 && desc->q != FLB_D3Q7)
+    return fail(FLB_ERR_ARG, "q must be 9 (D2Q9), 19 (D3Q19), 5 (D2Q5) or 7 (D3Q7), got %d", desc->q);
   if (!storage_ok(desc->storage)) return fail(FLB_ERR_ARG, "unknown storage kind %d", desc->storage);
+  const bool scalar = desc->q == FLB_D2Q5 || desc->q == FLB_D3Q7;
+  if (scalar && desc->storage != FLB_STORE_F32 && desc->storage != FLB_STORE_F64)
+    return fail(FLB_ERR_ARG, "D2Q5 / D3Q7 blocks store FP32 or FP64 populations");
   if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1)
     return fail(FLB_ERR_ARG, "extents must be positive (%d,%d,%d)", desc->nx, desc->ny, desc->nz);
-  if (desc->q == FLB_D2Q9 && desc->nz != 1) return fail(FLB_ERR_ARG, "D2Q9 needs nz == 1");
+  if ((desc->q == FLB_D2Q9 || desc->q == FLB_D2Q5) && desc->nz != 1)
+    return fail(FLB_ERR_ARG, "2-D lattices need nz == 1");
   if (desc->overlap < 1) return fail(FLB_ERR_ARG, "overlap must be >= 1");
-  const int d = desc->q == FLB_D3Q19 ? 3 : 2;
+  const int d = (desc->q == FLB_D3Q19 || desc->q == FLB_D3Q7) ? 3 : 2;
   const int n[3] = {desc->nx, desc->ny, desc->nz};
   for (int ax = 0; ax < d; ++ax) {
     const int need = desc->periodic[ax] ? 1 : 2 * desc->overlap + 1;
@@ -444,6 +539,7 @@ int flb_block_destroy(flb_block b) {
   DeviceGuard g(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   else (void)hipDeviceSynchronize();
+  detach_couplings(b);
   void* ptrs[] = {b->d_pops,  b->d_flags,   b->d_rho,   b->d_u,       b->d_partials, b->d_red,
                   b->d_stage, b->d_links, b->d_rho_old, b->d_u_old, b->d_res};
   for (void* p : ptrs)
@@ -484,9 +580,15 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
   b->version++;
   if (n < 0 || n > flb::kMaxTasks) return fail(FLB_ERR_ARG, "task count %d outside [0, %d]", n, flb::kMaxTasks);
   if (n > 0 && !tasks) return fail(FLB_ERR_ARG, "null task table");
+  const bool scalar = scalar_lattice(b);
   for (int t = 0; t < n; ++t) {
-    if (tasks[t].dynamics > FLB_DYN_BGK_WRITE)
-      return fail(FLB_ERR_ARG, "task %d: unknown dynamics %d", t, tasks[t].dynamics);
+    const int dyn = tasks[t].dynamics;
+    if (dyn > FLB_DYN_SCALAR_BGK_FIELD)
+      return fail(FLB_ERR_ARG, "task %d: unknown dynamics %d", t, dyn);
+    const bool scalar_dyn = dyn == FLB_DYN_SCALAR_BGK || dyn == FLB_DYN_SCALAR_BGK_FIELD;
+    if (dyn != FLB_DYN_NONE && scalar != scalar_dyn)
+      return fail(FLB_ERR_ARG, "task %d: dynamics %d is for %s lattices", t, dyn,
+                  scalar_dyn ? "D2Q5 / D3Q7" : "D2Q9 / D3Q19");
     if (tasks[t].mask == 0) return fail(FLB_ERR_ARG, "task %d: zero mask never matches", t);
   }
   flb_task saved[flb::kMaxTasks];
@@ -500,21 +602,16 @@ int flb_set_tasks(flb_block b, const flb_task* tasks, int n) {
     b->ntasks = saved_n;
     return rc;
   }
-  // rhoU<CELL, true> writes the RHO / VELOCITY fields: give the block fields
-  // at FreeLB's initial values (rho 1, u 0) when it has none yet
-  bool writes = false;
-  for (int t = 0; t < n; ++t) writes = writes || tasks[t].dynamics == FLB_DYN_BGK_WRITE;
-  if (writes && (!b->d_rho || !b->d_u)) {
-    DeviceGuard g(b->device);
-    if (!b->d_rho) {
-      const std::vector<double> one(size_t(b->N), 1.0);
-      if ((rc = upload_field(b, one.data(), 1, &b->d_rho))) return rc;
-    }
-    if (!b->d_u) {
-      const std::vector<double> zero(size_t(b->N) * size_t(b->d), 0.0);
-      if ((rc = upload_field(b, zero.data(), b->d, &b->d_u))) return rc;
-    }
+  // rhoU<CELL, true>, forcerhoU<..., true> and the scalar dynamics read or
+  // write the RHO / VELOCITY fields: give the block fields at FreeLB's
+  // initial values (rho 1, u 0) when it has none yet
+  bool fields = false;
+  for (int t = 0; t < n; ++t) {
+    const int dyn = tasks[t].dynamics;
+    fields = fields || dyn == FLB_DYN_BGK_WRITE || dyn == FLB_DYN_BGK_SCALAR_FORCE || dyn == FLB_DYN_SCALAR_BGK ||
+             dyn == FLB_DYN_SCALAR_BGK_FIELD;
   }
+  if (fields) return ensure_fields(b);
   return FLB_OK;
 }
 
@@ -524,14 +621,66 @@ int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n) {
   if (n < 0 || n > flb::kMaxBc) return fail(FLB_ERR_ARG, "boundary count %d outside [0, %d]", n, flb::kMaxBc);
   if (n > 0 && !bcs) return fail(FLB_ERR_ARG, "null boundary table");
   for (int k = 0; k < n; ++k) {
-    if (bcs[k].kind != FLB_HWBC_NORMAL && bcs[k].kind != FLB_HWBC_MOVING)
+    if (bcs[k].kind != FLB_HWBC_NORMAL && bcs[k].kind != FLB_HWBC_MOVING && bcs[k].kind != FLB_HWBC_ANTI)
       return fail(FLB_ERR_ARG, "boundary %d: unknown kind %d", k, bcs[k].kind);
+    if (bcs[k].kind == FLB_HWBC_MOVING && scalar_lattice(b))
+      return fail(FLB_ERR_ARG, "boundary %d: moving walls are for D2Q9 / D3Q19 lattices", k);
+    if (bcs[k].kind == FLB_HWBC_ANTI && !scalar_lattice(b))
+      return fail(FLB_ERR_ARG, "boundary %d: anti-bounce-back is for D2Q5 / D3Q7 lattices", k);
     if (bcs[k].void_mask == 0)
       return fail(FLB_ERR_ARG, "boundary %d: void mask 0 (isFlag needs a non-zero flag)", k);
   }
   for (int k = 0; k < n; ++k) b->bcs[k] = bcs[k];
   b->nbc = n;
   b->links_dirty = true;
+  for (int k = 0; k < n; ++k)
+    if (bcs[k].kind == FLB_HWBC_ANTI) return ensure_fields(b);  // the wall value is the RHO field
+  return FLB_OK;
+}
+
+int flb_share_velocity(flb_block scalar, flb_block flow) {
+  CHECK_BLOCK(scalar);
+  if (!flow) {
+    detach_couplings_one(scalar);
+    return FLB_OK;
+  }
+  int rc = check_pair(flow, scalar);
+  if (rc) return rc;
+  if ((rc = ensure_fields(flow)) || (rc = ensure_fields(scalar))) return rc;
+  detach_couplings_one(scalar);
+  scalar->vel_src = flow;
+  flow->users.push_back(scalar);
+  scalar->version++;
+  return FLB_OK;
+}
+
+int flb_add_buoyancy(flb_block flow, flb_block scalar, uint8_t mask, double gbeta, double rho_init) {
+  CHECK_BLOCK(flow);
+  CHECK_BLOCK(scalar);
+  int rc = check_pair(flow, scalar);
+  if (rc) return rc;
+  if (mask == 0) return fail(FLB_ERR_ARG, "zero mask never matches");
+  if (int(flow->buoy.size()) >= flb::kMaxCouplings)
+    return fail(FLB_ERR_ARG, "a flow block takes at most %d buoyancy couplings", flb::kMaxCouplings);
+  if ((rc = ensure_fields(flow)) || (rc = ensure_fields(scalar))) return rc;
+  flow->buoy.push_back({scalar, mask, gbeta, rho_init});
+  scalar->users.push_back(flow);
+  flow->version++;
+  return FLB_OK;
+}
+
+int flb_clear_buoyancy(flb_block flow) {
+  CHECK_BLOCK(flow);
+  for (const auto& c : flow->buoy) {
+    auto& u = c.src->users;
+    for (size_t k = 0; k < u.size(); ++k)
+      if (u[k] == flow) {
+        u.erase(u.begin() + std::ptrdiff_t(k));
+        break;
+      }
+  }
+  flow->buoy.clear();
+  flow->version++;
   return FLB_OK;
 }
 

@@ -190,6 +190,8 @@ int flb_batch_create(const flb_block* blocks, int n, void* stream, flb_batch* ou
   *out = nullptr;
   const flb_block_s* b0 = blocks[0];
   if (!b0) return fail(FLB_ERR_ARG, "null block 0");
+  if (b0->desc.q != FLB_D2Q9 && b0->desc.q != FLB_D3Q19)
+    return fail(FLB_ERR_UNSUPPORTED, "batches take D2Q9 / D3Q19 blocks");
   for (int k = 0; k < n; ++k) {
     const flb_block_s* b = blocks[k];
     if (!b) return fail(FLB_ERR_ARG, "null block %d", k);

@@ -63,7 +63,95 @@ struct Lat<9> {
   }
 };
 
+// Advection-diffusion sets D2Q5 (lattice_set.h:144-155) and D3Q7 (196-207):
+// the rest direction and the axis pairs, weights 1/3, 1/6 and 1/4, 1/8. Both
+// keep cs2 = 1/3 in the equilibrium, as FreeLB's do.
+template <>
+struct Lat<5> {
+  static constexpr int D = 2;
+  __host__ __device__ static constexpr int cx(int i) { return i == 1 ? 1 : (i == 2 ? -1 : 0); }
+  __host__ __device__ static constexpr int cy(int i) { return i == 3 ? 1 : (i == 4 ? -1 : 0); }
+  __host__ __device__ static constexpr int cz(int) { return 0; }
+  __host__ __device__ static constexpr int wclass(int i) { return i == 0 ? 0 : 1; }
+  template <class R>
+  __host__ __device__ static constexpr R w(int i) {
+    return i == 0 ? R(1) / R(3) : R(1) / R(6);
+  }
+};
+
+template <>
+struct Lat<7> {
+  static constexpr int D = 3;
+  __host__ __device__ static constexpr int cx(int i) { return i == 1 ? 1 : (i == 2 ? -1 : 0); }
+  __host__ __device__ static constexpr int cy(int i) { return i == 3 ? 1 : (i == 4 ? -1 : 0); }
+  __host__ __device__ static constexpr int cz(int i) { return i == 5 ? 1 : (i == 6 ? -1 : 0); }
+  __host__ __device__ static constexpr int wclass(int i) { return i == 0 ? 0 : 1; }
+  template <class R>
+  __host__ __device__ static constexpr R w(int i) {
+    return i == 0 ? R(1) / R(4) : R(1) / R(8);
+  }
+};
+
+// Advection-diffusion (scalar) velocity sets.
+template <int Q>
+constexpr bool kScalarLat = Q == 5 || Q == 7;
+
 __host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+// moment::rhoImpl<D2Q5 / D3Q7> (moment.ur.h:47-82): the scalar (temperature,
+// concentration) is the left-to-right sum of the populations.
+template <class R, int Q>
+__device__ __forceinline__ R rho_sum(const R (&c)[Q]) {
+  R r = c[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) r = r + c[i];
+  return r;
+}
+
+// Read-out of a scalar lattice for flb_moments: the sum and sum c f / sum.
+template <class R>
+__device__ __forceinline__ void rhou(const R (&c)[5], R& rho, R (&u)[3]) {
+  rho = rho_sum(c);
+  u[0] = (c[1] - c[2]) / rho;
+  u[1] = (c[3] - c[4]) / rho;
+  u[2] = R(0);
+}
+template <class R>
+__device__ __forceinline__ void rhou(const R (&c)[7], R& rho, R (&u)[3]) {
+  rho = rho_sum(c);
+  u[0] = (c[1] - c[2]) / rho;
+  u[1] = (c[3] - c[4]) / rho;
+  u[2] = (c[5] - c[6]) / rho;
+}
+
+// equilibrium::SecondOrderImpl<D2Q5> (equilibrium.ur.h:43-63) and <D3Q7>
+// (equilibrium.ur.h:96-119): one weight class per axis pair.
+template <class R, int Q>
+__device__ __forceinline__ void feq_axes(R (&e)[Q], R rho, const R (&u)[3]) {
+  const R InvCs2 = R(3);
+  const R InvCs2x = R(0.5) * InvCs2;
+  constexpr int D = Lat<Q>::D;
+  const R var0 = D == 3 ? R(1) - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * InvCs2x
+                        : R(1) - (u[0] * u[0] + u[1] * u[1]) * InvCs2x;
+  const R rhowk0 = Lat<Q>::template w<R>(0) * rho;
+  const R rhowk1 = Lat<Q>::template w<R>(1) * rho;
+  e[0] = rhowk0 * var0;
+#pragma unroll
+  for (int p = 0; p < D; ++p) {
+    const R a = InvCs2 * u[p];
+    const R v = var0 + a * a * R(0.5);
+    e[2 * p + 1] = rhowk1 * (v + a);
+    e[2 * p + 2] = rhowk1 * (v - a);
+  }
+}
+template <class R>
+__device__ __forceinline__ void feq(R (&e)[5], R rho, const R (&u)[3]) {
+  feq_axes<R, 5>(e, rho, u);
+}
+template <class R>
+__device__ __forceinline__ void feq(R (&e)[7], R rho, const R (&u)[3]) {
+  feq_axes<R, 7>(e, rho, u);
+}
 
 // ---------------------------------------------------------------------------
 // Moments: moment::rhoUImpl (moment.ur.h:461-472 D2Q9, 505-517 D3Q19).
@@ -146,6 +234,14 @@ __device__ __forceinline__ R cdotu(int i, const R (&u)[3]) {
 template <int Q, class R>
 __device__ __forceinline__ R movingwall_term(int k, R rho_f, const R (&u_f)[3]) {
   return R(6) * Lat<Q>::template w<R>(k) * rho_f * cdotu<Q>(k, u_f);
+}
+
+// Halfway anti-bounce-back of a scalar lattice, bounceback::anti_simple
+// (bounce_back_boundary.h:183-193): f_k = 2 RHO w_k - f*_{opp k}, RHO the
+// cell's scalar field (the wall value on Dirichlet walls).
+template <int Q, class R>
+__device__ __forceinline__ R anti_term(int k, R rho_f, R prev) {
+  return R(2) * rho_f * Lat<Q>::template w<R>(k) - prev;
 }
 
 // ---------------------------------------------------------------------------
@@ -240,6 +336,101 @@ __device__ __forceinline__ void bgk_force(R (&f)[Q], const R (&F)[3], R omega, R
   force_rhou(f, F, rho, u);
   R Fi[Q];
   force_pop(u, F, Fi);
+  R e[Q];
+  feq(e, rho, u);
+  const R fomega = R(1) - R(0.5) * omega;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = omega * e[i] + _omega * f[i] + fomega * Fi[i];
+}
+
+// ---------------------------------------------------------------------------
+// Scalar force along the last axis (y in 2-D, z in 3-D): force::ScalarForce
+// (force.h:109-133) as the buoyancy of a coupled temperature / concentration
+// lattice sets it.
+// ---------------------------------------------------------------------------
+
+// c_i . u of the first direction of opposite pair p (directions 2p+1, 2p+2),
+// summed the way the unrolled specialisations write it: u0, u1, u2, u0+u1,
+// u0+u2, u1+u2, u0-u1, u0-u2, u1-u2 (D3Q19), u0, u1, u0+u1, u0-u1 (D2Q9).
+template <int Q, class R>
+__device__ __forceinline__ R pair_cu(int p, const R (&u)[3]) {
+  if constexpr (Q == 19) {
+    switch (p) {
+      case 0: return u[0];
+      case 1: return u[1];
+      case 2: return u[2];
+      case 3: return u[0] + u[1];
+      case 4: return u[0] + u[2];
+      case 5: return u[1] + u[2];
+      case 6: return u[0] - u[1];
+      case 7: return u[0] - u[2];
+      default: return u[1] - u[2];
+    }
+  } else {
+    switch (p) {
+      case 0: return u[0];
+      case 1: return u[1];
+      case 2: return u[0] + u[1];
+      default: return u[0] - u[1];
+    }
+  }
+}
+
+// forcerhoUImpl's scalar overload (moment.h:423-435, unrolled at
+// moment.ur.h:564-574, 652-663): rho and the sums c f, then F/2 added to the
+// force axis, then each component divided by rho.
+template <int Q, class R>
+__device__ __forceinline__ void scalar_force_rhou(const R (&c)[Q], R F, R& rho, R (&u)[3]) {
+  constexpr int D = Lat<Q>::D;
+  if constexpr (Q == 19) {
+    rho = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] + c[11] + c[12] +
+          c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+    u[0] = c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16];
+    u[1] = c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18];
+    u[2] = c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18];
+  } else {
+    rho = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+    u[0] = c[1] - c[2] + c[5] - c[6] + c[7] - c[8];
+    u[1] = c[3] - c[4] + c[5] - c[6] - c[7] + c[8];
+    u[2] = R(0);
+  }
+  u[D - 1] += F * R(0.5);
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] /= rho;
+}
+
+// ScalarForcePopImpl<D2Q9, 1> / <D3Q19, 2> (force.ur.h:332-353, 584-616):
+// Fi = w_i InvCs2 F (c_id - u_d + InvCs2 (c_i.u) c_id), written per direction
+// as (+-1 - u_d) +- InvCs2 (c.u) of the pair, or -u_d where c_id = 0.
+template <int Q, class R>
+__device__ __forceinline__ void scalar_force_pop(const R (&u)[3], R F, R (&Fi)[Q]) {
+  constexpr int D = Lat<Q>::D;
+  const R I2 = R(3);
+  const R ud = u[D - 1];
+  const R T1_ud = R(1) - ud, _T1_ud = R(-1) - ud, _ud = -ud;
+  Fi[0] = Lat<Q>::template w<R>(0) * I2 * F * (_ud);
+#pragma unroll
+  for (int p = 0; p < (Q - 1) / 2; ++p) {
+    const int i = 2 * p + 1;
+    const int s = D == 3 ? Lat<Q>::cz(i) : Lat<Q>::cy(i);  // c_id of the pair's first direction
+    if (s == 0) {
+      Fi[i] = Lat<Q>::template w<R>(i) * I2 * F * (_ud);
+      Fi[i + 1] = Lat<Q>::template w<R>(i + 1) * I2 * F * (_ud);
+    } else {
+      const R cu = I2 * pair_cu<Q>(p, u);
+      Fi[i] = Lat<Q>::template w<R>(i) * I2 * F * (s > 0 ? T1_ud + cu : _T1_ud - cu);
+      Fi[i + 1] = Lat<Q>::template w<R>(i + 1) * I2 * F * (s > 0 ? _T1_ud + cu : T1_ud - cu);
+    }
+  }
+}
+
+// collision::BGKForce with forcerhoU<CELL, ScalarForce> and ScalarForce
+// (collision.h:68-87): f <- omega feq + (1 - omega) f + (1 - omega/2) Fi.
+template <int Q, class R>
+__device__ __forceinline__ void bgk_scalar_force(R (&f)[Q], R F, R omega, R _omega, R& rho, R (&u)[3]) {
+  scalar_force_rhou<Q>(f, F, rho, u);
+  R Fi[Q];
+  scalar_force_pop<Q>(u, F, Fi);
   R e[Q];
   feq(e, rho, u);
   const R fomega = R(1) - R(0.5) * omega;

@@ -141,6 +141,8 @@ int flb_halo_create(const flb_block* blocks, int n, const int* origins, const in
   if (!blocks || !origins || !out || n < 1) return fail(FLB_ERR_ARG, "halo needs blocks, origins and an output");
   *out = nullptr;
   const flb_block_s* b0 = blocks[0];
+  if (b0 && b0->desc.q != FLB_D2Q9 && b0->desc.q != FLB_D3Q19)
+    return fail(FLB_ERR_UNSUPPORTED, "block halos take D2Q9 / D3Q19 blocks");
   for (int k = 0; k < n; ++k) {
     const flb_block_s* b = blocks[k];
     if (!b) return fail(FLB_ERR_ARG, "null block %d", k);

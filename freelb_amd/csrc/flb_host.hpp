@@ -54,6 +54,18 @@ struct flb_block_s {
   void* d_rho_old = nullptr;    // RhoOld / UOld of the convergence residual (field slot layout)
   void* d_u_old = nullptr;
   unsigned long long* d_res = nullptr;  // residual as the bits of a non-negative double
+  // Coupled lattices (flb_share_velocity, flb_add_buoyancy). A scalar block
+  // advects with vel_src's VELOCITY field; a flow block's DYN_BGK_SCALAR_FORCE
+  // cells take the buoyancy of each entry of buoy. users lists the blocks that
+  // hold a reference to this one, so destroying either side detaches it.
+  struct Buoyancy {
+    flb_block src;
+    uint8_t mask;
+    double gbeta, rho0;
+  };
+  flb_block vel_src = nullptr;
+  std::vector<Buoyancy> buoy;
+  std::vector<flb_block> users;
 };
 
 struct flb_halo_s {

@@ -30,7 +30,12 @@ constexpr int kFrontPad = 256;
 
 // Dynamics families: the kernel for a family compiles only the collisions in
 // it, so the common BGK / bounce-back step does not pay the registers of MRT.
-enum Family { FAM_BGK = 0, FAM_FULL = 1 };
+// FAM_SCALAR is the advection-diffusion family of the D2Q5 / D3Q7 lattices.
+enum Family { FAM_BGK = 0, FAM_FULL = 1, FAM_SCALAR = 2 };
+
+// Buoyancy couplings one flow block can carry (FreeLB's examples couple a
+// temperature and a concentration lattice).
+constexpr int kMaxCouplings = 2;
 
 // Everything a lattice kernel needs, passed by value (kernel argument space).
 struct KArgs {
@@ -66,6 +71,17 @@ struct KArgs {
   double omega;
   double rates[19];      // MRT rates; < 0 or NaN marks the shear rates
   double force[3];       // constant body-force density of DYN_BGK_FORCE
+  // advection-diffusion dynamics: slot 0 of the VELOCITY field that advects
+  // the scalar (the coupled flow block's), or null for the block's own u_f
+  const void* u_ext;
+  // DYN_BGK_SCALAR_FORCE: force on the last axis = sum over the couplings k
+  // whose mask matches the flag of gbeta_k (C_k - C0_k), C_k the coupled
+  // block's RHO (scalar) field in slot layout
+  int ncpl;
+  uint8_t cpl_mask[kMaxCouplings];
+  const void* cpl_rho[kMaxCouplings];
+  double cpl_gbeta[kMaxCouplings];
+  double cpl_rho0[kMaxCouplings];
 };
 
 // Launchers, one set per storage kind (defined in flb_inst_*.hip).

@@ -38,6 +38,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "flb_cell.hpp"
 #include "flb_internal.hpp"
 #include "flb_storage.hpp"
@@ -55,9 +57,14 @@ enum : int {
   DYN_SMAG_BGK = 6,
   DYN_SMAG_MRT = 7,
   DYN_BGK_FORCE = 8,
-  DYN_BGK_WRITE = 9
+  DYN_BGK_WRITE = 9,
+  DYN_BGK_SCALAR_FORCE = 10,
+  DYN_SCALAR_BGK = 11,
+  DYN_SCALAR_BGK_FIELD = 12
 };
-enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1 };
+// HWBC_ANTI (scalar lattices) shares the second link word with HWBC_MOVING
+// (flow lattices): a block has one or the other.
+enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1, HWBC_ANTI = 2 };
 
 // Logical workgroup index: the hardware hands consecutive workgroups to the
 // 8 XCDs in turn, so workgroup b runs on XCD b % 8 as its (b / 8)-th one.
@@ -143,7 +150,7 @@ __device__ __forceinline__ void wall_links(const KArgs& a, int x, int y, int z, 
     for (int i = 1; i < Q; ++i) {
       if (a.flags[slot + nb_off<Q, PER, false>(a, i, x, y, z)] & vm) v |= 1u << i;
     }
-    if (a.bckind[k] == HWBC_MOVING) {
+    if (a.bckind[k] != HWBC_NORMAL) {
       lm |= v;
       ln &= ~v;
     } else {
@@ -178,6 +185,29 @@ __device__ __forceinline__ void store_fields(const KArgs& a, int slot, R rho, co
 template <int Q, class R, int FAM>
 __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
                                         R om, R _om) {
+  if constexpr (FAM == FAM_SCALAR) {
+    // Advection-diffusion (collision::BGK, collision.h:39-56): the scalar is
+    // advected by the VELOCITY field (moment::useFieldU, moment.h:56-69);
+    // DYN_SCALAR_BGK sums it and writes it to the field
+    // (MomentaTuple<rho<CELL, true>, useFieldU>, moment.h:159-182, 667-679),
+    // DYN_SCALAR_BGK_FIELD takes it from the field (useFieldrhoU, moment.h:71-90).
+    R rho, u[3] = {R(0), R(0), R(0)};
+    const R* ue = static_cast<const R*>(a.u_ext ? a.u_ext : a.u_f);
+    if (ue) {
+#pragma unroll
+      for (int d = 0; d < Lat<Q>::D; ++d) u[d] = ue[d * a.fstride + slot];
+    }
+    if (dyn == DYN_SCALAR_BGK) {
+      rho = rho_sum(f);
+      if (a.rho_f) static_cast<R*>(const_cast<void*>(a.rho_f))[slot] = rho;
+    } else if (dyn == DYN_SCALAR_BGK_FIELD) {
+      rho = a.rho_f ? static_cast<const R*>(a.rho_f)[slot] : R(1);
+    } else {
+      return;
+    }
+    bgk_with<Q>(f, rho, u, om, _om);
+    return;
+  } else {
   R rho, u[3], rho_f, u_f[3];
   switch (dyn) {
     case DYN_BGK:
@@ -202,6 +232,21 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
           bgk_force<Q>(f, F, om, _om);
           break;
         }
+        if (dyn == DYN_BGK_SCALAR_FORCE) {
+          // BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
+          // with SCALARFORCE = 0 + the force::Buoyancy terms (force.h:160-181)
+          // of the coupled scalar blocks, in coupling order
+          R F = R(0);
+          const uint8_t fl = a.flags[slot];
+#pragma unroll
+          for (int k = 0; k < kMaxCouplings; ++k) {
+            if (k < a.ncpl && (fl & a.cpl_mask[k]))
+              F += R(a.cpl_gbeta[k]) * (static_cast<const R*>(a.cpl_rho[k])[slot] - R(a.cpl_rho0[k]));
+          }
+          bgk_scalar_force<Q>(f, F, om, _om, rho, u);
+          store_fields<Q>(a, slot, rho, u);
+          break;
+        }
         rhou(f, rho, u);
         if (dyn == DYN_BGK_WRITE) {
           store_fields<Q>(a, slot, rho, u);
@@ -220,6 +265,7 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
         }
       }
       break;
+  }
   }
 }
 
@@ -412,8 +458,12 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     R rho_f, u_f[3];
     load_fields<Q>(a, s0, rho_f, u_f);
 #pragma unroll
-    for (int i = 1; i < Q; ++i)
-      if ((lm >> opp(i)) & 1u) f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
+    for (int i = 1; i < Q; ++i) {
+      if ((lm >> opp(i)) & 1u) {
+        if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(i, rho_f, f[i]);
+        else f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
+      }
+    }
   }
   const R om = R(a.omega);
   collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
@@ -427,7 +477,10 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
       R v = f[i];
-      if (i != 0 && ((lm >> i) & 1u)) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
+      if (i != 0 && ((lm >> i) & 1u)) {
+        if constexpr (kScalarLat<Q>) v = anti_term<Q>(opp(i), rho_f, v);
+        else v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
+      }
       E* p = wall ? (P + int64_t(opp(i)) * a.dpop) + c0
                   : (P + int64_t(i) * a.dpop) + c0 + nb_off<Q, PER, true>(a, i, x, y, z);
       st_pop<PAR>(p, S::template to<Q>(v, i));
@@ -748,7 +801,10 @@ __device__ __forceinline__ void read_logical(const KArgs& a, int x, int y, int z
     const int o = opp(i);
     if (i != 0 && ((lk >> o) & 1u)) {
       R v = S::template from<Q>(arr<S>(a, i)[pslot], i);
-      if ((lm >> o) & 1u) v = v + movingwall_term<Q>(i, rho_f, u_f);
+      if ((lm >> o) & 1u) {
+        if constexpr (kScalarLat<Q>) v = anti_term<Q>(i, rho_f, v);
+        else v = v + movingwall_term<Q>(i, rho_f, u_f);
+      }
       f[i] = v;
     } else {
       f[i] = S::template from<Q>(arr<S>(a, o)[pslot + nb_off<Q, true, true>(a, o, x, y, z)], i);
@@ -988,6 +1044,43 @@ hipError_t launch_batch(int q, int parity, int family, bool per, const KArgs* ta
   return hipErrorInvalidValue;
 }
 
+// Storage kinds of the advection-diffusion (D2Q5 / D3Q7) blocks: the 16-bit
+// kinds store f - w_i for populations near the rest state of unit density,
+// which a scalar field of arbitrary magnitude is not.
+template <class S>
+constexpr bool kPlainStore = std::is_same<S, StoreF32>::value || std::is_same<S, StoreF64>::value;
+
+// Calls f(std::integral_constant<int, Q>) for the block's lattice; false for a
+// lattice this storage kind does not support.
+template <class S, class F>
+bool with_q(int q, F&& f) {
+  switch (q) {
+    case 19: f(std::integral_constant<int, 19>{}); return true;
+    case 9: f(std::integral_constant<int, 9>{}); return true;
+    default: break;
+  }
+  if constexpr (kPlainStore<S>) {
+    if (q == 7) {
+      f(std::integral_constant<int, 7>{});
+      return true;
+    }
+    if (q == 5) {
+      f(std::integral_constant<int, 5>{});
+      return true;
+    }
+  }
+  return false;
+}
+
+// Scalar lattices step in the cell form only (no row form, no batches).
+template <int Q, class S, int PAR>
+hipError_t step_scalar_q(const KArgs& a, hipStream_t s) {
+  const dim3 grid(a.nblocks), block(kWave * a.wpb);
+  if (a.px || a.py || a.pz) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 // Non-temporal even steps for blocks with wide rows (pitch of whole
 // 256-slot groups) and for the row form; compact rows take PAR 2.
 template <class S>
@@ -1000,6 +1093,15 @@ hipError_t launch_step(int q, int parity, int family, const KArgs& a, hipStream_
   if (q == 9)
     return parity ? step_q<9, S, 1>(family, a, s)
                   : (nt ? step_q<9, S, 0>(family, a, s) : step_q<9, S, 2>(family, a, s));
+  if constexpr (kPlainStore<S>) {
+    const bool ntc = a.pitch >= 256;
+    if (q == 7)
+      return parity ? step_scalar_q<7, S, 1>(a, s)
+                    : (ntc ? step_scalar_q<7, S, 0>(a, s) : step_scalar_q<7, S, 2>(a, s));
+    if (q == 5)
+      return parity ? step_scalar_q<5, S, 1>(a, s)
+                    : (ntc ? step_scalar_q<5, S, 0>(a, s) : step_scalar_q<5, S, 2>(a, s));
+  }
   return hipErrorInvalidValue;
 }
 
@@ -1008,14 +1110,12 @@ hipError_t launch_gather(int q, int parity, const KArgs& a, void* out, hipStream
   const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
   using R = typename S::R;
   R* o = static_cast<R*>(out);
-  if (q == 19) {
-    if (parity) hipLaunchKernelGGL((k_gather<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
-    else hipLaunchKernelGGL((k_gather<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
-  } else {
-    if (parity) hipLaunchKernelGGL((k_gather<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
-    else hipLaunchKernelGGL((k_gather<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
-  }
-  return hipGetLastError();
+  const bool ok = with_q<S>(q, [&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    if (parity) hipLaunchKernelGGL((k_gather<Q, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+    else hipLaunchKernelGGL((k_gather<Q, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, o);
+  });
+  return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 template <class S>
@@ -1023,9 +1123,11 @@ hipError_t launch_scatter(int q, const KArgs& a, const void* in, hipStream_t s) 
   const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
   using R = typename S::R;
   const R* p = static_cast<const R*>(in);
-  if (q == 19) hipLaunchKernelGGL((k_scatter<19, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
-  else hipLaunchKernelGGL((k_scatter<9, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
-  return hipGetLastError();
+  const bool ok = with_q<S>(q, [&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    hipLaunchKernelGGL((k_scatter<Q, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  });
+  return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 template <class S>
@@ -1033,9 +1135,11 @@ hipError_t launch_init_feq(int q, const KArgs& a, const void* rhou, hipStream_t 
   const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
   using R = typename S::R;
   const R* p = static_cast<const R*>(rhou);
-  if (q == 19) hipLaunchKernelGGL((k_init_feq<19, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
-  else hipLaunchKernelGGL((k_init_feq<9, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
-  return hipGetLastError();
+  const bool ok = with_q<S>(q, [&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    hipLaunchKernelGGL((k_init_feq<Q, S>), dim3(nb), dim3(kMaxThreads), 0, s, a, p);
+  });
+  return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 template <class S>
@@ -1044,27 +1148,24 @@ hipError_t launch_moments(int q, int parity, const KArgs& a, uint8_t mask, void*
   const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
   using R = typename S::R;
   R* o = static_cast<R*>(out);
-  if (q == 19) {
-    if (parity) hipLaunchKernelGGL((k_moments<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
-    else hipLaunchKernelGGL((k_moments<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
-  } else {
-    if (parity) hipLaunchKernelGGL((k_moments<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
-    else hipLaunchKernelGGL((k_moments<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
-  }
-  return hipGetLastError();
+  const bool ok = with_q<S>(q, [&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    if (parity) hipLaunchKernelGGL((k_moments<Q, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+    else hipLaunchKernelGGL((k_moments<Q, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, o);
+  });
+  return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 template <class S>
 hipError_t launch_reduce(int q, int parity, const KArgs& a, uint8_t mask, double* partials,
                          double* out, hipStream_t s) {
   const unsigned nb = cell_blocks(int64_t(a.nx) * a.ny * a.nz);
-  if (q == 19) {
-    if (parity) hipLaunchKernelGGL((k_reduce<19, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
-    else hipLaunchKernelGGL((k_reduce<19, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
-  } else {
-    if (parity) hipLaunchKernelGGL((k_reduce<9, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
-    else hipLaunchKernelGGL((k_reduce<9, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
-  }
+  const bool ok = with_q<S>(q, [&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    if (parity) hipLaunchKernelGGL((k_reduce<Q, S, 1>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+    else hipLaunchKernelGGL((k_reduce<Q, S, 0>), dim3(nb), dim3(kMaxThreads), 0, s, a, mask, partials);
+  });
+  if (!ok) return hipErrorInvalidValue;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_reduce_final<S>, dim3(1), dim3(kMaxThreads), 0, s, partials, int(nb), out);

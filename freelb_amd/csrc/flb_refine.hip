@@ -389,6 +389,8 @@ int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void*
   *out = nullptr;
   if (coarse == fine) return fail(FLB_ERR_ARG, "coarse and fine block are the same block");
   if (coarse->desc.q != fine->desc.q) return fail(FLB_ERR_ARG, "blocks of different lattices");
+  if (coarse->desc.q != FLB_D2Q9 && coarse->desc.q != FLB_D3Q19)
+    return fail(FLB_ERR_UNSUPPORTED, "refinement takes D2Q9 / D3Q19 blocks");
   if (coarse->desc.storage != fine->desc.storage) return fail(FLB_ERR_ARG, "blocks of different storage kinds");
   if (coarse->device != fine->device) return fail(FLB_ERR_ARG, "blocks on different devices");
   const hipStream_t s = static_cast<hipStream_t>(stream);

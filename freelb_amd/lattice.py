@@ -34,10 +34,14 @@ class Key_TypePair:
 
 @dataclass(frozen=True)
 class HalfwayBB:
-    """One BBLikeFixedBlockBdManager<bounceback::normal|movingwall> (boundary/boundary.h:75-129)."""
+    """One BBLikeFixedBlockBdManager<bounceback::normal|movingwall|anti_simple>
+    (boundary/boundary.h:75-129, boundary/bounce_back_boundary.h:169-256).
+    `anti` (D2Q5 / D3Q7 blocks) is the fixed-value wall anti_simple, whose
+    value is the cell's RHO (scalar) field."""
     cell_mask: int
     void_mask: int = 1
     moving: bool = False
+    anti: bool = False
 
 
 def weights(q: int, dtype=np.float64) -> np.ndarray:
@@ -45,7 +49,17 @@ def weights(q: int, dtype=np.float64) -> np.ndarray:
     one = dtype(1)
     if q == 19:
         return np.array([one / dtype(3)] + [one / dtype(18)] * 6 + [one / dtype(36)] * 12, dtype=dtype)
-    return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
+    if q == 9:
+        return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
+    if q == 5:
+        return np.array([one / dtype(3)] + [one / dtype(6)] * 4, dtype=dtype)
+    if q == 7:
+        return np.array([one / dtype(4)] + [one / dtype(8)] * 6, dtype=dtype)
+    raise ValueError(f"unknown lattice q={q}")
+
+
+def dimension(q: int) -> int:
+    return 3 if q in (19, 7) else 2
 
 
 class BlockLattice:
@@ -53,7 +67,7 @@ class BlockLattice:
                  overlap: int = 1, periodic: Sequence[bool] = (False, False, False), stream=None):
         L = abi.lib()
         self.q = int(q)
-        self.d = 3 if self.q == 19 else 2
+        self.d = dimension(self.q)
         nx, ny = int(dims[0]), int(dims[1])
         nz = int(dims[2]) if len(dims) > 2 else 1
         self.dims = (nx, ny, nz)
@@ -116,7 +130,7 @@ class BlockLattice:
         arr = (abi.HWBC * max(1, len(bcs)))()
         for k, b in enumerate(bcs):
             arr[k].cell_mask, arr[k].void_mask = b.cell_mask, b.void_mask
-            arr[k].kind = abi.HWBC_MOVING if b.moving else abi.HWBC_NORMAL
+            arr[k].kind = abi.HWBC_ANTI if b.anti else (abi.HWBC_MOVING if b.moving else abi.HWBC_NORMAL)
         abi.check(abi.lib().flb_set_halfway_bc(self._h, arr, len(bcs)))
 
     def set_mrt_rates(self, rates=None) -> None:
@@ -134,6 +148,14 @@ class BlockLattice:
         """CONSTFORCE of the BGKForce cells (force::ConstForce, lbm/force.h:86-107)."""
         F = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1))
         abi.check(abi.lib().flb_set_force(self._h, abi.ptr(F, C.c_double), int(F.size)))
+
+    def share_velocity(self, flow: "BlockLattice | None") -> None:
+        """This (D2Q5 / D3Q7) block is advected by flow's VELOCITY field: the
+        field reference FreeLB passes to a temperature / concentration
+        lattice's constructor (examples/cazsblock2d/cazsblock2d.cpp:240-251).
+        None detaches."""
+        abi.check(abi.lib().flb_share_velocity(self._h, flow.handle if flow is not None else None))
+        self._vel_src = flow  # keep the flow block alive while it is referenced
 
     def set_omega(self, omega: float) -> None:
         abi.check(abi.lib().flb_set_omega(self._h, float(omega)))
@@ -231,6 +253,47 @@ class BlockLattice:
         out = np.zeros(4, dtype=np.float64)
         abi.check(abi.lib().flb_reduce_mass_momentum(self._h, mask & 0xFF, abi.ptr(out, C.c_double)))
         return out
+
+
+class BlockLatManagerCoupling:
+    """FreeLB's BlockLatManagerCoupling(flow, scalar) (data_struct/block_lattice.h:436-649)
+    running force::Buoyancy<NSCELL, CELL1> (lbm/force.h:160-181) through a
+    CoupledTaskSelector: each matching DYN_BGK_SCALAR_FORCE cell of `flow`
+    takes SCALARFORCE += gbeta (scalar - rho_init) on the last axis.
+
+    The buoyancy is formed inside the flow block's collision from the scalar
+    field the scalar block's last step wrote, which is what FreeLB's loop
+    (clear SCALARFORCE, coupling pass, flow step, scalar step;
+    examples/cazsblock2d/cazsblock2d.cpp:385-409) computes, without the two
+    extra passes over the domain.
+    """
+
+    def __init__(self, flow: BlockLattice, scalar: BlockLattice):
+        self.flow, self.scalar = flow, scalar
+
+    def getLat0(self) -> BlockLattice:
+        return self.flow
+
+    def getLat1(self) -> BlockLattice:
+        return self.scalar
+
+    def add_buoyancy(self, mask: int, gbeta: float, rho_init: float) -> None:
+        """Key_TypePair<mask, force::Buoyancy<NSCELL, CELL1>>; GBETA / RHOINIT values."""
+        abi.check(abi.lib().flb_add_buoyancy(self.flow.handle, self.scalar.handle, int(mask) & 0xFF,
+                                             float(gbeta), float(rho_init)))
+        refs = getattr(self.flow, "_buoy_src", [])
+        refs.append(self.scalar)
+        self.flow._buoy_src = refs
+
+    def clear(self) -> None:
+        abi.check(abi.lib().flb_clear_buoyancy(self.flow.handle))
+        self.flow._buoy_src = []
+
+    def step(self, n: int = 1) -> None:
+        """n iterations of the coupled loop: the flow step, then the scalar step."""
+        for _ in range(int(n)):
+            self.flow.step(1)
+            self.scalar.step(1)
 
 
 class BlockHalo:

@@ -57,6 +57,22 @@ struct D2Q9 {
   static constexpr unsigned int q = 9;
 };
 
+// advection-diffusion sets (lbm/lattice_set.h:144-155, 196-207)
+template <typename T>
+struct D2Q5 {
+  using FloatType = T;
+  static constexpr unsigned int d = 2;
+  static constexpr unsigned int q = 5;
+};
+template <typename T>
+struct D3Q7 {
+  using FloatType = T;
+  static constexpr unsigned int d = 3;
+  static constexpr unsigned int q = 7;
+};
+template <typename LatSet>
+constexpr bool is_scalar_lattice = LatSet::q == 5 || LatSet::q == 7;
+
 // The CELL of FreeLB's dynamics templates; only its lattice set matters here.
 template <typename T, typename LatSet>
 struct Cell {
@@ -72,11 +88,24 @@ template <typename CELL>
 struct useFieldrhoU {};  // lbm/moment.h:71-90
 template <typename CELL, typename ForceScheme, bool WriteToField = false>
 struct forcerhoU {};  // lbm/moment.h:441-480
+template <typename CELL, bool WriteToField = false>
+struct rho {};  // lbm/moment.h:159-182
+template <typename CELL>
+struct useFieldU {};  // lbm/moment.h:56-69
+template <typename... Momenta>
+struct MomentaTuple {};  // lbm/moment.h:666-679
 }  // namespace moment
 
 namespace force {
 template <typename CELL>
 struct ConstForce {};  // lbm/force.h:86-107 (the CONSTFORCE data field)
+template <typename CELL, unsigned int dir = 2>
+struct ScalarForce {};  // lbm/force.h:109-133 (the SCALARFORCE field, on the last axis)
+// lbm/force.h:160-181: SCALARFORCE of the flow cell += GBETA (rho1 - RHOINIT)
+template <typename CELL0, typename CELL1, bool Scalar = true>
+struct Buoyancy {
+  static_assert(Scalar, "the vector-force form of Buoyancy is not provided");
+};
 }  // namespace force
 
 namespace equilibrium {
@@ -97,7 +126,15 @@ struct BGK<moment::rhoU<CELL, true>, EQ> {  // the moments also go to the RHO / 
 };
 template <typename CELL, typename EQ>
 struct BGK<moment::useFieldrhoU<CELL>, EQ> {
-  static constexpr uint8_t kind = FLB_DYN_BGK_FIELD;
+  // on a scalar lattice: relax toward the field's scalar (a fixed-value wall)
+  static constexpr uint8_t kind =
+      is_scalar_lattice<typename CELL::LatticeSet> ? FLB_DYN_SCALAR_BGK_FIELD : FLB_DYN_BGK_FIELD;
+};
+// advection-diffusion: the scalar to the field, u from the (shared) VELOCITY field
+template <typename CELL, typename EQ>
+struct BGK<moment::MomentaTuple<moment::rho<CELL, true>, moment::useFieldU<CELL>>, EQ> {
+  static_assert(is_scalar_lattice<typename CELL::LatticeSet>, "a D2Q5 / D3Q7 dynamics");
+  static constexpr uint8_t kind = FLB_DYN_SCALAR_BGK;
 };
 template <typename CELL>
 struct BounceBack {  // lbm/collision.ur.h:62-93
@@ -117,6 +154,10 @@ struct SmagorinskyBGK {  // lbm/collisionLES.h:29-67
 };
 template <typename MomentaScheme, typename EquilibriumScheme, typename ForceScheme>
 struct BGKForce;  // lbm/collision.h:61-90
+template <typename CELL, typename EQ>
+struct BGKForce<moment::forcerhoU<CELL, force::ScalarForce<CELL>, true>, EQ, force::ScalarForce<CELL>> {
+  static constexpr uint8_t kind = FLB_DYN_BGK_SCALAR_FORCE;
+};
 template <typename CELL, typename EQ>
 struct BGKForce<moment::forcerhoU<CELL, force::ConstForce<CELL>>, EQ, force::ConstForce<CELL>> {
   static constexpr uint8_t kind = FLB_DYN_BGK_FORCE;
@@ -157,7 +198,18 @@ template <typename CELL>
 struct movingwall {
   static constexpr uint8_t kind = FLB_HWBC_MOVING;
 };
+template <typename CELL>
+struct anti_simple {  // bounce_back_boundary.h:183-193 (scalar lattices)
+  static constexpr uint8_t kind = FLB_HWBC_ANTI;
+};
 }  // namespace bounceback
+
+// CoupledTaskSelector<FlagType, CELL0, CELL1, Key_TypePair<mask, force::Buoyancy<...>>...>
+// (utils/tmp.h:333-337): the masks of the buoyancy tasks.
+template <typename FlagType, typename CELL0, typename CELL1, typename... Tasks>
+struct CoupledTaskSelector {
+  static std::vector<uint8_t> masks() { return {static_cast<uint8_t>(Tasks::key)...}; }
+};
 
 enum class Storage {
   F32 = FLB_STORE_F32,
@@ -276,6 +328,19 @@ class BlockLattice {
     return r;
   }
 
+  // A scalar lattice's VELOCITY field reference: FreeLB passes
+  // &NSLattice.getField<VELOCITY>() to the scalar lattice's constructor
+  // (examples/cazsblock2d/cazsblock2d.cpp:240-251).
+  template <typename FlowLat>
+  void shareVelocity(BlockLattice<T, FlowLat>& flow) {
+    check(flb_share_velocity(h_, flow.handle()));
+  }
+  // The GBETA / RHOINIT data of a scalar lattice (constant fields in FreeLB).
+  void setGBeta(double g) { gbeta_ = g; }
+  void setRhoInit(double r) { rho_init_ = r; }
+  double gbeta() const { return gbeta_; }
+  double rhoInit() const { return rho_init_; }
+
   // called by the halfway managers
   void addHalfway(const flb_hwbc& bc) {
     bcs_.push_back(bc);
@@ -297,6 +362,37 @@ class BlockLattice {
   std::vector<flb_task> tasks_;
   std::vector<flb_hwbc> bcs_;
   float smagorinsky_ = 0.0f;
+  double gbeta_ = 0.0, rho_init_ = 1.0;
+};
+
+// ---- coupled lattices (data_struct/block_lattice.h:436-649) --------------------------
+// BlockLatManagerCoupling(flow, scalar).ApplyCellDynamics<CoupledTaskSelector<...
+// force::Buoyancy ...>>(flags) registers the buoyancy of each task's mask
+// once; the force itself is formed inside the flow lattice's next
+// collision from the scalar field the scalar lattice's last step wrote, which
+// is the value FreeLB's coupling pass computes in its loop
+// (SCALARFORCE.InitValue(0); coupling pass; flow step; scalar step,
+// examples/cazsblock2d/cazsblock2d.cpp:385-409).
+template <typename T, typename FlowLat, typename ScalarLat>
+class BlockLatManagerCoupling {
+ public:
+  BlockLatManagerCoupling(BlockLattice<T, FlowLat>& flow, BlockLattice<T, ScalarLat>& scalar)
+      : flow_(flow), scalar_(scalar) {}
+  BlockLattice<T, FlowLat>& getLat0() { return flow_; }
+  BlockLattice<T, ScalarLat>& getLat1() { return scalar_; }
+
+  template <typename Selector>
+  void ApplyCellDynamics(const std::vector<uint8_t>&) {
+    if (registered_) return;
+    for (uint8_t m : Selector::masks())
+      check(flb_add_buoyancy(flow_.handle(), scalar_.handle(), m, scalar_.gbeta(), scalar_.rhoInit()));
+    registered_ = true;
+  }
+
+ private:
+  BlockLattice<T, FlowLat>& flow_;
+  BlockLattice<T, ScalarLat>& scalar_;
+  bool registered_ = false;
 };
 
 // ---- two refinement levels (flb_refine_*) ---------------------------------------------

@@ -42,6 +42,10 @@
  *                              lbm/moment.h:381-398, examples/cavity3d/cavity3d.cpp:192-194
  *   flb_init_equilibrium     BlockLatticeManager ctor pop init / equilibrium::SecondOrder
  *                              data_struct/block_lattice.hh:676-681, lbm/equilibrium.h:156-178
+ *   flb_share_velocity,      BlockLatManagerCoupling with force::Buoyancy and the VELOCITY
+ *   flb_add_buoyancy           field reference of a temperature / concentration lattice
+ *                              data_struct/block_lattice.h:436-649, lbm/force.h:160-181,
+ *                              examples/cazsblock2d/cazsblock2d.cpp:225-409
  */
 #ifndef FLB_ABI_H
 #define FLB_ABI_H
@@ -67,7 +71,9 @@ enum {
 };
 
 /* ---- enums ------------------------------------------------------------- */
-enum { FLB_D2Q9 = 9, FLB_D3Q19 = 19 };
+/* Flow lattices D2Q9 / D3Q19; advection-diffusion (scalar) lattices D2Q5 /
+ * D3Q7 (lbm/lattice_set.h:144-207), FP32 or FP64 storage only. */
+enum { FLB_D2Q9 = 9, FLB_D3Q19 = 19, FLB_D2Q5 = 5, FLB_D3Q7 = 7 };
 
 /* Population storage. The shifted forms store f_i - w_i (rho0 = 1) and do
  * all arithmetic in FP32: F16S as IEEE binary16, BF16S as bfloat16, F16C as
@@ -90,22 +96,36 @@ enum {
   FLB_DYN_BGK_FORCE = 8,     /* collision::BGKForce<moment::forcerhoU, SecondOrder, force::ConstForce>
                                 lbm/collision.h:61-90, lbm/force.h:29-36, lbm/moment.h:401-438;
                                 the force is the block's constant (flb_set_force) */
-  FLB_DYN_BGK_WRITE = 9      /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: BGK that also
+  FLB_DYN_BGK_WRITE = 9,     /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: BGK that also
                                 stores rho and u in the RHO / VELOCITY fields (lbm/moment.h:359-377),
                                 as the refined drivers do (examples/cavref2d/cavref2d.cpp:178-179) */
+  FLB_DYN_BGK_SCALAR_FORCE = 10, /* collision::BGKForce<moment::forcerhoU<CELL, force::ScalarForce, true>,
+                                SecondOrder, force::ScalarForce> (lbm/collision.h:61-90, lbm/force.h:109-133,
+                                lbm/moment.h:401-435): the force on the last axis is the SCALARFORCE of
+                                the block's buoyancy couplings (flb_add_buoyancy), and rho / u go to the
+                                RHO / VELOCITY fields (examples/cazsblock2d/cazsblock2d.cpp:306-309) */
+  FLB_DYN_SCALAR_BGK = 11,   /* D2Q5 / D3Q7: collision::BGK<moment::MomentaTuple<moment::rho<CELL, true>,
+                                moment::useFieldU<CELL>>, SecondOrder>: the scalar (sum of the populations)
+                                goes to the RHO field, advected by the VELOCITY field (lbm/moment.h:56-69,
+                                159-182, 667-679) */
+  FLB_DYN_SCALAR_BGK_FIELD = 12 /* D2Q5 / D3Q7: collision::BGK<moment::useFieldrhoU<CELL>, SecondOrder>:
+                                relaxes toward the equilibrium of the RHO field's scalar (a Dirichlet wall
+                                value) and the VELOCITY field (lbm/moment.h:71-90) */
 };
 
 /* Halfway bounce-back kinds: bounceback::normal / bounceback::movingwall
- * (boundary/bounce_back_boundary.h:169-181, 240-256). */
-enum { FLB_HWBC_NORMAL = 0, FLB_HWBC_MOVING = 1 };
+ * (boundary/bounce_back_boundary.h:169-181, 240-256), and for D2Q5 / D3Q7
+ * blocks bounceback::anti_simple (183-193), f_k = 2 RHO w_k - f*_{opp k}
+ * with RHO the cell's scalar field (a fixed-value wall). */
+enum { FLB_HWBC_NORMAL = 0, FLB_HWBC_MOVING = 1, FLB_HWBC_ANTI = 2 };
 
 /* ---- descriptors ------------------------------------------------------- */
 typedef struct flb_block_s* flb_block;
 
 typedef struct {
-  int q;            /* FLB_D2Q9 or FLB_D3Q19 */
+  int q;            /* FLB_D2Q9, FLB_D3Q19, FLB_D2Q5 or FLB_D3Q7 */
   int storage;      /* FLB_STORE_* */
-  int nx, ny, nz;   /* extents including the overlap layer; nz = 1 for D2Q9 */
+  int nx, ny, nz;   /* extents including the overlap layer; nz = 1 in 2-D */
   int overlap;      /* overlap (ghost) width on non-periodic axes, >= 1 */
   int periodic[3];  /* non-zero: the axis wraps (no overlap layer on it) */
   double omega;     /* 1/tau of the block (BlockLattice::Omega) */
@@ -153,6 +173,27 @@ int flb_synchronize(flb_block b);
 /* ---- set-up ------------------------------------------------------------- */
 int flb_set_tasks(flb_block b, const flb_task* tasks, int n);      /* n <= 8 */
 int flb_set_halfway_bc(flb_block b, const flb_hwbc* bcs, int n);   /* n <= 4 */
+
+/* ---- coupled lattices (a flow block and its temperature / concentration
+ * blocks, FreeLB's BlockLatManagerCoupling, data_struct/block_lattice.h:436-649) */
+
+/* The scalar block is advected by the flow block's VELOCITY field: FreeLB's
+ * &NSLattice.getField<VELOCITY>() passed to the scalar lattice's constructor
+ * (examples/cazsblock2d/cazsblock2d.cpp:240-251). The blocks must cover the
+ * same cells. flow = NULL detaches (the scalar block then uses its own
+ * VELOCITY field). Both blocks get RHO / VELOCITY fields if they have none. */
+int flb_share_velocity(flb_block scalar, flb_block flow);
+/* Adds force::Buoyancy<NSCELL, CELL1> (lbm/force.h:160-181) run by
+ * BlockLatManagerCoupling::ApplyCellDynamics with a CoupledTaskSelector on
+ * `mask` (cazsblock2d.cpp:326-333, 385-392): every FLB_DYN_BGK_SCALAR_FORCE
+ * cell of `flow` whose flag matches takes SCALARFORCE += gbeta (RHO_scalar -
+ * rho_init) (GBETA and RHOINIT are the scalar lattice's constant fields),
+ * starting from 0 each step (SCALARFORCE.InitValue(0)). The force is formed
+ * inside the flow block's collision, from the scalar field as the scalar
+ * block's last step left it. At most 2 couplings per flow block, applied in
+ * the order added. */
+int flb_add_buoyancy(flb_block flow, flb_block scalar, uint8_t mask, double gbeta, double rho_init);
+int flb_clear_buoyancy(flb_block flow);
 /* MRT relaxation rates, q values; NaN marks the shear-viscosity rates that
  * take omega. NULL restores the defaults. */
 int flb_set_mrt_rates(flb_block b, const double* rates, int q);

@@ -20,6 +20,17 @@ static const int C9[9][3] = {{0, 0, 0}, {1, 0, 0},   {-1, 0, 0}, {0, 1, 0}, {0, 
                              {1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}};
 static const int OPP9[9] = {0, 2, 1, 4, 3, 6, 5, 8, 7};
 
+/* Advection-diffusion sets D2Q5 (lattice_set.h:144-155) and D3Q7 (196-207). */
+static const int C5[5][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+static const int OPP5[5] = {0, 2, 1, 4, 3};
+static const int C7[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const int OPP7[7] = {0, 2, 1, 4, 3, 6, 5};
+
+static const int (*lat_c(int q))[3] { return q == 19 ? C19 : (q == 9 ? C9 : (q == 7 ? C7 : C5)); }
+static const int* lat_opp(int q) { return q == 19 ? OPP19 : (q == 9 ? OPP9 : (q == 7 ? OPP7 : OPP5)); }
+static int lat_d(int q) { return (q == 19 || q == 7) ? 3 : 2; }
+static int lat_ok(int q) { return q == 19 || q == 9 || q == 7 || q == 5; }
+
 /* D2Q9 MRT transform (collisionMRT.h:55-90), Lallemand & Luo 2000. */
 static const int M9[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},      {-4, -1, -1, -1, -1, 2, 2, 2, 2},
                              {4, -2, -2, -2, -2, 1, 1, 1, 1},  {0, 1, -1, 0, 0, 1, -1, 1, -1},
@@ -115,6 +126,16 @@ int flbo_run_f32(const flbo_lattice* L, float* pops, float* scratch, int steps) 
 int flbo_run_f64(const flbo_lattice* L, double* pops, double* scratch, int steps) {
   m19_init();
   return run_impl_f64(L, pops, scratch, steps);
+}
+int flbo_run_coupled_f32(const flbo_lattice* flow, float* pops, float* scratch, const flbo_lattice* const* scalar,
+                         float* const* spops, float* const* sscratch, const flbo_buoyancy* buoy, int nbuoy,
+                         int nscalar, int steps) {
+  return run_coupled_impl_f32(flow, pops, scratch, scalar, spops, sscratch, buoy, nbuoy, nscalar, steps);
+}
+int flbo_run_coupled_f64(const flbo_lattice* flow, double* pops, double* scratch,
+                         const flbo_lattice* const* scalar, double* const* spops, double* const* sscratch,
+                         const flbo_buoyancy* buoy, int nbuoy, int nscalar, int steps) {
+  return run_coupled_impl_f64(flow, pops, scratch, scalar, spops, sscratch, buoy, nbuoy, nscalar, steps);
 }
 void flbo_rhou_f32(const flbo_lattice* L, const float* pops, uint8_t mask, float* rhou) {
   rhou_impl_f32(L, pops, mask, rhou);

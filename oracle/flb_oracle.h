@@ -38,11 +38,15 @@ enum {
   FLBO_DYN_SMAG_MRT = 7,       /* MRT with the Smagorinsky effective shear rate (no reference form) */
   FLBO_DYN_BGK_FORCE = 8,      /* collision::BGKForce<forcerhoU, SecondOrder, ConstForce>
                                   collision.h:61-90, moment.ur.h:564-574/639-663, force.ur.h */
-  FLBO_DYN_BGK_WRITE = 9       /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: the moments
+  FLBO_DYN_BGK_WRITE = 9,      /* collision::BGK<moment::rhoU<CELL, true>, SecondOrder>: the moments
                                   go to the RHO / VELOCITY fields (moment.h:359-377) */
+  FLBO_DYN_BGK_SCALAR_FORCE = 10, /* BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
+                                  collision.h:61-90, force.ur.h:332-353/584-616; F = sforce[id] */
+  FLBO_DYN_SCALAR_BGK = 11,    /* D2Q5/D3Q7 BGK<MomentaTuple<rho<CELL, true>, useFieldU>, SecondOrder> */
+  FLBO_DYN_SCALAR_BGK_FIELD = 12 /* D2Q5/D3Q7 BGK<useFieldrhoU, SecondOrder> */
 };
 
-enum { FLBO_HWBC_NORMAL = 0, FLBO_HWBC_MOVING = 1 };
+enum { FLBO_HWBC_NORMAL = 0, FLBO_HWBC_MOVING = 1, FLBO_HWBC_ANTI = 2 };
 
 typedef struct {
   uint8_t mask;      /* first task whose (flag & mask) != 0 runs (tmp.h:283-309) */
@@ -59,7 +63,7 @@ typedef struct {
 } flbo_hwbc;
 
 typedef struct {
-  int q;              /* 9 (D2Q9) or 19 (D3Q19) */
+  int q;              /* 9 (D2Q9), 19 (D3Q19), 5 (D2Q5) or 7 (D3Q7) */
   int nx, ny, nz;     /* storage extents including the overlap layer; nz == 1 for D2Q9 */
   int overlap;        /* overlap (ghost) width, 1 in every reference driver */
   const uint8_t* flags;     /* nx*ny*nz */
@@ -75,12 +79,34 @@ typedef struct {
                                a NaN entry marks the shear rates that take omega */
   int nthreads;       /* OpenMP threads, <= 0 for the default */
   double force[3];    /* CONSTFORCE data field of BGKForce (force density, lattice units) */
+  double* sforce;     /* SCALARFORCE field (N) of FLBO_DYN_BGK_SCALAR_FORCE cells, or NULL (0) */
+  const double* u_ext; /* scalar lattices: VELOCITY field (d*N) advecting the scalar (the flow
+                          lattice's), or NULL for u_field */
 } flbo_lattice;
+
+/* One force::Buoyancy coupling: SCALARFORCE += gbeta (scalar_rho - rho_init)
+ * on flow cells whose flag matches mask (lbm/force.h:160-181). */
+typedef struct {
+  const double* scalar_rho;  /* the scalar lattice's RHO field (N), the TEMP / CONC of FreeLB */
+  uint8_t mask;
+  uint8_t pad[7];
+  double gbeta, rho_init;
+} flbo_buoyancy;
 
 /* Run `steps` reference time steps (collide, PS stream, halfway BCs) in place
  * on pops[q][N] (logical, natural layout). scratch must hold q*N values. */
 int flbo_run_f32(const flbo_lattice* L, float* pops, float* scratch, int steps);
 int flbo_run_f64(const flbo_lattice* L, double* pops, double* scratch, int steps);
+
+/* Coupled flow + scalar lattices (examples/cazsblock2d/cazsblock2d.cpp:385-409):
+ * per step clear flow->sforce, add the buoyancy couplings, step the flow
+ * lattice, then each of the ns scalar lattices. */
+int flbo_run_coupled_f32(const flbo_lattice* flow, float* pops, float* scratch, const flbo_lattice* const* scalar,
+                         float* const* spops, float* const* sscratch, const flbo_buoyancy* buoy, int nbuoy,
+                         int nscalar, int steps);
+int flbo_run_coupled_f64(const flbo_lattice* flow, double* pops, double* scratch,
+                         const flbo_lattice* const* scalar, double* const* spops, double* const* sscratch,
+                         const flbo_buoyancy* buoy, int nbuoy, int nscalar, int steps);
 
 /* moment::rhoUImpl (moment.ur.h:461-517): rhou[(d+1)][N] = rho, u_x, u_y[, u_z]
  * on every cell whose flag matches `mask` (0 elsewhere). */

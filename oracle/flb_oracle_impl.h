@@ -18,11 +18,26 @@ static void FN(weights)(int q, REAL* w) {
     w[0] = (REAL)1 / (REAL)3;
     for (int i = 1; i <= 6; ++i) w[i] = (REAL)1 / (REAL)18;
     for (int i = 7; i <= 18; ++i) w[i] = (REAL)1 / (REAL)36;
-  } else {
+  } else if (q == 9) {
     w[0] = (REAL)4 / (REAL)9;
     for (int i = 1; i <= 4; ++i) w[i] = (REAL)1 / (REAL)9;
     for (int i = 5; i <= 8; ++i) w[i] = (REAL)1 / (REAL)36;
+  } else if (q == 5) {
+    /* D2Q5: T(1)/T(3), T(1)/T(6) (lattice_set.h:146-150) */
+    w[0] = (REAL)1 / (REAL)3;
+    for (int i = 1; i <= 4; ++i) w[i] = (REAL)1 / (REAL)6;
+  } else {
+    /* D3Q7: T(1)/T(4), T(1)/T(8) (lattice_set.h:198-202) */
+    w[0] = (REAL)1 / (REAL)4;
+    for (int i = 1; i <= 6; ++i) w[i] = (REAL)1 / (REAL)8;
   }
+}
+
+/* moment::rhoImpl<D2Q5 / D3Q7> (moment.ur.h:47-82): left-to-right sum. */
+static REAL FN(scalar_sum)(int q, const REAL* c) {
+  REAL r = c[0];
+  for (int i = 1; i < q; ++i) r = r + c[i];
+  return r;
 }
 
 /* moment::rhoUImpl<D3Q19> (moment.ur.h:505-517) and <D2Q9> (moment.ur.h:461-472). */
@@ -34,11 +49,16 @@ static void FN(rhou_cell)(int q, const REAL* c, REAL* rho, REAL* u) {
     u[0] = (c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16]) / r;
     u[1] = (c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18]) / r;
     u[2] = (c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18]) / r;
-  } else {
+  } else if (q == 9) {
     REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
     *rho = r;
     u[0] = (c[1] - c[2] + c[5] - c[6] + c[7] - c[8]) / r;
     u[1] = (c[3] - c[4] + c[5] - c[6] - c[7] + c[8]) / r;
+  } else {
+    /* scalar lattices: the sum and (sum c f) / sum as a read-out */
+    const REAL r = FN(scalar_sum)(q, c);
+    *rho = r;
+    for (int a = 0; a < lat_d(q); ++a) u[a] = (c[2 * a + 1] - c[2 * a + 2]) / r;
   }
 }
 
@@ -107,11 +127,73 @@ static void FN(force_pop_cell)(int q, const REAL* w, const REAL* u, const REAL* 
   }
 }
 
+/* forcerhoUImpl's scalar-force overload (moment.h:423-435; unrolled
+ * moment.ur.h:574-583 D2Q9, 652-663 D3Q19): sums, F/2 on the last axis, then
+ * every component divided by rho. */
+static void FN(scalar_force_rhou_cell)(int q, const REAL* c, REAL F, REAL* rho, REAL* u) {
+  if (q == 19) {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8] + c[9] + c[10] + c[11] +
+             c[12] + c[13] + c[14] + c[15] + c[16] + c[17] + c[18];
+    *rho = r;
+    u[0] = c[1] - c[2] + c[7] - c[8] + c[9] - c[10] + c[13] - c[14] + c[15] - c[16];
+    u[1] = c[3] - c[4] + c[7] - c[8] + c[11] - c[12] - c[13] + c[14] + c[17] - c[18];
+    u[2] = c[5] - c[6] + c[9] - c[10] + c[11] - c[12] - c[15] + c[16] - c[17] + c[18];
+    u[2] += F * (REAL)0.5;
+    u[0] /= r;
+    u[1] /= r;
+    u[2] /= r;
+  } else {
+    REAL r = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+    *rho = r;
+    u[0] = c[1] - c[2] + c[5] - c[6] + c[7] - c[8];
+    u[1] = c[3] - c[4] + c[5] - c[6] - c[7] + c[8];
+    u[1] += F * (REAL)0.5;
+    u[0] /= r;
+    u[1] /= r;
+  }
+}
+
+/* ScalarForcePopImpl<D2Q9, 1> (force.ur.h:332-353) and <D3Q19, 2>
+ * (force.ur.h:584-616): Fi = w_i InvCs2 F (c_id - u_d + InvCs2 (c_i.u) c_id). */
+static void FN(scalar_force_pop_cell)(int q, const REAL* w, const REAL* u, REAL F, REAL* Fi) {
+  const REAL I2 = (REAL)3;
+  if (q == 19) {
+    const REAL u0 = u[0], u1 = u[1], u2 = u[2];
+    const REAL T1 = (REAL)1 - u2, _T1 = (REAL)-1 - u2, _u = -u2;
+    const REAL k5 = I2 * (u2), k9 = I2 * (u0 + u2), k11 = I2 * (u1 + u2), k15 = I2 * (u0 - u2),
+               k17 = I2 * (u1 - u2);
+    const REAL t[19] = {_u, _u, _u, _u, _u, T1 + k5, _T1 + k5, _u, _u, T1 + k9, _T1 + k9, T1 + k11, _T1 + k11,
+                        _u, _u, _T1 - k15, T1 - k15, _T1 - k17, T1 - k17};
+    for (int i = 0; i < 19; ++i) Fi[i] = w[i] * I2 * F * (t[i]);
+  } else {
+    const REAL u0 = u[0], u1 = u[1];
+    const REAL T1 = (REAL)1 - u1, _T1 = (REAL)-1 - u1, _u = -u1;
+    const REAL k3 = I2 * (u1), k5 = I2 * (u0 + u1), k7 = I2 * (u0 - u1);
+    const REAL t[9] = {_u, _u, _u, T1 + k3, _T1 + k3, T1 + k5, _T1 + k5, _T1 - k7, T1 - k7};
+    for (int i = 0; i < 9; ++i) Fi[i] = w[i] * I2 * F * (t[i]);
+  }
+}
+
 /* equilibrium::SecondOrderImpl<D3Q19> (equilibrium.ur.h:165-214) and <D2Q9>
  * (equilibrium.ur.h:65-94). */
 static void FN(feq_cell)(int q, const REAL* w, REAL rho, const REAL* u, REAL* feq) {
   const REAL InvCs2 = (REAL)3;
   const REAL InvCs2x = (REAL)0.5 * InvCs2;
+  if (q == 5 || q == 7) {
+    /* SecondOrderImpl<D2Q5> (equilibrium.ur.h:43-63), <D3Q7> (96-119) */
+    const REAL var0 = q == 7 ? (REAL)1 - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * InvCs2x
+                             : (REAL)1 - (u[0] * u[0] + u[1] * u[1]) * InvCs2x;
+    const REAL rhowk0 = w[0] * rho;
+    const REAL rhowk1 = w[1] * rho;
+    feq[0] = rhowk0 * var0;
+    for (int a = 0; a < lat_d(q); ++a) {
+      const REAL uc = InvCs2 * (u[a]);
+      const REAL v = var0 + uc * uc * (REAL)0.5;
+      feq[2 * a + 1] = rhowk1 * (v + uc);
+      feq[2 * a + 2] = rhowk1 * (v - uc);
+    }
+    return;
+  }
   if (q == 19) {
     const REAL var0 = (REAL)1 - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * InvCs2x;
     const REAL rhowk0 = w[0] * rho;
@@ -148,8 +230,8 @@ static void FN(feq_cell)(int q, const REAL* w, REAL rho, const REAL* u, REAL* fe
 /* c_i . u for direction i, summed over axes in order from zero the way
  * Vector<T,D>::operator* does (Vector.h:229-236). */
 static REAL FN(cdotu)(int q, int i, const REAL* u) {
-  const int* ci = (q == 19) ? &C19[i][0] : &C9[i][0];
-  const int d = (q == 19) ? 3 : 2;
+  const int* ci = &lat_c(q)[i][0];
+  const int d = lat_d(q);
   REAL r = (REAL)0;
   for (int a = 0; a < d; ++a) r += u[a] * (REAL)ci[a];
   return r;
@@ -157,7 +239,7 @@ static REAL FN(cdotu)(int q, int i, const REAL* u) {
 
 /* Fetch RHO / VELOCITY field values of a cell (fields default to rho = 1, u = 0). */
 static void FN(fields)(const flbo_lattice* L, size_t id, size_t N, REAL* rho, REAL* u) {
-  const int d = (L->q == 19) ? 3 : 2;
+  const int d = lat_d(L->q);
   *rho = L->rho_field ? (REAL)L->rho_field[id] : (REAL)1;
   for (int a = 0; a < d; ++a) u[a] = L->u_field ? (REAL)L->u_field[(size_t)a * N + id] : (REAL)0;
 }
@@ -200,7 +282,7 @@ static REAL FN(smag_omega)(int q, const REAL* c, REAL rho, const REAL* u, REAL o
 static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const REAL* w, REAL omega,
                              REAL _omega, REAL* c, size_t id, size_t N) {
   const int q = L->q;
-  const int d = (q == 19) ? 3 : 2;
+  const int d = lat_d(q);
   REAL rho, u[3] = {0, 0, 0}, feq[19];
   switch (t->dynamics) {
     case FLBO_DYN_BGK:
@@ -236,7 +318,7 @@ static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const RE
       FN(fields)(L, id, N, &rf, uf);
       const REAL rhox = (REAL)6 * rf;
       for (int i = 1; i < q; i += 2) {
-        const int* ci = (q == 19) ? &C19[i][0] : &C9[i][0];
+        const int* ci = &lat_c(q)[i][0];
         /* the unrolled form sums only the non-zero components: u0, u1, u0+u1, u0-u1, ... */
         REAL cu = 0;
         int first = 1;
@@ -283,6 +365,40 @@ static void FN(collide_cell)(const flbo_lattice* L, const flbo_task* t, const RE
       for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i] + fomega * Fi[i];
       break;
     }
+    case FLBO_DYN_BGK_SCALAR_FORCE: {
+      /* collision::BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder,
+       * ScalarForce> (collision.h:68-87): F is the cell's SCALARFORCE field;
+       * rho, u go to the RHO / VELOCITY fields (moment.h:430-434) */
+      const REAL F = L->sforce ? (REAL)L->sforce[id] : (REAL)0;
+      REAL Fi[19];
+      FN(scalar_force_rhou_cell)(q, c, F, &rho, u);
+      if (L->rho_field) L->rho_field[id] = (double)rho;
+      if (L->u_field)
+        for (int a = 0; a < d; ++a) L->u_field[(size_t)a * N + id] = (double)u[a];
+      FN(scalar_force_pop_cell)(q, w, u, F, Fi);
+      FN(feq_cell)(q, w, rho, u, feq);
+      const REAL fomega = (REAL)1 - (REAL)0.5 * omega;
+      for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i] + fomega * Fi[i];
+      break;
+    }
+    case FLBO_DYN_SCALAR_BGK:
+    case FLBO_DYN_SCALAR_BGK_FIELD: {
+      /* collision::BGK on a D2Q5 / D3Q7 lattice with u from the VELOCITY field
+       * (useFieldU, moment.h:56-69); the scalar is the population sum written
+       * to the field (rho<CELL, true>, moment.h:159-182) or the field value
+       * (useFieldrhoU, moment.h:71-90) */
+      const double* ue = L->u_ext ? L->u_ext : L->u_field;
+      for (int a = 0; a < d; ++a) u[a] = ue ? (REAL)ue[(size_t)a * N + id] : (REAL)0;
+      if (t->dynamics == FLBO_DYN_SCALAR_BGK) {
+        rho = FN(scalar_sum)(q, c);
+        if (L->rho_field) L->rho_field[id] = (double)rho;
+      } else {
+        rho = L->rho_field ? (REAL)L->rho_field[id] : (REAL)1;
+      }
+      FN(feq_cell)(q, w, rho, u, feq);
+      for (int i = 0; i < q; ++i) c[i] = omega * feq[i] + _omega * c[i];
+      break;
+    }
     default:
       break;
   }
@@ -305,8 +421,8 @@ static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
   const int q = L->q;
   const size_t nx = (size_t)L->nx, ny = (size_t)L->ny, nz = (size_t)L->nz;
   const size_t N = nx * ny * nz;
-  const int (*C)[3] = (q == 19) ? C19 : C9;
-  const int* OPP = (q == 19) ? OPP19 : OPP9;
+  const int (*C)[3] = lat_c(q);
+  const int* OPP = lat_opp(q);
   REAL w[19];
   FN(weights)(q, w);
   const REAL omega = (REAL)L->omega;
@@ -346,7 +462,7 @@ static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
   }
 
   const int ov = L->overlap;
-  const int d = (q == 19) ? 3 : 2;
+  const int d = lat_d(q);
   for (int b = 0; b < L->nbc; ++b) {
     const flbo_hwbc* bc = &L->bcs[b];
     const size_t zlo = d == 3 ? (size_t)ov : 0, zhi = d == 3 ? nz - (size_t)ov : 1;
@@ -357,7 +473,7 @@ static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
           const size_t id = x + nx * (y + ny * z);
           if (!(flags[id] & bc->cell_mask)) continue;
           REAL rf = 1, uf[3] = {0, 0, 0};
-          if (bc->kind == FLBO_HWBC_MOVING) FN(fields)(L, id, N, &rf, uf);
+          if (bc->kind != FLBO_HWBC_NORMAL) FN(fields)(L, id, N, &rf, uf);
           for (int k = 1; k < q; ++k) {
             const size_t nb = (size_t)((ptrdiff_t)id + delta[k]);
             if (!(flags[nb] & bc->void_mask)) continue;
@@ -366,6 +482,10 @@ static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
             if (bc->kind == FLBO_HWBC_MOVING) {
               /* 2 * InvCs2 * w_o * RHO * (VELOCITY . c_o) */
               v = v + (REAL)6 * w[o] * rf * FN(cdotu)(q, o, uf);
+            } else if (bc->kind == FLBO_HWBC_ANTI) {
+              /* bounceback::anti_simple (bounce_back_boundary.h:183-193):
+               * 2 * RHO * w_o - getPrevious(k) */
+              v = (REAL)2 * rf * w[o] - v;
             }
             B[(size_t)o * N + id] = v;
           }
@@ -376,7 +496,7 @@ static void FN(step)(const flbo_lattice* L, REAL* A, REAL* B) {
 }
 
 int FN(run_impl)(const flbo_lattice* L, REAL* pops, REAL* scratch, int steps) {
-  if (!L || !pops || !scratch || (L->q != 19 && L->q != 9) || steps < 0) return -1;
+  if (!L || !pops || !scratch || !lat_ok(L->q) || steps < 0) return -1;
   const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
   REAL* A = pops;
   REAL* B = scratch;
@@ -390,9 +510,57 @@ int FN(run_impl)(const flbo_lattice* L, REAL* pops, REAL* scratch, int steps) {
   return 0;
 }
 
+/* The coupled loop of examples/cazsblock2d/cazsblock2d.cpp:385-409 for one
+ * flow lattice and its scalar lattices, per step:
+ *   SCALARFORCE.InitValue(0) (field_struct.h:238);
+ *   for each coupling: BlockLatManagerCoupling::ApplyCellDynamics with
+ *     Key_TypePair<mask, force::Buoyancy> (block_lattice.h:488-509,
+ *     force.h:173-181): SCALARFORCE += gbeta (RHO_scalar - rho_init) on the
+ *     cells whose flag matches;
+ *   the flow step (collide, stream, halfway managers);
+ *   each scalar lattice's step (its u_ext is the flow's VELOCITY field).
+ * pops / scratch as flbo_run, per lattice. */
+int FN(run_coupled_impl)(const flbo_lattice* flow, REAL* pf, REAL* sf, const flbo_lattice* const* scal,
+                         REAL* const* ps, REAL* const* ss, const flbo_buoyancy* bu, int nb, int ns, int steps) {
+  if (!flow || !pf || !sf || !flow->sforce || !lat_ok(flow->q) || nb < 0 || ns < 0 || steps < 0) return -1;
+  const size_t N = (size_t)flow->nx * (size_t)flow->ny * (size_t)flow->nz;
+  REAL* A = pf;
+  REAL* B = sf;
+  REAL* SA[4];
+  REAL* SB[4];
+  if (ns > 4 || nb > 4) return -1;
+  for (int k = 0; k < ns; ++k) {
+    SA[k] = ps[k];
+    SB[k] = ss[k];
+  }
+  for (int s = 0; s < steps; ++s) {
+    for (size_t id = 0; id < N; ++id) flow->sforce[id] = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      const REAL gb = (REAL)bu[k].gbeta, r0 = (REAL)bu[k].rho_init;
+      const double* rs = bu[k].scalar_rho;
+      for (size_t id = 0; id < N; ++id)
+        if (flow->flags[id] & bu[k].mask) flow->sforce[id] = (double)((REAL)flow->sforce[id] + gb * ((REAL)rs[id] - r0));
+    }
+    FN(step)(flow, A, B);
+    REAL* t = A;
+    A = B;
+    B = t;
+    for (int k = 0; k < ns; ++k) {
+      FN(step)(scal[k], SA[k], SB[k]);
+      t = SA[k];
+      SA[k] = SB[k];
+      SB[k] = t;
+    }
+  }
+  if (A != pf) memcpy(pf, A, sizeof(REAL) * (size_t)flow->q * N);
+  for (int k = 0; k < ns; ++k)
+    if (SA[k] != ps[k]) memcpy(ps[k], SA[k], sizeof(REAL) * (size_t)scal[k]->q * N);
+  return 0;
+}
+
 void FN(rhou_impl)(const flbo_lattice* L, const REAL* pops, uint8_t mask, REAL* rhou) {
   const int q = L->q;
-  const int d = (q == 19) ? 3 : 2;
+  const int d = lat_d(q);
   const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
 #pragma omp parallel for schedule(static)
   for (size_t id = 0; id < N; ++id) {
@@ -408,7 +576,7 @@ void FN(rhou_impl)(const flbo_lattice* L, const REAL* pops, uint8_t mask, REAL* 
 
 void FN(init_feq_impl)(const flbo_lattice* L, const REAL* rhou, REAL* pops) {
   const int q = L->q;
-  const int d = (q == 19) ? 3 : 2;
+  const int d = lat_d(q);
   const size_t N = (size_t)L->nx * (size_t)L->ny * (size_t)L->nz;
   REAL w[19];
   FN(weights)(q, w);
@@ -428,11 +596,11 @@ void FN(init_feq_impl)(const flbo_lattice* L, const REAL* rhou, REAL* pops) {
  * uc = u . c_k and u2 = |u|^2 summed from 0 in component order,
  * w rho (1 + 3 uc + uc uc 0.5 9 - 3 u2 0.5) evaluated left to right. */
 static void FN(feq_order2)(int q, const REAL* w, REAL rho, const REAL* u, REAL* e) {
-  const int d = (q == 19) ? 3 : 2;
+  const int d = lat_d(q);
   REAL u2 = 0;
   for (int a = 0; a < d; ++a) u2 += u[a] * u[a];
   for (int k = 0; k < q; ++k) {
-    const int* ck = (q == 19) ? &C19[k][0] : &C9[k][0];
+    const int* ck = &lat_c(q)[k][0];
     REAL uc = 0;
     for (int a = 0; a < d; ++a) uc += u[a] * (REAL)ck[a];
     e[k] = w[k] * rho * ((REAL)1 + (REAL)3 * uc + uc * uc * (REAL)0.5 * (REAL)9 - (REAL)3 * u2 * (REAL)0.5);
@@ -448,7 +616,7 @@ static REAL FN(field_u)(const flbo_lattice* L, int a, size_t id, size_t N) {
 
 void CAT(flbo_refine_average, SFX)(const flbo_lattice* C, REAL* popsC, const flbo_lattice* F, const REAL* popsF,
                                    const int* origin) {
-  const int q = C->q, d = (q == 19) ? 3 : 2, ns = 1 << d;
+  const int q = C->q, d = lat_d(q), ns = 1 << d;
   const size_t NC = (size_t)C->nx * C->ny * C->nz, NF = (size_t)F->nx * F->ny * F->nz;
   const int m[3] = {(F->nx - 4) / 2, (F->ny - 4) / 2, d == 3 ? (F->nz - 4) / 2 : 1};
   const REAL omega_f = (REAL)F->omega, scale = d == 3 ? (REAL)0.125 : (REAL)0.25;
@@ -490,7 +658,7 @@ void CAT(flbo_refine_average, SFX)(const flbo_lattice* C, REAL* popsC, const flb
 
 void CAT(flbo_refine_interpolate, SFX)(const flbo_lattice* C, const REAL* popsC, const flbo_lattice* F,
                                        REAL* popsF, const int* origin) {
-  const int q = C->q, d = (q == 19) ? 3 : 2, ns = 1 << d;
+  const int q = C->q, d = lat_d(q), ns = 1 << d;
   const size_t NC = (size_t)C->nx * C->ny * C->nz, NF = (size_t)F->nx * F->ny * F->nz;
   const int fn[3] = {F->nx, F->ny, d == 3 ? F->nz : 1};
   const REAL omega_c = (REAL)C->omega;

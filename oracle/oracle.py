@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 (DYN_NONE, DYN_BGK, DYN_BGK_FIELD, DYN_BOUNCEBACK, DYN_BOUNCEBACK_MW, DYN_MRT, DYN_SMAG_BGK, DYN_SMAG_MRT,
- DYN_BGK_FORCE, DYN_BGK_WRITE) = range(10)
-HWBC_NORMAL, HWBC_MOVING = 0, 1
+ DYN_BGK_FORCE, DYN_BGK_WRITE, DYN_BGK_SCALAR_FORCE, DYN_SCALAR_BGK, DYN_SCALAR_BGK_FIELD) = range(13)
+HWBC_NORMAL, HWBC_MOVING, HWBC_ANTI = 0, 1, 2
 
 # Cavity flags of the reference drivers (benchmarks/cavity3d/cavity3d.cpp:121-124).
 VOID, AABB, BB, BBMW = 1, 2, 4, 8
@@ -44,7 +44,13 @@ class _Lattice(C.Structure):
                 ("tasks", _Task * 8), ("nbc", C.c_int), ("bcs", _HWBC * 4),
                 ("omega", C.c_double), ("rho_field", C.POINTER(C.c_double)),
                 ("u_field", C.POINTER(C.c_double)), ("mrt_rates", C.POINTER(C.c_double)),
-                ("nthreads", C.c_int), ("force", C.c_double * 3)]
+                ("nthreads", C.c_int), ("force", C.c_double * 3),
+                ("sforce", C.POINTER(C.c_double)), ("u_ext", C.POINTER(C.c_double))]
+
+
+class _Buoyancy(C.Structure):
+    _fields_ = [("scalar_rho", C.POINTER(C.c_double)), ("mask", C.c_uint8), ("pad", C.c_uint8 * 7),
+                ("gbeta", C.c_double), ("rho_init", C.c_double)]
 
 
 _lib = None
@@ -71,6 +77,13 @@ def lib():
                                                                      P, P2]
             getattr(_lib, "flbo_refine_interpolate_" + name).argtypes = [C.POINTER(_Lattice), P,
                                                                          C.POINTER(_Lattice), P, P2]
+        for name, T in (("f32", C.c_float), ("f64", C.c_double)):
+            P = C.POINTER(T)
+            PP = C.POINTER(P)
+            fn = getattr(_lib, "flbo_run_coupled_" + name)
+            fn.argtypes = [C.POINTER(_Lattice), P, P, C.POINTER(C.POINTER(_Lattice)), PP, PP,
+                           C.POINTER(_Buoyancy), C.c_int, C.c_int, C.c_int]
+            fn.restype = C.c_int
         _lib.flbo_cavity_flags.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)]
         _lib.flbo_convert_from_rt.argtypes = [C.c_float, C.c_float, C.c_float,
                                               C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -128,7 +141,7 @@ class Lattice:
 
     @property
     def d(self):
-        return 3 if self.q == 19 else 2
+        return 3 if self.q in (19, 7) else 2
 
     def run(self, pops, steps):
         """Advance pops[q][N] (float32 or float64, natural layout) in place."""
@@ -162,6 +175,39 @@ class Lattice:
         rho = self.buffers.get("rho_field", np.ones(self.N))
         u = self.buffers.get("u_field", np.zeros(self.d * self.N)).reshape(self.d, self.N)
         return np.concatenate([rho.reshape(1, -1), u], axis=0)
+
+
+def run_coupled(flow, pops, scalars, spops, couplings, steps):
+    """The coupled loop (flbo_run_coupled): flow lattice with a SCALARFORCE
+    field, scalar lattices advected by the flow's VELOCITY field, buoyancy
+    couplings (scalar index, mask, gbeta, rho_init). pops / spops are advanced
+    in place; the flow's fields and SCALARFORCE and the scalars' RHO fields
+    are updated in their buffers."""
+    T = C.c_float if pops.dtype == np.float32 else C.c_double
+    name = "f32" if pops.dtype == np.float32 else "f64"
+    if "u_field" not in flow.buffers or "rho_field" not in flow.buffers:
+        raise ValueError("the flow lattice needs RHO and VELOCITY fields")
+    sforce = flow.buffers.setdefault("sforce", np.zeros(flow.N))
+    flow.L.sforce = _ptr(sforce, C.c_double)
+    uf = flow.buffers["u_field"]
+    for sc in scalars:
+        sc.L.u_ext = _ptr(uf, C.c_double)
+        sc._keep.append(uf)
+    n = len(scalars)
+    LP = (C.POINTER(_Lattice) * max(1, n))(*[C.pointer(sc.L) for sc in scalars])
+    scratch = [np.empty_like(p) for p in spops]
+    PA = (C.POINTER(T) * max(1, n))(*[_ptr(p, T) for p in spops])
+    PB = (C.POINTER(T) * max(1, n))(*[_ptr(p, T) for p in scratch])
+    B = (_Buoyancy * max(1, len(couplings)))()
+    for k, (si, mask, gbeta, rho0) in enumerate(couplings):
+        B[k].scalar_rho = _ptr(scalars[si].buffers["rho_field"], C.c_double)
+        B[k].mask, B[k].gbeta, B[k].rho_init = mask, float(gbeta), float(rho0)
+    fscratch = np.empty_like(pops)
+    rc = getattr(lib(), "flbo_run_coupled_" + name)(C.byref(flow.L), _ptr(pops, T), _ptr(fscratch, T), LP, PA,
+                                                    PB, B, len(couplings), n, int(steps))
+    if rc != 0:
+        raise RuntimeError(f"flbo_run_coupled_{name} failed ({rc})")
+    return pops
 
 
 def refine_average(coarse, popsC, fine, popsF, origin):

@@ -10,7 +10,8 @@ reference sources, so it runs in the build container, not on the GPU box.
 
     python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures,
                                                           "refined" for the refined-cavity ones,
-                                                          "vtk" for FreeLB's VTK files)
+                                                          "vtk" for FreeLB's VTK files,
+                                                          "natconv" for the coupled-lattice ones)
 """
 from __future__ import annotations
 
@@ -50,6 +51,12 @@ EXE_REF = os.path.join(ROOT, "oracle", "_ref", "refine_ref")
 # rhoU<CELL, true> read-out: (lattice, Ni, Nj, Nk, steps, overlap threshold, name)
 VTK = [("d3q19", 8, 7, 6, 40, 0, "cav3"), ("d2q9", 16, 13, 1, 100, -1, "cav2")]
 EXE_VTK = os.path.join(ROOT, "oracle", "_ref", "vtk_ref")
+# coupled flow + temperature lattices, a differentially heated cavity (oracle/_ref/natconv_ref,
+# the coupling loop of examples/cazsblock2d/cazsblock2d.cpp): (lattice, Ni, Nj, Nk, steps, RT_flow,
+# RT_temp, gbeta, T_hot, T_cold, T_init)
+NATCONV = [("d2q9", 24, 16, 1, 300, "0.6", "0.7", "0.001", "1.5", "0.5", "1.0"),
+           ("d3q19", 10, 8, 6, 120, "0.6", "0.8", "0.002", "1.5", "0.5", "1.0")]
+EXE_NC = os.path.join(ROOT, "oracle", "_ref", "natconv_ref")
 
 
 def read_dump(path):
@@ -94,6 +101,38 @@ def read_refined(path):
     return dict(d=d, q=q, steps=steps, uw=uw, blocks=blocks)
 
 
+def read_natconv(path):
+    """FLBNC001: flags, the initial and final flow / temperature populations,
+    the final RHO, VELOCITY, TEMP and SCALARFORCE fields."""
+    b = open(path, "rb").read()
+    assert b[:8] == b"FLBNC001"
+    d, qf, qt, nx, ny, nz, steps = (int(v) for v in np.frombuffer(b[8:36], dtype=np.int32))
+    prm = np.frombuffer(b[36:60], dtype=np.float32).copy()
+    n = nx * ny * nz
+    off = 60
+    flags = np.frombuffer(b[off:off + n], dtype=np.uint8).copy()
+    off += n
+
+    def take(k):
+        nonlocal off
+        a = np.frombuffer(b[off:off + 4 * k], dtype=np.float32).copy()
+        off += 4 * k
+        return a
+
+    out = dict(d=d, qf=qf, qt=qt, dims=np.array([nx, ny, nz]), steps=steps, params=prm, flags=flags)
+    out["flow0"] = take(qf * n).reshape(qf, n)
+    out["temp0"] = take(qt * n).reshape(qt, n)
+    out["T0"] = take(n)
+    out["flow"] = take(qf * n).reshape(qf, n)
+    out["temp"] = take(qt * n).reshape(qt, n)
+    out["rho"] = take(n)
+    out["u"] = take(d * n).reshape(d, n)
+    out["T"] = take(n)
+    out["sforce"] = take(n)
+    assert off == len(b)
+    return out
+
+
 def run_refined(lat, n, cell, p0, p1, counts, tmp):
     out = os.path.join(tmp, f"refine_{lat}_{n}_{cell}_{p0}_{p1}_{counts}.bin")
     nk = n if lat == "d3q19" else 1
@@ -132,6 +171,20 @@ def main():
                     with open(os.path.join(src, fn), "rb") as a, open(os.path.join(outdir, fn), "wb") as b:
                         b.write(a.read())
                     print("wrote vtk/" + fn, flush=True)
+            if only:
+                return
+        if only in (None, "natconv"):
+            if not os.path.exists(EXE_NC):
+                subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "_ref/natconv_ref"], check=True)
+            for case in NATCONV:
+                lat, ni, nj, nk, steps = case[:5]
+                out = os.path.join(tmp, f"natconv_{lat}.bin")
+                subprocess.run([EXE_NC, lat, str(ni), str(nj), str(nk), str(steps), *case[5:], out], check=True,
+                               capture_output=True)
+                r = read_natconv(out)
+                name = f"natconv_{lat}_{ni}x{nj}x{nk}_{steps}.npz"
+                np.savez_compressed(os.path.join(HERE, name), **{k: np.asarray(v) for k, v in r.items()})
+                print("wrote", name, flush=True)
             if only:
                 return
         if only in (None, "refined"):
