@@ -450,10 +450,11 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
   const int ex_x = desc->periodic[0] ? desc->nx : desc->nx - 2 * desc->overlap;
   // Short rows (at most 222 updated cells) take a compact pitch instead:
   // room for the two overlap cells, rounded up to FLB_PITCH_ROUND slots
-  // (default 8), so that neighbouring rows share cache lines rather than
-  // each filling a mostly empty 320-slot row (512 cavities of 16..96 cells
-  // per edge: 8 slots +6 % over 32, 1 slot +5 %).
-  int round = 8;
+  // (default 1: no rounding), so that neighbouring rows share cache lines
+  // rather than each filling a mostly empty 320-slot row. On 512 cavities of
+  // 16..96 cells per edge, 1 slot runs 3 % faster than 8 and 22 % faster
+  // than 16.
+  int round = 1;
   if (const char* e = std::getenv("FLB_PITCH_ROUND")) round = std::max(1, std::min(256, std::atoi(e)));
   const int pitch = ex_x + 2 <= 224 ? (ex_x + 2 + round - 1) / round * round
                                     : std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
@@ -496,6 +497,9 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
     b->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
   }
   b->elem = desc->storage == FLB_STORE_F64 ? 8 : (desc->storage == FLB_STORE_F32 ? 4 : 2);
+  // one wavefront per workgroup for the 16-bit kinds (measured ~1 % faster on
+  // the 256^3 cavity; FP32 / FP64 do not care)
+  if (b->elem == 2) b->wpb = 1;
   b->real = desc->storage == FLB_STORE_F64 ? 8 : 4;
   b->stream = static_cast<hipStream_t>(desc->stream);
   if (hipGetDevice(&b->device) != hipSuccess) {
