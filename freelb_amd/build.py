@@ -36,11 +36,14 @@ FLAGS = [
 # Per-source flags. The 16-bit storage kinds have no FreeLB counterpart to
 # match bit for bit, so their collision arithmetic may contract to FMAs and
 # use the hardware reciprocal: their kernels are instruction-bound rather
-# than memory-bound.
+# than memory-bound. Contraction stays within one source expression
+# (-ffp-contract=on, as fast as =fast here), so the same cell update inlined
+# into different kernels (the one-cell and two-cell steps, batches) rounds
+# the same way and their results agree bit for bit.
 SOURCE_FLAGS = {
-    "flb_inst_f16s.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-    "flb_inst_bf16s.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-    "flb_inst_f16c.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "flb_inst_f16s.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "flb_inst_bf16s.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "flb_inst_f16c.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -488,6 +488,65 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   }
 }
 
+// The even step of the 16-bit storage kinds with two cells per lane: each
+// direction is one 32-bit word per lane, so a wavefront moves 256 bytes per
+// load or store instruction, as the FP32 step does with one cell per lane
+// (one cell per lane moves 128). The even step has no halfway handling (its
+// walls are the odd step's), so the two cells of a lane are independent:
+// each is collided on its own, and a pair whose cells are not both updated
+// stores its live half alone. Needs an even row pitch (every row and
+// direction starts on a 4-byte boundary) and row segments, a.wpr counting
+// wavefronts of 128 cells.
+#ifndef FLB_EVEN_PAIR
+#define FLB_EVEN_PAIR 1
+#endif
+template <int Q, class S, int PAR, int FAM>
+__global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
+  using R = typename S::R;
+  using E = typename S::E;
+  static_assert(sizeof(E) == 2, "two 16-bit populations per 32-bit word");
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  const int wib = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int gw = int(xcd_block(blockIdx.x, unsigned(a.nblocks))) * a.wpb + wib;
+  const int row = gw / a.wpr;
+  if (row >= a.ey * a.ez) return;
+  const int wv = gw - row * a.wpr;
+  const int y = a.ys + row % a.ey;
+  const int z = a.zs + row / a.ey;
+  const int c0 = 2 * (wv * kWave + lane);
+  if (c0 >= a.ex) return;
+  const bool two = c0 + 1 < a.ex;
+  const int s0 = c0 + a.pitch * y + int(a.plane) * z;
+  E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
+  float cs0 = 0.0f, cs1 = 0.0f;
+  const int d0 = match_task(a, a.flags[s0], cs0);
+  const int d1 = two ? match_task(a, a.flags[s0 + 1], cs1) : int(DYN_NONE);
+  if (d0 == DYN_NONE && d1 == DYN_NONE) return;
+  R f0[Q], f1[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>((P + int64_t(i) * a.dpop) + c0);
+    const uint32_t w = ld_pop<PAR>(p);
+    f0[i] = S::template from<Q>(E(w & 0xffffu), i);
+    f1[i] = S::template from<Q>(E(w >> 16), i);
+  }
+  const R om = R(a.omega);
+  if (d0 != DYN_NONE) collide<Q, R, FAM>(a, s0, d0, cs0, f0, om, R(1) - om);
+  if (d1 != DYN_NONE) collide<Q, R, FAM>(a, s0 + 1, d1, cs1, f1, om, R(1) - om);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    E* const q = (P + int64_t(opp(i)) * a.dpop) + c0;
+    const E e0 = S::template to<Q>(f0[i], i), e1 = S::template to<Q>(f1[i], i);
+    if (d0 != DYN_NONE && d1 != DYN_NONE) {
+      st_pop<PAR>(reinterpret_cast<uint32_t*>(q), uint32_t(e0) | (uint32_t(e1) << 16));
+    } else if (d0 != DYN_NONE) {
+      st_pop<PAR>(q, e0);
+    } else {
+      st_pop<PAR>(q + 1, e1);
+    }
+  }
+}
+
 template <int Q, class S, int PAR, int FAM, bool PER>
 __global__ __launch_bounds__(kMaxThreads) void k_step_cell(KArgs a) {
   const int lane = int(threadIdx.x) & (kWave - 1);
@@ -1081,12 +1140,35 @@ hipError_t step_scalar_q(const KArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The even step of a 16-bit block in the cell form over row segments, with
+// an even pitch: two cells per lane (k_step_even_pair).
+template <int Q, class S>
+hipError_t even_pair_q(int family, bool nt, const KArgs& a0, hipStream_t s) {
+  KArgs a = a0;
+  a.wpr = (a.ex + 2 * kWave - 1) / (2 * kWave);
+  a.nblocks = int((int64_t(a.wpr) * a.ey * a.ez + a.wpb - 1) / a.wpb);
+  const dim3 grid(a.nblocks), block(kWave * a.wpb);
+  if (family == FAM_FULL) {
+    if (nt) hipLaunchKernelGGL((k_step_even_pair<Q, S, 0, FAM_FULL>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_step_even_pair<Q, S, 2, FAM_FULL>), grid, block, 0, s, a);
+  } else {
+    if (nt) hipLaunchKernelGGL((k_step_even_pair<Q, S, 0, FAM_BGK>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_step_even_pair<Q, S, 2, FAM_BGK>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
 // Non-temporal even steps for blocks with wide rows (pitch of whole
 // 256-slot groups) and for the row form; compact rows take PAR 2.
 template <class S>
 hipError_t launch_step(int q, int parity, int family, const KArgs& a, hipStream_t s) {
   const bool per = a.px || a.py || a.pz;
   const bool nt = a.pitch >= 256 || !(per || a.vec <= 0);
+  if constexpr (sizeof(typename S::E) == 2 && FLB_EVEN_PAIR) {
+    if (!parity && !per && (a.vec == 0 || a.vec == -1) && a.pitch % 2 == 0 && (q == 19 || q == 9))
+      return q == 19 ? even_pair_q<19, S>(family, a.pitch >= 256, a, s)
+                     : even_pair_q<9, S>(family, a.pitch >= 256, a, s);
+  }
   if (q == 19)
     return parity ? step_q<19, S, 1>(family, a, s)
                   : (nt ? step_q<19, S, 0>(family, a, s) : step_q<19, S, 2>(family, a, s));
