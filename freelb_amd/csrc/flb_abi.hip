@@ -149,6 +149,8 @@ bool linear_form(int ex) {
 KArgs make_args(const flb_block_s* b) {
   KArgs a{};
   a.pops = static_cast<char*>(b->d_pops) + flb::kFrontPad * b->elem;
+  const uint64_t pop_bytes = uint64_t(b->pop_elems) * b->elem;
+  a.pop_bytes = pop_bytes < (uint64_t(1) << 32) ? uint32_t(pop_bytes) : 0u;
   a.dpop = b->dpop;
   a.ypop = b->ypop;
   a.zpop = b->zpop;

@@ -68,6 +68,7 @@ struct KArgs {
   uint8_t bckind[kMaxBc];
   uint8_t bcany;         // OR of bcmask: cells that may have halfway links
   const uint64_t* links; // slot 0 of the precomputed wall links (normal | moving << 32)
+  uint32_t pop_bytes;    // bytes of the population allocation when below 4 GiB, else 0
   double omega;
   double rates[19];      // MRT rates; < 0 or NaN marks the shear rates
   double force[3];       // constant body-force density of DYN_BGK_FORCE
