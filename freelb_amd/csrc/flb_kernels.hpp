@@ -85,21 +85,23 @@ __device__ __forceinline__ int ystr(const KArgs& a) {
   return POP ? a.ypop : a.pitch;
 }
 template <bool POP>
-__device__ __forceinline__ int zstr(const KArgs& a) {
-  return POP ? int(a.zpop) : int(a.plane);
+__device__ __forceinline__ int64_t zstr(const KArgs& a) {
+  return POP ? a.zpop : a.plane;
 }
 
 // Slot offset of the y/z part of c_i for a cell in row (y, z), wrapping
-// periodic axes (uniform over a wavefront, which covers one row).
+// periodic axes (uniform over a wavefront, which covers one row). 64-bit:
+// the wrap across a periodic z axis spans the whole allocation, which holds
+// more than 2^31 populations in blocks of 113 M cells and up.
 template <int Q, bool PER, bool POP>
-__device__ __forceinline__ int yz_off(const KArgs& a, int i, int y, int z) {
+__device__ __forceinline__ int64_t yz_off(const KArgs& a, int i, int y, int z) {
   const int cy = Lat<Q>::cy(i), cz = Lat<Q>::cz(i);
-  int o = cy * ystr<POP>(a) + cz * zstr<POP>(a);
+  int64_t o = int64_t(cy * ystr<POP>(a)) + cz * zstr<POP>(a);
   if (PER) {
     if (a.py && cy != 0) {
       const int yn = y + cy;
-      if (yn < 0) o += a.ny * ystr<POP>(a);
-      else if (yn >= a.ny) o -= a.ny * ystr<POP>(a);
+      if (yn < 0) o += int64_t(a.ny) * ystr<POP>(a);
+      else if (yn >= a.ny) o -= int64_t(a.ny) * ystr<POP>(a);
     }
     if (Lat<Q>::D == 3 && a.pz && cz != 0) {
       const int zn = z + cz;
@@ -112,9 +114,9 @@ __device__ __forceinline__ int yz_off(const KArgs& a, int i, int y, int z) {
 
 // Full slot offset of the neighbour x + c_i.
 template <int Q, bool PER, bool POP>
-__device__ __forceinline__ int nb_off(const KArgs& a, int i, int x, int y, int z) {
+__device__ __forceinline__ int64_t nb_off(const KArgs& a, int i, int x, int y, int z) {
   const int cx = Lat<Q>::cx(i);
-  int o = cx + yz_off<Q, PER, POP>(a, i, y, z);
+  int64_t o = cx + yz_off<Q, PER, POP>(a, i, y, z);
   if (PER && a.px && cx != 0) {
     const int xn = x + cx;
     if (xn < 0) o += a.nx;
