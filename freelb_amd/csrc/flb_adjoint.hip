@@ -60,6 +60,7 @@ struct AdjArgs {
   int px, py, pz;
   int64_t N;
   const uint8_t* solid;  // nonzero: not updated, links into it bounce back
+  const uint32_t* link;  // per cell: bit 0 solid, bit i the link along c_i is a wall
   const R* s;            // Brinkman solid fraction per cell
   R omega;
   R G[3];
@@ -86,6 +87,17 @@ __device__ __forceinline__ int64_t neighbour(const AdjArgs<R>& a, int x, int y, 
   }
   const int64_t id = int64_t(xn) + int64_t(a.nx) * (yn + int64_t(a.ny) * zn);
   return a.solid[id] ? -1 : id;
+}
+
+// Neighbour id along c_i of a cell whose link word marks that link open (so
+// only a periodic axis can wrap).
+template <class R>
+__device__ __forceinline__ int64_t open_neighbour(const AdjArgs<R>& a, int x, int y, int z, int i) {
+  int xn = x + Lat<Q>::cx(i), yn = y + Lat<Q>::cy(i), zn = z + Lat<Q>::cz(i);
+  if (Lat<Q>::cx(i)) xn = xn < 0 ? xn + a.nx : (xn >= a.nx ? xn - a.nx : xn);
+  if (Lat<Q>::cy(i)) yn = yn < 0 ? yn + a.ny : (yn >= a.ny ? yn - a.ny : yn);
+  if (Lat<Q>::cz(i)) zn = zn < 0 ? zn + a.nz : (zn >= a.nz ? zn - a.nz : zn);
+  return int64_t(xn) + int64_t(a.nx) * (yn + int64_t(a.ny) * zn);
 }
 
 // rho - 1, rho, u, F and D of a cell from its shifted populations g (the
@@ -137,11 +149,30 @@ __device__ __forceinline__ void cell_xyz(int64_t id, int nx, int ny, int& x, int
   z = int(zz);
 }
 
+// ---- link words: bit 0 the cell is solid, bit i its neighbour along c_i is a
+// wall (solid, or beyond a non-periodic face). Rebuilt whenever the solid mask
+// changes; the steps then test one word per cell instead of loading the 18
+// neighbours' solid flags.
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_links(AdjArgs<R> a, uint32_t* __restrict__ link) {
+  const int64_t id = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (id >= a.N) return;
+  int x, y, z;
+  cell_xyz(id, a.nx, a.ny, x, y, z);
+  uint32_t w = a.solid[id] ? 1u : 0u;
+#pragma unroll
+  for (int i = 1; i < Q; ++i)
+    if (neighbour(a, x, y, z, i) < 0) w |= 1u << i;
+  link[id] = w;
+}
+
 // ---- forward step: state src -> dst (push, halfway bounce-back) -------------
 template <class R>
 __device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restrict__ src,
                                          R* __restrict__ dst, int64_t id) {
-  if (id >= a.N || a.solid[id]) return;
+  if (id >= a.N) return;
+  const uint32_t lk = a.link[id];
+  if (lk & 1u) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
   R g[Q];
@@ -159,9 +190,8 @@ __device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restric
     const R t2 = R(3) * (R(Lat<Q>::cz(i)) - u[2]) + R(9) * cu * R(Lat<Q>::cz(i));
     const R Fi = Lat<Q>::template w<R>(i) * (F[0] * t0 + F[1] * t1 + F[2] * t2);
     const R v = om * geq(i, drho, rho, u, usq) + _om * g[i] + fom * Fi;
-    const int64_t nb = i == 0 ? id : neighbour(a, x, y, z, i);
-    if (nb < 0) dst[int64_t(opp(i)) * a.N + id] = v;
-    else dst[int64_t(i) * a.N + nb] = v;
+    if (i != 0 && ((lk >> i) & 1u)) dst[int64_t(opp(i)) * a.N + id] = v;
+    else dst[int64_t(i) * a.N + (i == 0 ? id : open_neighbour(a, x, y, z, i))] = v;
   }
 }
 
@@ -253,7 +283,9 @@ template <class R>
 __device__ __forceinline__ void bwd_cell(const AdjArgs<R>& a, const R* __restrict__ fst,
                                          const R* __restrict__ lam1, R* __restrict__ lam0,
                                          R* __restrict__ grad, int with_obj, int64_t id) {
-  if (id >= a.N || a.solid[id]) return;
+  if (id >= a.N) return;
+  const uint32_t lk = a.link[id];
+  if (lk & 1u) return;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
   R g[Q];
@@ -266,8 +298,8 @@ __device__ __forceinline__ void bwd_cell(const AdjArgs<R>& a, const R* __restric
   R mu[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const int64_t nb = i == 0 ? id : neighbour(a, x, y, z, i);
-    mu[i] = nb < 0 ? lam1[int64_t(opp(i)) * a.N + id] : lam1[int64_t(i) * a.N + nb];
+    mu[i] = (i != 0 && ((lk >> i) & 1u)) ? lam1[int64_t(opp(i)) * a.N + id]
+                                          : lam1[int64_t(i) * a.N + (i == 0 ? id : open_neighbour(a, x, y, z, i))];
   }
   R M = R(0), V[3] = {R(0), R(0), R(0)}, Txx = R(0), Tyy = R(0), Tzz = R(0), Txy = R(0), Txz = R(0),
     Tyz = R(0);
@@ -408,6 +440,7 @@ struct flb_adjoint_s {
   size_t real = 4;
   std::vector<uint8_t> h_solid;
   uint8_t* d_solid = nullptr;
+  uint32_t* d_link = nullptr;  // link words (k_links) of d_solid
   void* d_s = nullptr;
   void* d_f0 = nullptr;
   void* d_work[2] = {nullptr, nullptr};
@@ -443,6 +476,7 @@ A::AdjArgs<R> adj_args(const flb_adjoint_s* h) {
   a.pz = h->desc.periodic[2] != 0;
   a.N = h->N;
   a.solid = h->d_solid;
+  a.link = h->d_link;
   a.s = static_cast<const R*>(h->d_s);
   a.omega = R(h->desc.omega);
   for (int d = 0; d < 3; ++d) a.G[d] = R(h->desc.force[d]);
@@ -453,6 +487,13 @@ A::AdjArgs<R> adj_args(const flb_adjoint_s* h) {
 }
 
 unsigned grid_of(const flb_adjoint_s* h) { return unsigned((h->N + A::kThreads - 1) / A::kThreads); }
+
+// Link words of the current solid mask (on the object's stream).
+hipError_t build_links(const flb_adjoint_s* h) {
+  hipLaunchKernelGGL(A::k_links<float>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<float>(h),
+                     h->d_link);
+  return hipGetLastError();
+}
 
 template <class R>
 hipError_t fwd_step(const flb_adjoint_s* h, const void* src, void* dst) {
@@ -888,12 +929,13 @@ int flb_adjoint_create(const flb_adjoint_desc* desc, flb_adjoint* out) {
   if (rc == FLB_OK && (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess))
     rc = fail(FLB_ERR_HIP, "event creation failed");
   if (rc == FLB_OK && (hipMalloc(reinterpret_cast<void**>(&h->d_solid), size_t(N)) != hipSuccess ||
+                       hipMalloc(reinterpret_cast<void**>(&h->d_link), size_t(N) * sizeof(uint32_t)) != hipSuccess ||
                        hipMalloc(&h->d_s, size_t(N) * h->real) != hipSuccess))
     rc = fail(FLB_ERR_NOMEM, "adjoint field allocation failed");
   if (rc == FLB_OK) rc = alloc_state(h, &h->d_f0);
   if (rc == FLB_OK && (hipMemsetAsync(h->d_solid, 0, size_t(N), h->stream) != hipSuccess ||
                        hipMemsetAsync(h->d_s, 0, size_t(N) * h->real, h->stream) != hipSuccess ||
-                       hipStreamSynchronize(h->stream) != hipSuccess))
+                       build_links(h) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess))
     rc = fail(FLB_ERR_HIP, "initial clear failed");
   if (rc != FLB_OK) {
     flb_adjoint_destroy(h);
@@ -909,7 +951,7 @@ int flb_adjoint_destroy(flb_adjoint h) {
   (void)hipStreamSynchronize(h->stream);
   free_vec(h->ckpt, h->d_f0);
   free_vec(h->tape, h->d_f0);
-  void* ptrs[] = {h->d_solid, h->d_s, h->d_f0, h->d_work[0], h->d_work[1], h->d_lam[0], h->d_lam[1],
+  void* ptrs[] = {h->d_solid, h->d_link, h->d_s, h->d_f0, h->d_work[0], h->d_work[1], h->d_lam[0], h->d_lam[1],
                   h->d_grad, h->d_jt, h->d_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -927,6 +969,7 @@ int flb_adjoint_set_solid(flb_adjoint h, const uint8_t* solid) {
   for (int64_t i = 0; i < h->N; ++i) m[size_t(i)] = solid[i] ? 1 : 0;
   h->h_solid = m;
   HIP_TRY(hipMemcpyAsync(h->d_solid, m.data(), size_t(h->N), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(build_links(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->recorded = false;
   return FLB_OK;
