@@ -198,7 +198,7 @@ __device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restric
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
                                                   R* __restrict__ dst) {
-  fwd_cell(a, src, dst, int64_t(blockIdx.x) * kThreads + threadIdx.x);
+  fwd_cell(a, src, dst, int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x);
 }
 
 // ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
@@ -355,7 +355,7 @@ template <class R>
 __global__ __launch_bounds__(kThreads) void k_bwd(AdjArgs<R> a, const R* __restrict__ fst,
                                                   const R* __restrict__ lam1, R* __restrict__ lam0,
                                                   R* __restrict__ grad, int with_obj) {
-  bwd_cell(a, fst, lam1, lam0, grad, with_obj, int64_t(blockIdx.x) * kThreads + threadIdx.x);
+  bwd_cell(a, fst, lam1, lam0, grad, with_obj, int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x);
 }
 
 // ---- many boxes in one launch ------------------------------------------------------
@@ -387,7 +387,7 @@ __device__ __forceinline__ int entry_of(const Entry<R>* tab, int n, int64_t k) {
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_fwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
                                                     int src, int dst) {
-  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t k = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x;
   if (k >= total) return;
   const Entry<R>& E = tab[entry_of(tab, n, k)];
   fwd_cell(E.a, E.slots[src], E.slots[dst], k - E.cell0);
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(kThreads) void k_seed_b(const Entry<R>* __restrict_
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_bwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
                                                     int fst, int lam1, int lam0, int with_obj) {
-  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t k = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x;
   if (k >= total) return;
   const Entry<R>& E = tab[entry_of(tab, n, k)];
   bwd_cell(E.a, E.slots[fst], E.slots[lam1], E.slots[lam0], E.grad, with_obj, k - E.cell0);

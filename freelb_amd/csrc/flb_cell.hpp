@@ -14,6 +14,15 @@
 
 namespace flb {
 
+// Logical workgroup index: the hardware hands consecutive workgroups to the
+// 8 XCDs in turn, so workgroup b runs on XCD b % 8 as its (b / 8)-th one.
+// Mapping it to xcd * (nb / 8) + b / 8 gives every XCD (and its L2) a
+// contiguous range of the grid's work.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb >> 3, rem = nb & 7u, xcd = b & 7u, slot = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
 // ---------------------------------------------------------------------------
 // Velocity sets in FreeLB's order (lbm/lattice_set.h:168-194 D2Q9,
 // 259-296 D3Q19). Opposite pairs are (2k-1, 2k).

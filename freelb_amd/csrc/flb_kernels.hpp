@@ -66,13 +66,6 @@ enum : int {
 // (flow lattices): a block has one or the other.
 enum : int { HWBC_NORMAL = 0, HWBC_MOVING = 1, HWBC_ANTI = 2 };
 
-// Logical workgroup index: the hardware hands consecutive workgroups to the
-// 8 XCDs in turn, so workgroup b runs on XCD b % 8 as its (b / 8)-th one.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
-  const unsigned per = nb >> 3, rem = nb & 7u, xcd = b & 7u, slot = b >> 3;
-  return xcd * per + (xcd < rem ? xcd : rem) + slot;
-}
-
 template <class S>
 __device__ __forceinline__ typename S::E* arr(const KArgs& a, int i) {
   return static_cast<typename S::E*>(a.pops) + int64_t(i) * a.dpop;
