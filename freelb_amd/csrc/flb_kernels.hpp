@@ -1192,7 +1192,10 @@ bool with_q(int q, F&& f) {
 template <int Q, class S, int PAR>
 hipError_t step_scalar_q(const KArgs& a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(kWave * a.wpb);
-  if (a.px || a.py || a.pz) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, true>), grid, block, 0, s, a);
+  const bool per = a.px || a.py || a.pz;
+  const bool buf = FLB_BUFFER_ADDR && PAR != 0 && a.pop_bytes != 0 && !per;
+  if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, true>), grid, block, 0, s, a);
+  else if (buf) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, false, true>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
