@@ -448,8 +448,9 @@ def main():
                     help="time the batch with launches issued one by one (no hipGraph), e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(each GPU would run its own share of blocks; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here "
+              "(each GPU would run its own share of blocks, or its own slab of one domain "
+              "with a face exchange per step).", file=sys.stderr)
         sys.exit(2)
 
     if args.config4:
