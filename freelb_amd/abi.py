@@ -26,7 +26,7 @@ HWBC_NORMAL, HWBC_MOVING, HWBC_ANTI = 0, 1, 2
 
 # Every symbol include/flb_abi.h declares (checked by the CPU test suite).
 EXPORTS = (
-    "flb_last_error", "flb_abi_version", "flb_block_create", "flb_block_destroy", "flb_get_info",
+    "flb_last_error", "flb_abi_version", "flb_block_layout", "flb_block_create", "flb_block_destroy", "flb_get_info",
     "flb_synchronize", "flb_set_tasks", "flb_set_halfway_bc", "flb_set_mrt_rates", "flb_set_omega",
     "flb_set_force", "flb_share_velocity", "flb_add_buoyancy", "flb_clear_buoyancy",
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
@@ -69,6 +69,12 @@ class BlockInfo(C.Structure):
                 ("d_flags", C.c_void_p)]
 
 
+class Layout(C.Structure):
+    _fields_ = [("pitch", C.c_int), ("xs", C.c_int), ("plane", C.c_int64), ("stride", C.c_int64),
+                ("front_pad", C.c_int64), ("dpop", C.c_int64), ("ypop", C.c_int64), ("zpop", C.c_int64),
+                ("pop_elems", C.c_int64), ("pop_bytes", C.c_int64)]
+
+
 class LaunchCfg(C.Structure):
     _fields_ = [("waves_per_group", C.c_int), ("grid", C.c_int)]
 
@@ -103,6 +109,7 @@ def lib() -> C.CDLL:
     L.flb_last_error.argtypes = []
     L.flb_abi_version.restype = C.c_int
     sig = {
+        "flb_block_layout": [C.POINTER(BlockDesc), C.POINTER(Layout)],
         "flb_block_create": [C.POINTER(BlockDesc), C.POINTER(P)],
         "flb_block_destroy": [P],
         "flb_get_info": [P, C.POINTER(BlockInfo)],

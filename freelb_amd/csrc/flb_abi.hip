@@ -326,6 +326,11 @@ int check_pair(flb_block flow, flb_block scal) {
     return fail(FLB_ERR_ARG, "flow and scalar blocks cover different cells");
   if (flow->real != scal->real) return fail(FLB_ERR_ARG, "flow and scalar blocks compute in different types");
   if (flow->device != scal->device) return fail(FLB_ERR_ARG, "flow and scalar blocks on different devices");
+  // The scalar step reads the flow block's VELOCITY field while the flow step
+  // writes it, and the flow step reads the scalar's RHO field: only one
+  // stream orders the two blocks' launches.
+  if (flow->stream != scal->stream)
+    return fail(FLB_ERR_ARG, "coupled flow and scalar blocks must share one stream");
   return FLB_OK;
 }
 
@@ -425,9 +430,9 @@ const char* flb_last_error(void) { return g_err.c_str(); }
 
 int flb_abi_version(void) { return FLB_ABI_VERSION; }
 
-int flb_block_create(const flb_block_desc* desc, flb_block* out) {
+int flb_block_layout(const flb_block_desc* desc, flb_layout* out) {
   if (!desc || !out) return fail(FLB_ERR_ARG, "null descriptor or output");
-  *out = nullptr;
+  *out = flb_layout{};
   if (desc->q != FLB_D2Q9 && desc->q != FLB_D3Q19 && desc->q != FLB_D2Q5 && desc->q != FLB_D3Q7)
     return fail(FLB_ERR_ARG, "q must be 9 (D2Q9), 19 (D3Q19), 5 (D2Q5) or 7 (D3Q7), got %d", desc->q);
   if (!storage_ok(desc->storage)) return fail(FLB_ERR_ARG, "unknown storage kind %d", desc->storage);
@@ -445,20 +450,26 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
     const int need = desc->periodic[ax] ? 1 : 2 * desc->overlap + 1;
     if (n[ax] < need) return fail(FLB_ERR_ARG, "axis %d extent %d leaves no interior", ax, n[ax]);
   }
-  const int64_t N = int64_t(desc->nx) * desc->ny * desc->nz;
-  // row pitch: the updated cells of a row rounded up to whole 1024-element
-  // groups (the widest vector row a workgroup covers), plus one cache line of
-  // padding that takes the odd step's whole-line stores past the row ends
+  // Slot of cell (x, y, z): (x - xs) + pitch y + plane z, xs the first updated
+  // x. Cells x < xs (the left overlap) take the last slots of the previous
+  // row, so a row of nx cells needs pitch >= nx for the slots to stay
+  // distinct.
+  const int xs = desc->periodic[0] ? 0 : desc->overlap;
   const int ex_x = desc->periodic[0] ? desc->nx : desc->nx - 2 * desc->overlap;
-  // Short rows (at most 222 updated cells) take a compact pitch instead:
-  // room for the two overlap cells, rounded up to FLB_PITCH_ROUND slots
-  // (default 1: no rounding), so that neighbouring rows share cache lines
-  // rather than each filling a mostly empty 320-slot row. On 512 cavities of
-  // 16..96 cells per edge, 1 slot runs 3 % faster than 8 and 22 % faster
-  // than 16.
+  // Short rows (at most 222 updated cells) take a compact pitch: the row's
+  // own cells (the overlap cells included; two spare slots on a periodic
+  // axis), rounded up to FLB_PITCH_ROUND slots (default 1: no rounding), so
+  // that neighbouring rows share cache lines rather than each filling a
+  // mostly empty 320-slot row. On 512 cavities of 16..96 cells per edge, 1
+  // slot runs 3 % faster than 8 and 22 % faster than 16.
+  // Longer rows: the updated cells rounded up to whole 256-slot groups plus
+  // one cache line of padding that takes the odd step's whole-line stores
+  // past the row ends, so every row's first updated cell sits on a 256-byte
+  // boundary.
   int round = 1;
   if (const char* e = std::getenv("FLB_PITCH_ROUND")) round = std::max(1, std::min(256, std::atoi(e)));
-  const int pitch = ex_x + 2 <= 224 ? (ex_x + 2 + round - 1) / round * round
+  const int compact = desc->periodic[0] ? ex_x + 2 : desc->nx;
+  const int pitch = ex_x + 2 <= 224 ? (compact + round - 1) / round * round
                                     : std::max((ex_x + 255) / 256 * 256 + 64, (desc->nx + 63) / 64 * 64);
   const int64_t plane = int64_t(pitch) * desc->ny;
   // tail: one extra row plus the widest vector run past a row end
@@ -466,38 +477,57 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
   const int64_t stride = (flb::kFrontPad + plane * desc->nz + tail + 255) / 256 * 256;
   if (stride >= (int64_t(1) << 31))
     return fail(FLB_ERR_ARG, "block of %lld padded cells exceeds 2^31", (long long)stride);
-  if (!(desc->omega > 0.0 && desc->omega < 2.0))
-    return fail(FLB_ERR_ARG, "omega must lie in (0, 2), got %g", desc->omega);
-
-  flb_block b = new flb_block_s();
-  b->desc = *desc;
-  b->d = d;
-  if (d == 2) b->desc.periodic[2] = 0;
-  b->N = N;
-  b->stride = stride;
-  b->pitch = pitch;
-  b->plane = plane;
+  out->pitch = pitch;
+  out->xs = xs;
+  out->plane = plane;
+  out->stride = stride;
+  out->front_pad = flb::kFrontPad;
   // population layout: directions interleaved per row (default), per plane,
   // or one array per direction
   const char* lay = std::getenv("FLB_POP_LAYOUT");
   const std::string layout = lay ? lay : "row";
   const int64_t q = desc->q;
   if (layout == "array") {
-    b->dpop = stride;
-    b->ypop = pitch;
-    b->zpop = plane;
-    b->pop_elems = q * stride;
+    out->dpop = stride;
+    out->ypop = pitch;
+    out->zpop = plane;
+    out->pop_elems = q * stride;
   } else if (layout == "plane") {
-    b->dpop = plane;
-    b->ypop = pitch;
-    b->zpop = q * plane;
-    b->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
+    out->dpop = plane;
+    out->ypop = pitch;
+    out->zpop = q * plane;
+    out->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
   } else {
-    b->dpop = pitch;
-    b->ypop = int(q * pitch);
-    b->zpop = q * plane;
-    b->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
+    out->dpop = pitch;
+    out->ypop = q * pitch;
+    out->zpop = q * plane;
+    out->pop_elems = (flb::kFrontPad + q * plane * desc->nz + tail + 255) / 256 * 256;
   }
+  const int64_t elem = desc->storage == FLB_STORE_F64 ? 8 : (desc->storage == FLB_STORE_F32 ? 4 : 2);
+  out->pop_bytes = out->pop_elems * elem;
+  return FLB_OK;
+}
+
+int flb_block_create(const flb_block_desc* desc, flb_block* out) {
+  if (!desc || !out) return fail(FLB_ERR_ARG, "null descriptor or output");
+  *out = nullptr;
+  flb_layout lay{};
+  if (int rc = flb_block_layout(desc, &lay)) return rc;
+  if (!(desc->omega > 0.0 && desc->omega < 2.0))
+    return fail(FLB_ERR_ARG, "omega must lie in (0, 2), got %g", desc->omega);
+
+  flb_block b = new flb_block_s();
+  b->desc = *desc;
+  b->d = (desc->q == FLB_D3Q19 || desc->q == FLB_D3Q7) ? 3 : 2;
+  if (b->d == 2) b->desc.periodic[2] = 0;
+  b->N = int64_t(desc->nx) * desc->ny * desc->nz;
+  b->stride = lay.stride;
+  b->pitch = lay.pitch;
+  b->plane = lay.plane;
+  b->dpop = lay.dpop;
+  b->ypop = int(lay.ypop);
+  b->zpop = lay.zpop;
+  b->pop_elems = lay.pop_elems;
   b->elem = desc->storage == FLB_STORE_F64 ? 8 : (desc->storage == FLB_STORE_F32 ? 4 : 2);
   // one wavefront per workgroup for the 16-bit kinds (measured ~1 % faster on
   // the 256^3 cavity; FP32 / FP64 do not care)
@@ -529,7 +559,7 @@ int flb_block_create(const flb_block_desc* desc, flb_block* out) {
     flb_block_destroy(b);
     return fail(FLB_ERR_NOMEM, "flag allocation failed");
   }
-  b->h_flags.assign(size_t(N), 0);
+  b->h_flags.assign(size_t(b->N), 0);
   if (hipMemsetAsync(b->d_flags, 0, size_t(b->stride), b->stream) != hipSuccess ||
       hipMemsetAsync(b->d_pops, 0, pop_bytes, b->stream) != hipSuccess) {
     flb_block_destroy(b);

@@ -307,10 +307,16 @@ int flb_batch_capture(flb_batch t, int n) {
   if (rc) return rc;
   if ((rc = refresh(t, false))) return rc;
   drop_graph(t);
+  // the halo's tables are refreshed (host copies and a stream sync) before
+  // the capture opens, as flb_batch_step does before its launches
+  if (t->halo && (rc = halo_refresh(t->halo))) return rc;
   hipStream_t cs = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  if (t->halo && (rc = halo_refresh(t->halo))) return rc;
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return fail(FLB_ERR_HIP, "graph capture failed to begin: %s", hipGetErrorString(e));
+  }
   for (int s = 0; e == hipSuccess && s < n; ++s) {
     e = launch_batch(t, (parity + s) & 1, cs);
     if (e == hipSuccess && t->halo) e = halo_launch(t->halo, (parity + s + 1) & 1, cs);
@@ -371,13 +377,20 @@ int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* bes
   if ((rc = refresh(t, false))) return rc;
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device));
-  // snapshot of every block's populations, restored after the trials
-  std::vector<size_t> bytes(t->blocks.size());
-  size_t total = 0;
-  for (size_t k = 0; k < t->blocks.size(); ++k) {
-    bytes[k] = size_t(t->blocks[k]->pop_elems) * t->blocks[k]->elem;
-    total += bytes[k];
+  // snapshot of every block's populations and of the RHO / VELOCITY fields
+  // that field-writing dynamics update each step, restored after the trials
+  struct Region {
+    void* p;
+    size_t bytes;
+  };
+  std::vector<Region> regions;
+  for (flb_block_s* b : t->blocks) {
+    regions.push_back({b->d_pops, size_t(b->pop_elems) * b->elem});
+    if (b->d_rho) regions.push_back({b->d_rho, size_t(b->stride) * b->real});
+    if (b->d_u) regions.push_back({b->d_u, size_t(b->stride) * b->real * size_t(b->d)});
   }
+  size_t total = 0;
+  for (const Region& r : regions) total += r.bytes;
   char* snap = nullptr;
   if (hipMalloc(reinterpret_cast<void**>(&snap), total) != hipSuccess)
     return fail(FLB_ERR_NOMEM, "autotune snapshot of %zu bytes failed", total);
@@ -386,10 +399,10 @@ int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* bes
     return rc;
   }
   size_t off = 0;
-  for (size_t k = 0; k < t->blocks.size(); ++k) {
-    if (hipMemcpyAsync(snap + off, t->blocks[k]->d_pops, bytes[k], hipMemcpyDeviceToDevice, t->stream) != hipSuccess)
+  for (const Region& r : regions) {
+    if (hipMemcpyAsync(snap + off, r.p, r.bytes, hipMemcpyDeviceToDevice, t->stream) != hipSuccess)
       rc = fail(FLB_ERR_HIP, "autotune snapshot copy failed");
-    off += bytes[k];
+    off += r.bytes;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rc == FLB_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
@@ -426,11 +439,10 @@ int flb_batch_autotune(flb_batch t, int steps, flb_launch_cfg* best, double* bes
     }
   }
   off = 0;
-  for (size_t k = 0; k < t->blocks.size(); ++k) {
-    if (hipMemcpyAsync(t->blocks[k]->d_pops, snap + off, bytes[k], hipMemcpyDeviceToDevice, t->stream) != hipSuccess &&
-        rc == FLB_OK)
+  for (const Region& r : regions) {
+    if (hipMemcpyAsync(r.p, snap + off, r.bytes, hipMemcpyDeviceToDevice, t->stream) != hipSuccess && rc == FLB_OK)
       rc = fail(FLB_ERR_HIP, "autotune restore copy failed");
-    off += bytes[k];
+    off += r.bytes;
   }
   if (hipStreamSynchronize(t->stream) != hipSuccess && rc == FLB_OK) rc = fail(FLB_ERR_HIP, "autotune sync failed");
   (void)hipFree(snap);

@@ -162,9 +162,27 @@ typedef struct {
   const void* d_flags;    /* device flag field */
 } flb_block_info;
 
+/* Device layout of a block (HBM slots). Per-cell arrays (flags, each field
+ * component) put cell (x, y, z) at slot (x - xs) + pitch*y + plane*z of an
+ * array of `stride` slots that starts front_pad slots in; populations put
+ * direction i at (x - xs) + ypop*y + zpop*z + dpop*i. Distinct cells get
+ * distinct slots for every overlap width. */
+typedef struct {
+  int pitch, xs;
+  int64_t plane, stride, front_pad;
+  int64_t dpop, ypop, zpop;
+  int64_t pop_elems;      /* population elements allocated */
+  int64_t pop_bytes;
+} flb_layout;
+
 /* ---- lifecycle ---------------------------------------------------------- */
 const char* flb_last_error(void);
 int flb_abi_version(void);
+/* The layout flb_block_create would give `desc`, checked the same way; host
+ * only (no device call), e.g. to size device memory before creating blocks.
+ * BlockField's storage of the reference is one plain array per field
+ * (data_struct/field.h:1419-1575), so it has no counterpart to this query. */
+int flb_block_layout(const flb_block_desc* desc, flb_layout* out);
 int flb_block_create(const flb_block_desc* desc, flb_block* out);
 int flb_block_destroy(flb_block b);
 int flb_get_info(flb_block b, flb_block_info* info);

@@ -311,7 +311,7 @@ def weights(q, dtype=np.float32):
     return np.array([dtype(4) / dtype(9)] + [one / dtype(9)] * 4 + [one / dtype(36)] * 4, dtype=dtype)
 
 
-def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float32, cs=0.17):
+def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float32, cs=0.17, rates=None):
     """The reference cavity driver as an oracle Lattice plus its initial pops.
 
     mode 'fw'      : benchmarks/cavity3d/cavity3d.cpp:170-182 (wall cells swap).
@@ -321,6 +321,9 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
     mode 'smag'    : fullway walls, fluid SmagorinskyBGK with C_s = cs.
     mode 'force'   : like 'hw' with BGKForce<forcerhoU, SecondOrder, ConstForce> and the
                      constant force density FORCE (float values).
+    mode 'mrt_hw'  : every non-void cell MRT; halfway normal / moving wall.
+    mode 'smag_mrt', 'smag_mrt_hw' : as 'mrt', 'mrt_hw' with MRT + Smagorinsky (C_s = cs).
+    rates          : MRT rates (NaN: the shear rate), None for the defaults.
     """
     q = 19 if Nk > 0 else 9
     flags = cavity_flags(Ni, Nj, Nk)
@@ -350,6 +353,16 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
     elif mode == "force":
         tasks = [(AABB | BB | BBMW, DYN_BGK_FORCE)]
         bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+    elif mode == "mrt_hw":
+        tasks = [(AABB | BB | BBMW, DYN_MRT)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+    elif mode == "smag_mrt":
+        tasks = [(AABB, DYN_SMAG_MRT, float(np.float32(cs))), (BB, DYN_BOUNCEBACK),
+                 (BBMW, DYN_BOUNCEBACK_MW)]
+        bcs = []
+    elif mode == "smag_mrt_hw":
+        tasks = [(AABB | BB | BBMW, DYN_SMAG_MRT, float(np.float32(cs)))]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
     else:
         raise ValueError(mode)
     if dtype == np.float64:
@@ -360,7 +373,7 @@ def cavity_case(mode, Ni, Nj, Nk, RT=0.53, visc=0.01, u_phys=0.1, dtype=np.float
         om = float(omega)
     force = [float(v) for v in FORCE[:d]] if mode == "force" else (0.0, 0.0, 0.0)
     lat = Lattice(q, dims, flags, tasks, bcs, omega=om, rho_field=rho_field, u_field=u_field,
-                  force=force)
+                  force=force, mrt_rates=rates)
     w = weights(q, dtype)
     pops = np.repeat(w[:, None], N, axis=1).astype(dtype)
     pops = np.ascontiguousarray(pops)

@@ -8,7 +8,8 @@ from freelb_amd import BlockLattice, HalfwayBB, Key_TypePair
 
 def gpu_lattice(case, storage="f32", pops=None):
     """BlockLattice on the GPU with the same flags, tasks, BCs and fields as `case`."""
-    lat = BlockLattice(case.q, (case.nx, case.ny, case.nz), omega=case.omega, storage=storage)
+    lat = BlockLattice(case.q, (case.nx, case.ny, case.nz), omega=case.omega, storage=storage,
+                       overlap=int(case.L.overlap))
     lat.upload_flags(case.flags)
     lat.set_tasks([Key_TypePair(t[0], t[1], float(t[2]) if len(t) > 2 else 0.0) for t in case.tasks])
     lat.set_halfway_bc([HalfwayBB(b[0], b[1], b[2] == 1) for b in case.bcs])

@@ -138,3 +138,41 @@ def test_output_null_arguments():
     r = C.c_double()
     assert L.flb_tolerance(None, 1, 0, C.byref(r)) == abi.FLB_ERR_ARG
     assert L.flb_enable_tolerance(None) == abi.FLB_ERR_ARG
+
+
+def _layout(**kw):
+    lay = abi.Layout()
+    abi.check(abi.lib().flb_block_layout(C.byref(_desc(**kw)), C.byref(lay)))
+    return lay
+
+
+@pytest.mark.parametrize("overlap", [1, 2, 3])
+@pytest.mark.parametrize("nx", [7, 12, 40, 96, 200, 224, 225, 226, 230, 300, 520])
+@pytest.mark.parametrize("periodic_x", [0, 1])
+def test_slots_are_distinct_for_every_overlap(overlap, nx, periodic_x):
+    """Every host cell gets its own flag / field slot and its own population
+    slot per direction, for compact (short) and wide rows and overlap
+    widths 1-3 (FreeLB's refined blocks use 2); slots stay inside the
+    allocation with the front pad before them."""
+    per = (C.c_int * 3)(periodic_x, 0, 0)
+    ny, nz = 2 * overlap + 3, 2 * overlap + 2
+    lay = _layout(nx=nx, ny=ny, nz=nz, overlap=overlap, periodic=per)
+    x = np.arange(nx)[None, None, :]
+    y = np.arange(ny)[None, :, None]
+    z = np.arange(nz)[:, None, None]
+    assert lay.xs == (0 if periodic_x else overlap)
+    slots = ((x - lay.xs) + lay.pitch * y + lay.plane * z).reshape(-1)
+    assert np.unique(slots).size == slots.size
+    assert slots.min() + lay.front_pad >= 0 and slots.max() + lay.front_pad < lay.stride
+    pops = ((x - lay.xs) + lay.ypop * y + lay.zpop * z).reshape(1, -1) + lay.dpop * np.arange(19)[:, None]
+    assert np.unique(pops).size == pops.size
+    assert pops.min() + lay.front_pad >= 0 and pops.max() + lay.front_pad < lay.pop_elems
+    # the odd step reads and writes one cell past an updated cell: still inside
+    assert pops.max() + 1 + lay.front_pad < lay.pop_elems
+
+
+def test_layout_rejects_like_create():
+    L = abi.lib()
+    lay = abi.Layout()
+    assert L.flb_block_layout(C.byref(_desc(overlap=0)), C.byref(lay)) == abi.FLB_ERR_ARG
+    assert "overlap" in L.flb_last_error().decode()

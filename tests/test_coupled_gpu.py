@@ -233,3 +233,26 @@ def _outer(dims):
     nx, ny, nz = dims
     z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     return ((x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1)).reshape(-1)
+
+
+def test_coupling_rejects_blocks_on_different_streams():
+    """The scalar step reads the flow's VELOCITY field and the flow step the
+    scalar's RHO field; on two streams nothing would order them."""
+    import torch
+
+    s1 = torch.cuda.Stream()
+    s2 = torch.cuda.Stream()
+    dims = (8, 7, 6)
+    flow = BlockLattice(19, dims, omega=1.5, stream=s1.cuda_stream)
+    temp = BlockLattice(7, dims, omega=1.2, stream=s2.cuda_stream)
+    same = BlockLattice(7, dims, omega=1.2, stream=s1.cuda_stream)
+    try:
+        with pytest.raises(FLBError, match="share one stream"):
+            temp.share_velocity(flow)
+        with pytest.raises(FLBError, match="share one stream"):
+            BlockLatManagerCoupling(flow, temp).add_buoyancy(ALL, 1e-3, 1.0)
+        same.share_velocity(flow)
+    finally:
+        for b in (same, temp, flow):
+            b.close()
+        torch.cuda.synchronize()

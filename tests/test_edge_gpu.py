@@ -61,3 +61,43 @@ def test_zero_steps_leave_state(storage):
         assert np.array_equal(lat.pops(), before)
     finally:
         lat.close()
+
+
+def _padded_cavity(mode, dims, pad, dtype):
+    """The cavity of cavity_case wrapped in `pad` more void layers, as a block
+    of overlap 1 + pad (FreeLB's refined blocks carry overlap 2)."""
+    case, p0 = O.cavity_case(mode, *dims, dtype=dtype)
+    d = 3 if case.q == 19 else 2
+    shp = (case.nz, case.ny, case.nx)
+    padw = [(pad, pad) if (a > 0 or d == 3) else (0, 0) for a in range(3)]
+    flags = np.pad(case.flags.reshape(shp), padw, constant_values=O.VOID)
+    nz, ny, nx = flags.shape
+    N = flags.size
+    rho = np.pad(case.rho_field.reshape(shp), padw, constant_values=1.0).reshape(-1)
+    u = np.stack([np.pad(c.reshape(shp), padw) for c in case.u_field.reshape(d, -1)]).reshape(d, N)
+    lat = O.Lattice(case.q, (nx, ny, nz), flags.reshape(-1), case.tasks, case.bcs, omega=case.omega,
+                    rho_field=rho, u_field=u, overlap=1 + pad)
+    w = O.weights(case.q, dtype)
+    pops = np.ascontiguousarray(np.repeat(w[:, None], N, axis=1).astype(dtype))
+    return lat, pops
+
+
+@pytest.mark.parametrize("pad", [1, 2])
+@pytest.mark.parametrize("mode,dims", [("hw", (10, 7, 6)), ("hw", (150, 4, 3)), ("fw", (9, 6, 5)),
+                                       ("hw", (14, 9, 0))])
+def test_overlap_wider_than_one(pad, mode, dims):
+    """Blocks whose overlap layer is 2 or 3 cells wide: every cell keeps its
+    own slot (the compact row pitch covers the whole row), bit for bit
+    against the oracle."""
+    case, p0 = _padded_cavity(mode, dims, pad, np.float32)
+    print(f"[case] overlap={1 + pad} mode={mode} dims={dims}", flush=True)
+    lat = gpu_lattice(case, pops=p0)
+    try:
+        lat.step(13)
+        g = lat.pops()
+    finally:
+        lat.close()
+    ref = case.run(p0.copy(), 13)
+    sel = case.flags == O.AABB if mode == "fw" else (case.flags != O.VOID) & interior_mask(case, 1 + pad)
+    assert sel.sum() == np.prod([v for v in dims if v > 0])
+    assert np.array_equal(g[:, sel], ref[:, sel]), f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}"
