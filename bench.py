@@ -150,6 +150,18 @@ def precision_study(n=N_EDGE, steps=2000, collisions=("bgk", "mrt_les"),
     return out
 
 
+def batch_specs(nblocks=512, seed=2025):
+    """The batched-cavity workload: per block, edges drawn per axis from
+    U{16..96}, tau from U[0.51, 0.9] and the lid speed from U[0.02, 0.1]
+    (numpy default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(16, 97, size=(nblocks, 3))
+    taus = rng.uniform(0.51, 0.9, size=nblocks)
+    lids = rng.uniform(0.02, 0.1, size=nblocks)
+    return [dict(shape=tuple(int(v) for v in edges[k]), tau=float(taus[k]), u_lid=float(lids[k]))
+            for k in range(nblocks)]
+
+
 def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", loop_steps=20, eager=False):
     """512 independent D3Q19 BGK halfway cavities, edges drawn per axis from
     U{16..96}, tau from U[0.51, 0.9], lid speed from U[0.02, 0.1] (numpy
@@ -161,15 +173,11 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
     profilers that do not follow graph launches)."""
     from freelb_amd import BlockLatticeManager
     from freelb_amd.cases import cavity_block
-    rng = np.random.default_rng(seed)
-    edges = rng.integers(16, 97, size=(nblocks, 3))
-    taus = rng.uniform(0.51, 0.9, size=nblocks)
-    lids = rng.uniform(0.02, 0.1, size=nblocks)
+    specs = batch_specs(nblocks, seed)
     t0 = time.perf_counter()
-    blocks = [cavity_block(tuple(int(v) for v in edges[k]), storage=storage, tau=float(taus[k]),
-                           u_lid=float(lids[k])) for k in range(nblocks)]
+    blocks = [cavity_block(sp["shape"], storage=storage, tau=sp["tau"], u_lid=sp["u_lid"]) for sp in specs]
     setup_s = time.perf_counter() - t0
-    cells = int(np.prod(edges, axis=1).sum())
+    cells = int(sum(np.prod(sp["shape"]) for sp in specs))
     try:
         # one launch per block and step
         for b in blocks:

@@ -9,7 +9,6 @@ FreeLB's BlockLattice interface.
 from .abi import FLBError  # noqa: F401
 from .adjoint import Adjoint, AdjointBatch  # noqa: F401
 from .lattice import (BlockHalo, BlockLatManagerCoupling, BlockLattice, BlockLatticeManager, HalfwayBB,  # noqa: F401
-                      Key_TypePair,
-                      Refinement, weights, write_vtm)
+                      Key_TypePair, Refinement, RefinementManager, weights, write_vtm)
 
-__all__ = ["Adjoint", "AdjointBatch", "BlockHalo", "BlockLatManagerCoupling", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "Refinement", "weights", "write_vtm", "FLBError"]
+__all__ = ["Adjoint", "AdjointBatch", "BlockHalo", "BlockLatManagerCoupling", "BlockLattice", "BlockLatticeManager", "HalfwayBB", "Key_TypePair", "Refinement", "RefinementManager", "weights", "write_vtm", "FLBError"]

@@ -31,17 +31,19 @@ EXPORTS = (
     "flb_set_force", "flb_share_velocity", "flb_add_buoyancy", "flb_clear_buoyancy",
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
     "flb_upload_pops_f64", "flb_download_pops_f32", "flb_download_pops_f64", "flb_init_equilibrium",
-    "flb_apply_cell_dynamics", "flb_stream", "flb_apply_halfway_bc", "flb_step", "flb_moments",
+    "flb_apply_cell_dynamics", "flb_stream", "flb_set_level", "flb_apply_halfway_bc", "flb_step", "flb_moments",
     "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
     "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
     "flb_batch_synchronize", "flb_batch_autotune", "flb_batch_set_halo", "flb_set_ghost_overlap",
     "flb_halo_create", "flb_halo_destroy", "flb_halo_info", "flb_halo_exchange",
-    "flb_refine_create", "flb_refine_destroy", "flb_refine_info", "flb_refine_communicate", "flb_refine_run",
+    "flb_refine_create", "flb_refine_destroy", "flb_refine_info", "flb_refine_set_levels", "flb_refine_average",
+    "flb_refine_interpolate", "flb_refine_communicate", "flb_refine_run",
     "flb_download_fields", "flb_enable_tolerance", "flb_tolerance", "flb_store_moments", "flb_write_vtm", "flb_adjoint_create", "flb_adjoint_destroy",
     "flb_adjoint_set_solid", "flb_adjoint_set_fraction", "flb_adjoint_set_fraction_at",
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
     "flb_adjoint_timing", "flb_adjoint_batch_create", "flb_adjoint_batch_destroy", "flb_adjoint_batch_forward",
     "flb_adjoint_batch_backward", "flb_adjoint_batch_timing", "flb_bandwidth_probe",
+    "flb_bandwidth_probe_ex",
 )
 
 
@@ -130,8 +132,9 @@ def lib() -> C.CDLL:
         "flb_download_pops_f32": [P, C.POINTER(C.c_float)],
         "flb_download_pops_f64": [P, C.POINTER(C.c_double)],
         "flb_init_equilibrium": [P, C.POINTER(C.c_double)],
-        "flb_apply_cell_dynamics": [P],
-        "flb_stream": [P],
+        "flb_apply_cell_dynamics": [P, C.c_int64],
+        "flb_stream": [P, C.c_int64],
+        "flb_set_level": [P, C.c_int, C.c_int],
         "flb_apply_halfway_bc": [P],
         "flb_step": [P, C.c_int],
         "flb_moments": [P, C.c_uint8, P],
@@ -155,6 +158,9 @@ def lib() -> C.CDLL:
         "flb_refine_create": [P, P, C.POINTER(C.c_int), P, C.POINTER(P)],
         "flb_refine_destroy": [P],
         "flb_refine_info": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "flb_refine_set_levels": [P, C.c_int, C.c_int],
+        "flb_refine_average": [P, C.c_int64],
+        "flb_refine_interpolate": [P, C.c_int64],
         "flb_refine_communicate": [P, C.c_int64],
         "flb_refine_run": [P, C.c_int64, C.c_int],
         "flb_download_fields": [P, C.POINTER(C.c_double)],
@@ -180,6 +186,8 @@ def lib() -> C.CDLL:
         "flb_adjoint_batch_backward": [P, C.POINTER(C.c_double)],
         "flb_adjoint_batch_timing": [P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "flb_bandwidth_probe": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
+        "flb_bandwidth_probe_ex": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_int,
+                                   C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

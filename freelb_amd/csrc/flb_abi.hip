@@ -958,9 +958,21 @@ static int launch_one(flb_block b) {
   return FLB_OK;
 }
 
-int flb_apply_cell_dynamics(flb_block b) {
+int flb_set_level(flb_block b, int level, int max_level) {
   CHECK_BLOCK(b);
+  if (level < 0 || max_level < level || max_level > 30)
+    return fail(FLB_ERR_ARG, "level %d / max level %d", level, max_level);
+  if (b->pending) return fail(FLB_ERR_STATE, "level change between ApplyCellDynamics and Stream");
+  b->level = level;
+  b->max_level = max_level;
+  return FLB_OK;
+}
+
+int flb_apply_cell_dynamics(flb_block b, int64_t count) {
+  CHECK_BLOCK(b);
+  if (count < 0) return fail(FLB_ERR_ARG, "negative count");
   if (b->pending) return fail(FLB_ERR_STATE, "ApplyCellDynamics called twice without Stream");
+  if (!level_due(count, b->level, b->max_level)) return FLB_OK;
   DeviceGuard g(b->device);
   const int rc = launch_one(b);
   if (rc) return rc;
@@ -968,8 +980,13 @@ int flb_apply_cell_dynamics(flb_block b) {
   return FLB_OK;
 }
 
-int flb_stream(flb_block b) {
+int flb_stream(flb_block b, int64_t count) {
   CHECK_BLOCK(b);
+  if (count < 0) return fail(FLB_ERR_ARG, "negative count");
+  if (!level_due(count, b->level, b->max_level)) {
+    if (b->pending) return fail(FLB_ERR_STATE, "Stream skipped at this count after an ApplyCellDynamics");
+    return FLB_OK;
+  }
   if (!b->pending) return fail(FLB_ERR_STATE, "Stream without a preceding ApplyCellDynamics");
   b->pending = 0;
   b->parity ^= 1;
@@ -1031,33 +1048,51 @@ int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]) {
   return FLB_OK;
 }
 
-int flb_bandwidth_probe(size_t bytes, int reps, double* gbs) {
-  if (!gbs || reps < 1 || bytes < 16) return fail(FLB_ERR_ARG, "bad probe arguments");
-  const size_t n4 = bytes / 16;
+}  // extern "C"
+
+namespace {
+// Times `reps` launches of one probe form over two fresh buffers of `bytes`
+// (rounded down to whole tiles); *gbs gets the bytes moved per launch (read +
+// written) over the mean launch time.
+int run_probe(int mode, int unroll, bool nt, int wg_per_cu, bool legacy, size_t bytes, int reps, double* gbs) {
+  const size_t tile = size_t(unroll) * flb::kMaxThreads;
+  const size_t n4 = legacy ? bytes / 16 : (bytes / 16) / tile * tile;
+  if (n4 == 0) return fail(FLB_ERR_ARG, "probe buffer smaller than one tile");
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    return fail(FLB_ERR_HIP, "device query failed");
+  const size_t ntile = n4 / tile;
+  const size_t grid = wg_per_cu > 0 ? std::min(ntile, size_t(wg_per_cu) * size_t(cus)) : ntile;
+  if (grid > size_t(0x7fffffff)) return fail(FLB_ERR_ARG, "probe grid too large");
   void *src = nullptr, *dst = nullptr;
   if (hipMalloc(&src, n4 * 16) != hipSuccess) return fail(FLB_ERR_NOMEM, "probe allocation failed");
   if (hipMalloc(&dst, n4 * 16) != hipSuccess) {
     (void)hipFree(src);
     return fail(FLB_ERR_NOMEM, "probe allocation failed");
   }
+  auto launch = [&]() {
+    return legacy ? flb::launch_copy_probe(dst, src, n4, nullptr)
+                  : flb::launch_bw_probe(mode, unroll, nt, unsigned(grid), dst, src, n4, nullptr);
+  };
   hipEvent_t e0, e1;
   int rc = FLB_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = fail(FLB_ERR_HIP, "event creation failed");
-  } else if (hipMemset(src, 0, n4 * 16) != hipSuccess ||
-             flb::launch_copy_probe(dst, src, n4, nullptr) != hipSuccess ||
-             hipDeviceSynchronize() != hipSuccess) {
+  } else if (hipMemset(src, 0, n4 * 16) != hipSuccess || hipMemset(dst, 0, n4 * 16) != hipSuccess ||
+             launch() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
     rc = fail(FLB_ERR_HIP, "probe warm-up failed");
   } else {
     (void)hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; ++r) (void)flb::launch_copy_probe(dst, src, n4, nullptr);
+    for (int r = 0; r < reps; ++r) (void)launch();
     (void)hipEventRecord(e1, nullptr);
     (void)hipEventSynchronize(e1);
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
+    if (hipGetLastError() != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
       rc = fail(FLB_ERR_HIP, "probe timing failed");
     } else {
-      *gbs = 2.0 * double(n4 * 16) * reps / (double(ms) * 1e-3) / 1e9;
+      const double moved = (mode == 0 ? 2.0 : 1.0) * double(n4 * 16);
+      *gbs = moved * reps / (double(ms) * 1e-3) / 1e9;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1065,6 +1100,22 @@ int flb_bandwidth_probe(size_t bytes, int reps, double* gbs) {
   (void)hipFree(src);
   (void)hipFree(dst);
   return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int flb_bandwidth_probe(size_t bytes, int reps, double* gbs) {
+  if (!gbs || reps < 1 || bytes < 16) return fail(FLB_ERR_ARG, "bad probe arguments");
+  return run_probe(0, 1, false, 0, true, bytes, reps, gbs);
+}
+
+int flb_bandwidth_probe_ex(int mode, int unroll, int nontemporal, int wg_per_cu, size_t bytes, int reps,
+                           double* gbs) {
+  if (!gbs || reps < 1 || mode < 0 || mode > 2 || wg_per_cu < 0 ||
+      !(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8))
+    return fail(FLB_ERR_ARG, "bad probe arguments");
+  return run_probe(mode, unroll, nontemporal != 0, wg_per_cu, false, bytes, reps, gbs);
 }
 
 }  // extern "C"

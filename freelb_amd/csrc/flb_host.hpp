@@ -44,6 +44,9 @@ struct flb_block_s {
   int64_t step = 0;
   int parity = 0;
   int pending = 0;  // 1: cell dynamics applied, stream not yet called
+  // refinement level of the block and the hierarchy's finest level: the
+  // count-gated calls step the block when count % 2^(max_level - level) == 0
+  int level = 0, max_level = 0;
   int vec = 0;  // step kernel form: 0 automatic cell form, -1 row segments, -3 linear runs,
                 // 1/2/4 row form with that many cells per thread
   int wpb = 4;   // wavefronts per workgroup
@@ -112,6 +115,11 @@ struct DeviceGuard {
 };
 
 KArgs make_args(const flb_block_s* b);
+// True when a count-gated call acts on a block of `level` in a hierarchy whose
+// finest level is max_level (BlockLatticeManager's 2^(maxlevel - level) rule).
+inline bool level_due(int64_t count, int level, int max_level) {
+  return count % (int64_t(1) << (max_level - level)) == 0;
+}
 // Builds the block's wall-link words when flags or halfway managers changed
 // (enqueued on the block's stream); a no-op without halfway managers.
 int ensure_links(flb_block_s* b);

@@ -121,6 +121,12 @@ hipError_t launch_links(int q, const KArgs& a, uint64_t* links, hipStream_t s);
 
 // float4 copy of n4 16-byte elements, one element per thread (bandwidth calibration).
 hipError_t launch_copy_probe(void* dst, const void* src, size_t n4, hipStream_t s);
+// Bandwidth probe forms (flb_probe.hip): mode 0 copy, 1 read-only, 2
+// write-only; unroll 16-byte accesses per lane in flight (1, 2, 4, 8);
+// non-temporal or plain; grid workgroups striding over n4 elements (n4 a
+// multiple of unroll * 256).
+hipError_t launch_bw_probe(int mode, int unroll, bool nt, unsigned grid, void* dst, const void* src, size_t n4,
+                           hipStream_t s);
 
 // Convergence residual of one field (ncomp components at fstride, field and
 // old in slot layout from slot 0) over the host cells [shift, n - shift) per

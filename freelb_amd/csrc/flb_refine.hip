@@ -312,6 +312,7 @@ struct flb_refine_s {
   bool avg_links = false, intp_links = false;  // some source cell is a halfway boundary cell
   RF::AvgItem* d_avg = nullptr;
   RF::IntpItem* d_intp = nullptr;
+  int coarse_level = 0, max_level = 1;  // level gates of the exchanges
 };
 
 namespace {
@@ -374,10 +375,26 @@ int launch_comm(flb_refine_s* r, bool average) {
   return FLB_OK;
 }
 
+// Average into the coarse block (gated by its level), interpolation into the
+// fine block (gated by the fine level).
+int comm_average(flb_refine_s* r, int64_t count) {
+  return level_due(count, r->coarse_level, r->max_level) ? launch_comm(r, true) : FLB_OK;
+}
+int comm_interpolate(flb_refine_s* r, int64_t count) {
+  return level_due(count, r->coarse_level + 1, r->max_level) ? launch_comm(r, false) : FLB_OK;
+}
+
 int communicate(flb_refine_s* r, int64_t count) {
-  int rc = FLB_OK;
-  if (count % 2 == 0 && (rc = launch_comm(r, true))) return rc;
-  return launch_comm(r, false);
+  const int rc = comm_average(r, count);
+  return rc ? rc : comm_interpolate(r, count);
+}
+
+int check_comm(flb_refine_s* r, int64_t count) {
+  if (!r) return fail(FLB_ERR_ARG, "null refinement handle");
+  if (count < 0) return fail(FLB_ERR_ARG, "negative count");
+  if (r->coarse->pending || r->fine->pending)
+    return fail(FLB_ERR_STATE, "communication between ApplyCellDynamics and Stream");
+  return FLB_OK;
 }
 
 }  // namespace
@@ -507,11 +524,32 @@ int flb_refine_info(flb_refine r, int64_t* n_average, int64_t* n_interp) {
   return FLB_OK;
 }
 
-int flb_refine_communicate(flb_refine r, int64_t count) {
+int flb_refine_set_levels(flb_refine r, int coarse_level, int max_level) {
   if (!r) return fail(FLB_ERR_ARG, "null refinement handle");
-  if (count < 0) return fail(FLB_ERR_ARG, "negative count");
-  if (r->coarse->pending || r->fine->pending)
-    return fail(FLB_ERR_STATE, "communication between ApplyCellDynamics and Stream");
+  if (coarse_level < 0 || max_level < coarse_level + 1 || max_level > 30)
+    return fail(FLB_ERR_ARG, "coarse level %d / max level %d", coarse_level, max_level);
+  r->coarse_level = coarse_level;
+  r->max_level = max_level;
+  return FLB_OK;
+}
+
+int flb_refine_average(flb_refine r, int64_t count) {
+  int rc = check_comm(r, count);
+  if (rc) return rc;
+  DeviceGuard g(r->device);
+  return comm_average(r, count);
+}
+
+int flb_refine_interpolate(flb_refine r, int64_t count) {
+  int rc = check_comm(r, count);
+  if (rc) return rc;
+  DeviceGuard g(r->device);
+  return comm_interpolate(r, count);
+}
+
+int flb_refine_communicate(flb_refine r, int64_t count) {
+  int rc = check_comm(r, count);
+  if (rc) return rc;
   DeviceGuard g(r->device);
   return communicate(r, count);
 }
@@ -519,6 +557,8 @@ int flb_refine_communicate(flb_refine r, int64_t count) {
 int flb_refine_run(flb_refine r, int64_t count0, int n) {
   if (!r) return fail(FLB_ERR_ARG, "null refinement handle");
   if (count0 < 0 || n < 0) return fail(FLB_ERR_ARG, "negative count");
+  if (r->coarse_level != 0 || r->max_level != 1)
+    return fail(FLB_ERR_STATE, "flb_refine_run steps a two-level pair; a deeper hierarchy steps its blocks by count");
   if (r->coarse->pending || r->fine->pending) return fail(FLB_ERR_STATE, "step while a Stream is pending");
   DeviceGuard g(r->device);
   int rc = FLB_OK;

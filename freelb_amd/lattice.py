@@ -230,11 +230,19 @@ class BlockLattice:
         return out
 
     # ---- the hot path -------------------------------------------------------
-    def ApplyCellDynamics(self) -> None:
-        abi.check(abi.lib().flb_apply_cell_dynamics(self._h))
+    def ApplyCellDynamics(self, count: int = 0) -> None:
+        """The fused collide-and-stream step, gated by the block's level
+        (BlockLatticeManager::ApplyCellDynamics(count, ...): the block steps
+        when count % 2^(max_level - level) == 0)."""
+        abi.check(abi.lib().flb_apply_cell_dynamics(self._h, int(count)))
 
-    def Stream(self) -> None:
-        abi.check(abi.lib().flb_stream(self._h))
+    def Stream(self, count: int = 0) -> None:
+        abi.check(abi.lib().flb_stream(self._h, int(count)))
+
+    def set_level(self, level: int, max_level: int) -> None:
+        """Refinement level of the block and the finest level of its hierarchy."""
+        abi.check(abi.lib().flb_set_level(self._h, int(level), int(max_level)))
+        self.level, self.max_level = int(level), int(max_level)
 
     def ApplyBoundaries(self) -> None:
         abi.check(abi.lib().flb_apply_halfway_bc(self._h))
@@ -382,6 +390,16 @@ class Refinement:
         abi.check(abi.lib().flb_refine_info(self._h, C.byref(a), C.byref(b)))
         return {"average": a.value, "interp": b.value}
 
+    def set_levels(self, coarse_level: int, max_level: int) -> None:
+        """Levels of the pair in a deeper hierarchy (default 0 / 1)."""
+        abi.check(abi.lib().flb_refine_set_levels(self._h, int(coarse_level), int(max_level)))
+
+    def average(self, count: int) -> None:
+        abi.check(abi.lib().flb_refine_average(self._h, int(count)))
+
+    def interpolate(self, count: int) -> None:
+        abi.check(abi.lib().flb_refine_interpolate(self._h, int(count)))
+
     def communicate(self, count: int) -> None:
         abi.check(abi.lib().flb_refine_communicate(self._h, int(count)))
 
@@ -405,6 +423,46 @@ class Refinement:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class RefinementManager:
+    """Blocks of several refinement levels and the refined pairs between them
+    (any number of patches and levels), stepped by count as FreeLB's
+    BlockLatticeManager does (data_struct/block_lattice.hh:736-764,
+    1462-1561): at count c each block of level L steps when
+    c % 2^(Lmax - L) == 0 (ApplyCellDynamics(count), Stream(count)); then
+    every pair averages into its coarse block (AverageCommunicate(count),
+    gated by the coarse level); then every pair interpolates into its fine
+    block (InterpolateCommunicate(count), gated by the fine level).
+    levels[k] is the level of blocks[k]; a pair's fine block must be one
+    level above its coarse block."""
+
+    def __init__(self, blocks: Sequence[BlockLattice], levels: Sequence[int], refinements: Sequence[Refinement]):
+        self.blocks, self.levels, self.refinements = list(blocks), [int(v) for v in levels], list(refinements)
+        if len(self.levels) != len(self.blocks):
+            raise ValueError("one level per block")
+        self.max_level = max(self.levels)
+        lev = {id(b): lv for b, lv in zip(self.blocks, self.levels)}
+        for b, lv in zip(self.blocks, self.levels):
+            b.set_level(lv, self.max_level)
+        for r in self.refinements:
+            lc, lf = lev.get(id(r.coarse)), lev.get(id(r.fine))
+            if lc is None or lf is None or lf != lc + 1:
+                raise ValueError("a refinement's blocks must be managed blocks one level apart")
+            r.set_levels(lc, self.max_level)
+        self.count = 0
+
+    def run(self, n: int) -> None:
+        for _ in range(int(n)):
+            c = self.count
+            for b in self.blocks:
+                b.ApplyCellDynamics(c)
+                b.Stream(c)
+            for r in self.refinements:
+                r.average(c)
+            for r in self.refinements:
+                r.interpolate(c)
+            self.count += 1
 
 
 class BlockLatticeManager:

@@ -295,10 +295,22 @@ class BlockLattice {
       check(flb_set_tasks(h_, t.data(), int(t.size())));
       tasks_ = t;
     }
-    check(flb_apply_cell_dynamics(h_));
+    check(flb_apply_cell_dynamics(h_, count_));
   }
-  // BlockLattice::Stream (block_lattice.hh:64-70): completes the fused step.
-  void Stream() { check(flb_stream(h_)); }
+  // BlockLatticeManager::ApplyCellDynamics(std::int64_t count, BFM)
+  // (block_lattice.hh:751-764): steps the block when its level is due at count.
+  template <typename Selector>
+  void ApplyCellDynamics(std::int64_t count, const std::vector<uint8_t>& flags) {
+    count_ = count;
+    ApplyCellDynamics<Selector>(flags);
+    count_ = 0;
+  }
+  // BlockLattice::Stream (block_lattice.hh:64-70): completes the fused step;
+  // Stream(count) is the level-gated form (block_lattice.hh:737-742).
+  void Stream() { check(flb_stream(h_, 0)); }
+  void Stream(std::int64_t count) { check(flb_stream(h_, count)); }
+  // BasicBlock level and BlockLatticeManager::getMaxLevel of the hierarchy.
+  void setLevel(int level, int max_level) { check(flb_set_level(h_, level, max_level)); }
 
   // rho, u of cells with flag & mask (moment::rhoU<CELL, true>): (d+1)*N SoA.
   std::vector<T> moments(uint8_t mask) const {
@@ -358,6 +370,7 @@ class BlockLattice {
 
   flb_block h_ = nullptr;
   size_t n_ = 0;
+  std::int64_t count_ = 0;  // count of the level-gated call in progress (0: always due)
   const std::vector<uint8_t>* flags_src_ = nullptr;
   std::vector<flb_task> tasks_;
   std::vector<flb_hwbc> bcs_;
@@ -427,6 +440,12 @@ class RefinedPair {
     if (count % 2 == 0) coarse_.Stream();
   }
   void FullCommunicate(int64_t count) { check(flb_refine_communicate(handle(), count)); }
+  // the pair's AverageCommunicate(count) / InterpolateCommunicate(count), for
+  // hierarchies that run every pair's average before any interpolation
+  void AverageCommunicate(int64_t count) { check(flb_refine_average(handle(), count)); }
+  void InterpolateCommunicate(int64_t count) { check(flb_refine_interpolate(handle(), count)); }
+  // levels of the pair in a deeper hierarchy (coarse level, finest level)
+  void setLevels(int coarse_level, int max_level) { check(flb_refine_set_levels(handle(), coarse_level, max_level)); }
   void Run(int64_t count0, int n) { check(flb_refine_run(handle(), count0, n)); }
 
   // created on first use: both blocks need their flags and tasks first

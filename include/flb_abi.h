@@ -262,10 +262,20 @@ int flb_download_pops_f64(flb_block b, double* pops);
 int flb_init_equilibrium(flb_block b, const double* rhou);
 
 /* ---- the hot path ------------------------------------------------------- */
-/* Collide + stream + halfway fix-up of one time step, fused (AA pattern). */
-int flb_apply_cell_dynamics(flb_block b);
-/* Completes the step begun by flb_apply_cell_dynamics (AA parity flip, O(1)). */
-int flb_stream(flb_block b);
+/* Collide + stream + halfway fix-up of one time step, fused (AA pattern),
+ * gated by the block's refinement level as BlockLatticeManager::ApplyCellDynamics(
+ * std::int64_t count, BFM) gates it (data_struct/block_lattice.hh:751-764): the
+ * block steps when count % 2^(max_level - level) == 0 and the call is a no-op
+ * otherwise. A block of level 0 in a hierarchy of one level steps at every count. */
+int flb_apply_cell_dynamics(flb_block b, int64_t count);
+/* Completes the step begun by flb_apply_cell_dynamics (AA parity flip, O(1)),
+ * with the same gate (BlockLatticeManager::Stream(std::int64_t count),
+ * block_lattice.hh:737-742). */
+int flb_stream(flb_block b, int64_t count);
+/* The block's level and the finest level of its hierarchy (BasicBlock::getLevel
+ * and BlockLatticeManager::getMaxLevel, data_struct/block_lattice.h:364-372);
+ * default 0 / 0. */
+int flb_set_level(flb_block b, int level, int max_level);
 /* Halfway managers are fused into flb_apply_cell_dynamics; this checks call
  * order only and keeps the reference's three-call step shape. */
 int flb_apply_halfway_bc(flb_block b);
@@ -363,11 +373,23 @@ typedef struct flb_refine_s* flb_refine;
 int flb_refine_create(flb_block coarse, flb_block fine, const int* origin, void* stream, flb_refine* out);
 int flb_refine_destroy(flb_refine r);  /* the blocks stay */
 int flb_refine_info(flb_refine r, int64_t* n_average, int64_t* n_interp);
-/* FullCommunicate(count) of the pair: average at even counts, then interpolation. */
+/* Levels of a pair inside a deeper hierarchy: the coarse block is at
+ * coarse_level, the fine one at coarse_level + 1, the finest level is
+ * max_level (default 0 / 1). The average into the coarse block runs at counts
+ * with count % 2^(max_level - coarse_level) == 0 and the interpolation into the
+ * fine block at counts with count % 2^(max_level - coarse_level - 1) == 0,
+ * each gated by its receiving block's level as FreeLB's AverageCommunicate(count)
+ * and InterpolateCommunicate(count) gate them (block_lattice.hh:1520-1561). */
+int flb_refine_set_levels(flb_refine r, int coarse_level, int max_level);
+/* AverageCommunicate(count) / InterpolateCommunicate(count) of the pair alone,
+ * for managers that run every pair's average before any interpolation. */
+int flb_refine_average(flb_refine r, int64_t count);
+int flb_refine_interpolate(flb_refine r, int64_t count);
+/* FullCommunicate(count) of the pair: the average, then the interpolation. */
 int flb_refine_communicate(flb_refine r, int64_t count);
-/* n counts of the refined loop from count0 (examples/cavref2d/cavref2d.cpp:221-228):
- * the fine block steps every count, the coarse block on even counts, then
- * flb_refine_communicate(count). */
+/* n counts of the refined loop from count0 (examples/cavref2d/cavref2d.cpp:221-228)
+ * for a pair that is the whole hierarchy (levels 0 / 1): the fine block steps
+ * every count, the coarse block on even counts, then flb_refine_communicate(count). */
 int flb_refine_run(flb_refine r, int64_t count0, int n);
 
 /* ---- differentiation -------------------------------------------------------
@@ -433,6 +455,14 @@ int flb_adjoint_batch_timing(flb_adjoint_batch b, double* forward_ms, double* ba
  * (read + written bytes) / mean launch time. No reference counterpart: it
  * calibrates the roofline figures the benchmark reports. */
 int flb_bandwidth_probe(size_t bytes, int reps, double* gbs);
+/* The probe in the other shapes a streaming kernel can take: mode 0 copy,
+ * 1 read-only, 2 write-only; `unroll` 16-byte accesses in flight per lane
+ * (1, 2, 4 or 8); plain or non-temporal accesses; `wg_per_cu` workgroups per
+ * CU striding over the buffer (0: one workgroup per tile of unroll * 256
+ * elements, each visited once). *gbs counts read + written bytes. No
+ * reference counterpart: it measures the achievable HBM rate. */
+int flb_bandwidth_probe_ex(int mode, int unroll, int nontemporal, int wg_per_cu, size_t bytes, int reps,
+                           double* gbs);
 
 #ifdef __cplusplus
 }

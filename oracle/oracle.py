@@ -291,6 +291,92 @@ class RefinedCavity:
             self.count += 1
 
 
+class RefinedHierarchy:
+    """A lid-driven cavity refined in several patches and levels, stepped as
+    FreeLB's BlockLatticeManager loop does for any number of levels
+    (data_struct/block_lattice.hh:736-764, 1462-1561): at count c every block
+    of level L with c % 2^(Lmax - L) == 0 steps (ApplyCellDynamics(count) and
+    Stream(count)); then every pair's average into its coarse block, gated by
+    the coarse level (AverageCommunicate(count)); then every pair's
+    interpolation into its fine block, gated by the fine level
+    (InterpolateCommunicate(count)).
+
+    patches: (parent, origin, m) per patch, in creation order: the index of
+    the parent block (0 is the cavity), the parent cell where the patch
+    starts (per axis, the parent's local index, as RefinedCavity's origin)
+    and the patch extent in parent cells per axis. A patch block has 2m + 4
+    cells per axis (two ghost layers) and the parent's cells inside the patch
+    beyond its outer layer are COVERED. Each level's omega is
+    RefineConverter::getOmegaF of its parent's (lbm/unit_converter.h:627).
+    The cavity and its tasks are RefinedCavity's; a hierarchy of one patch
+    is RefinedCavity."""
+
+    def __init__(self, lattice, Ni, Nj, Nk, patches, RT=0.6, visc=0.01, u_phys=0.05, dtype=np.float32):
+        q = 19 if lattice == "d3q19" else 9
+        d = 3 if q == 19 else 2
+        self.q, self.d, self.dtype = q, d, dtype
+        omega, ulat = convert_from_rt(np.float32(RT), np.float32(visc), np.float32(u_phys))
+        if dtype == np.float64:
+            omega = np.float64(1.0 / RT)
+        self.ulat = ulat
+        cflags = cavity_flags(Ni, Nj, Nk if d == 3 else 0)
+        dims = [(Ni + 2, Nj + 2, (Nk + 2) if d == 3 else 1)]
+        flags = [cflags.copy()]
+        levels, omegas, self.patches = [0], [omega], []
+        for parent, origin, m in patches:
+            o = [int(v) for v in (list(origin) + [0, 0])[:3]]
+            mm = [int(v) for v in (list(m) + [1, 1])[:3]] if not np.isscalar(m) else [int(m)] * d + [1] * (3 - d)
+            pd = dims[parent]
+            f3 = flags[parent].reshape(pd[::-1])
+            inner = tuple(slice(o[a] + 1, o[a] + mm[a] - 1) for a in range(d))
+            if d == 3:
+                f3[inner[::-1]] = COVERED
+            else:
+                f3[0][inner[::-1]] = COVERED
+            fd = tuple(2 * mm[a] + 4 if a < d else 1 for a in range(3))
+            dims.append(fd)
+            flags.append(np.full(int(np.prod(fd)), AABB, dtype=np.uint8))
+            levels.append(levels[parent] + 1)
+            om_p = omegas[parent]
+            if dtype == np.float64:
+                omegas.append(np.float64(1.0) / (np.float64(2.0) / om_p - np.float64(0.5)))
+            else:
+                omegas.append(np.float32(1) / (np.float32(2) / np.float32(om_p) - np.float32(0.5)))
+            self.patches.append((parent, len(dims) - 1, o))
+        self.levels, self.omegas, self.dims = levels, omegas, dims
+        self.max_level = max(levels)
+        tasks = [(AABB, DYN_BGK_WRITE), (BB | BBMW, DYN_BGK_FIELD)]
+        bcs = [(BB, VOID, HWBC_NORMAL), (BBMW, VOID, HWBC_MOVING)]
+        w = weights(q, dtype)
+        self.blocks, self.pops = [], []
+        for k, (dm, fl) in enumerate(zip(dims, flags)):
+            N = fl.size
+            u = np.zeros((d, N))
+            if k == 0:
+                u[0, fl == BBMW] = float(ulat)
+            self.blocks.append(Lattice(q, dm, fl, tasks, bcs if k == 0 else [], omega=float(omegas[k]),
+                                       rho_field=np.ones(N), u_field=u))
+            self.pops.append(np.ascontiguousarray(np.repeat(w[:, None], N, axis=1).astype(dtype)))
+        self.count = 0
+
+    def due(self, level, count):
+        return count % (1 << (self.max_level - level)) == 0
+
+    def run(self, n):
+        for _ in range(n):
+            c = self.count
+            for k, b in enumerate(self.blocks):
+                if self.due(self.levels[k], c):
+                    b.run(self.pops[k], 1)
+            for pa, ch, o in self.patches:
+                if self.due(self.levels[pa], c):
+                    refine_average(self.blocks[pa], self.pops[pa], self.blocks[ch], self.pops[ch], o)
+            for pa, ch, o in self.patches:
+                if self.due(self.levels[ch], c):
+                    refine_interpolate(self.blocks[pa], self.pops[pa], self.blocks[ch], self.pops[ch], o)
+            self.count += 1
+
+
 def cavity_flags(Ni, Nj, Nk=0):
     n = (Ni + 2) * (Nj + 2) * ((Nk + 2) if Nk > 0 else 1)
     out = np.empty(n, dtype=np.uint8)

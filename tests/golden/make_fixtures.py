@@ -11,7 +11,8 @@ reference sources, so it runs in the build container, not on the GPU box.
     python tests/golden/make_fixtures.py [--only MODE]   (MODE: write only that mode's small fixtures,
                                                           "refined" for the refined-cavity ones,
                                                           "vtk" for FreeLB's VTK files,
-                                                          "natconv" for the coupled-lattice ones)
+                                                          "natconv" for the coupled-lattice ones,
+                                                          "shipped" for the as-shipped build's one)
 """
 from __future__ import annotations
 
@@ -57,6 +58,13 @@ EXE_VTK = os.path.join(ROOT, "oracle", "_ref", "vtk_ref")
 NATCONV = [("d2q9", 24, 16, 1, 300, "0.6", "0.7", "0.001", "1.5", "0.5", "1.0"),
            ("d3q19", 10, 8, 6, 120, "0.6", "0.8", "0.002", "1.5", "0.5", "1.0")]
 EXE_NC = os.path.join(ROOT, "oracle", "_ref", "natconv_ref")
+# The same harness built with FreeLB's benchmark flags (floating-point
+# contraction on, -march=native): the halfway 100^3 cavity after 1000 steps,
+# rho / u on a sample of cells (two planes, a strided lattice of cells and the
+# cells where the two builds differ most), plus the full-field differences
+# between that build and the contraction-off one the bit-exact fixtures use.
+EXE_SHIPPED = os.path.join(ROOT, "oracle", "_ref", "cavity_ref_shipped")
+SHIPPED = ("d3q19", "hw", 100, 100, 100, 1000)
 
 
 def read_dump(path):
@@ -151,6 +159,47 @@ def run(lat, mode, ni, nj, nk, steps, tmp):
     return read_dump(out)
 
 
+def shipped_errors(ref, got, nonvoid):
+    """rho: max |d rho| / |rho|; u: max |d u| / max |u| over the cells, and the
+    pointwise form max |d u| / max(|u|, 1e-3 max |u|)."""
+    r, g = ref[:, nonvoid].astype(np.float64), got[:, nonvoid].astype(np.float64)
+    umax = float(np.abs(r[1:]).max())
+    du = np.abs(g[1:] - r[1:])
+    return {"rho_max_rel": float((np.abs(g[0] - r[0]) / np.abs(r[0])).max()),
+            "u_max_abs": float(du.max()), "u_max_over_umax": float(du.max() / umax),
+            "u_pointwise_floor1e-3": float((du / np.maximum(np.abs(r[1:]), 1e-3 * umax)).max()),
+            "u_max": umax}
+
+
+def write_shipped(tmp):
+    if not os.path.exists(EXE_SHIPPED):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "_ref/cavity_ref_shipped"], check=True)
+    lat, mode, ni, nj, nk, steps = SHIPPED
+    off = run(lat, mode, ni, nj, nk, steps, tmp)
+    args = [EXE_SHIPPED, lat, mode, str(ni), str(nj), str(nk), str(steps), RT, VISC, U,
+            os.path.join(tmp, "shipped.bin")]
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+    subprocess.run(args, check=True, capture_output=True, env=env)
+    shp = read_dump(os.path.join(tmp, "shipped.bin"))
+    nx, ny, nz = shp["dims"]
+    nonvoid = shp["flags"] != 1
+    # sample: the mid-plane y = ny // 2, the lid's fluid layer, every 5th cell
+    # per axis, and the 2000 cells whose u differs most between the builds
+    ids = np.arange(nx * ny * nz).reshape(nz, ny, nx)
+    pick = [ids[:, ny // 2, :].ravel(), ids[nz - 2, :, :].ravel(), ids[1:-1:5, 1:-1:5, 1:-1:5].ravel()]
+    du = np.abs(shp["rhou"][1:].astype(np.float64) - off["rhou"][1:]).max(axis=0)
+    du[~nonvoid] = -1.0
+    pick.append(np.argsort(du)[-2000:])
+    cells = np.unique(np.concatenate(pick))
+    cells = cells[nonvoid[cells]].astype(np.int32)
+    stats = shipped_errors(off["rhou"], shp["rhou"], nonvoid)
+    name = f"cavity_{lat}_{mode}_{ni}x{nj}x{nk}_{steps}_shipped.npz"
+    np.savez_compressed(os.path.join(HERE, name), cells=cells, rhou=shp["rhou"][:, cells],
+                        dims=np.array([nx, ny, nz]), steps=steps,
+                        stats_vs_contraction_off=json.dumps(stats))
+    print("wrote", name, stats, flush=True)
+
+
 def main():
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if not os.path.exists(EXE):
@@ -171,6 +220,10 @@ def main():
                     with open(os.path.join(src, fn), "rb") as a, open(os.path.join(outdir, fn), "wb") as b:
                         b.write(a.read())
                     print("wrote vtk/" + fn, flush=True)
+            if only:
+                return
+        if only in (None, "shipped"):
+            write_shipped(tmp)
             if only:
                 return
         if only in (None, "natconv"):

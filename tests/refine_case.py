@@ -87,3 +87,45 @@ def compare_masks(rc):
     else:
         fm[0, 1:-1, 1:-1] = True
     return cm, fm.reshape(-1)
+
+
+def gpu_hierarchy(rh, storage="f32"):
+    """The oracle hierarchy's blocks as GPU blocks (each patch block's outer
+    ghost layer is its overlap), its pairs as Refinements and the manager
+    that steps them, at the oracle's current state."""
+    from freelb_amd import Refinement, RefinementManager
+    from gpu_case import gpu_lattice
+    blocks = [gpu_lattice(rh.blocks[0], storage, rh.pops[0])]
+    for k in range(1, len(rh.blocks)):
+        b = rh.blocks[k]
+        fl = b.flags.reshape(rh.dims[k][::-1]).copy()
+        if rh.d == 3:
+            fl[0, :, :] = fl[-1, :, :] = O.COVERED
+            fl[:, 0, :] = fl[:, -1, :] = O.COVERED
+            fl[:, :, 0] = fl[:, :, -1] = O.COVERED
+        else:
+            fl[0, 0, :] = fl[0, -1, :] = O.COVERED
+            fl[0, :, 0] = fl[0, :, -1] = O.COVERED
+        case = O.Lattice(rh.q, rh.dims[k], fl.reshape(-1), b.tasks, [], omega=b.omega,
+                         rho_field=b.buffers["rho_field"], u_field=b.buffers["u_field"])
+        blocks.append(gpu_lattice(case, storage, rh.pops[k]))
+    refs = [Refinement(blocks[pa], blocks[ch], o) for pa, ch, o in rh.patches]
+    return blocks, refs, RefinementManager(blocks, rh.levels, refs)
+
+
+def hierarchy_masks(rh):
+    """Cells whose state the GPU must reproduce: per block, the updated cells
+    (the cavity without its void shell, a patch without its outer ghost
+    layer) that no finer patch covers."""
+    out = []
+    for k, b in enumerate(rh.blocks):
+        m = (b.flags != O.VOID) & (b.flags != O.COVERED)
+        if k > 0:
+            inner = np.zeros(rh.dims[k][::-1], dtype=bool)
+            if rh.d == 3:
+                inner[1:-1, 1:-1, 1:-1] = True
+            else:
+                inner[0, 1:-1, 1:-1] = True
+            m &= inner.reshape(-1)
+        out.append(m)
+    return out

@@ -67,8 +67,14 @@ def test_null_arguments():
     assert L.flb_block_create(None, None) == abi.FLB_ERR_ARG
     for fn in ("flb_block_destroy",):
         assert getattr(L, fn)(None) == abi.FLB_OK  # destroying nothing is a no-op
-    for fn in ("flb_apply_cell_dynamics", "flb_stream", "flb_synchronize", "flb_apply_halfway_bc"):
+    for fn in ("flb_synchronize", "flb_apply_halfway_bc"):
         assert getattr(L, fn)(None) == abi.FLB_ERR_ARG
+    for fn in ("flb_apply_cell_dynamics", "flb_stream"):
+        assert getattr(L, fn)(None, 0) == abi.FLB_ERR_ARG
+    assert L.flb_set_level(None, 0, 0) == abi.FLB_ERR_ARG
+    for fn in ("flb_refine_average", "flb_refine_interpolate", "flb_refine_communicate"):
+        assert getattr(L, fn)(None, 0) == abi.FLB_ERR_ARG
+    assert L.flb_refine_set_levels(None, 0, 1) == abi.FLB_ERR_ARG
     assert L.flb_step(None, 1) == abi.FLB_ERR_ARG
     assert L.flb_set_tasks(None, None, 0) == abi.FLB_ERR_ARG
     assert "null block" in L.flb_last_error().decode()

@@ -1,24 +1,37 @@
-"""Batched stepping (one launch per step over a block table) against the same
-blocks stepped one by one: the batch runs the very same per-cell update, so
-populations must agree bit for bit, through the persistent grid, the hipGraph
-replay, the autotuner and settings changed after the batch was built."""
+"""Batched stepping (one launch per step over a block table) against the oracle
+and against the same blocks stepped one by one: the batch runs the very same
+per-cell update, so populations must agree bit for bit, through the
+persistent grid, the hipGraph replay, the autotuner and settings changed
+after the batch was built. In FP32 and FP64 every block is also stepped by
+its oracle twin (tests/twins.py) from the same initial bits."""
 import numpy as np
 import pytest
 
+import bench
 from freelb_amd import BlockLattice, BlockLatticeManager, FLBError, HalfwayBB, Key_TypePair, abi
 from freelb_amd.cases import cavity_block
+from oracle import oracle as O
+from tests.twins import PeriodicXTwin, cavity_twin, interior
 
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(12, 9, 7), (17, 16, 15), (70, 5, 9), (33, 33, 3), (5, 6, 40), (64, 64, 8)]
 
 
+CHANNEL = dict(nx=24, ny=10, nz=7, tau=0.7)
+
+
+def _channel_flags(nx, ny, nz):
+    f = np.full((nz + 2, ny + 2, nx), 1, dtype=np.uint8)
+    f[1:-1, 1:-1, :] = 2
+    return f
+
+
 def _channel(storage="f32", nx=24, ny=10, nz=7, tau=0.7):
     """Periodic-x channel, halfway walls in y and z (exercises the wrapped kernel)."""
     lat = BlockLattice(19, (nx, ny + 2, nz + 2), omega=1.0 / tau, storage=storage,
                        periodic=(True, False, False))
-    f = np.full((nz + 2, ny + 2, nx), 1, dtype=np.uint8)
-    f[1:-1, 1:-1, :] = 2
+    f = _channel_flags(nx, ny, nz)
     lat.upload_flags(f.reshape(-1))
     lat.set_tasks([Key_TypePair(2, abi.DYN_BGK)])
     lat.set_halfway_bc([HalfwayBB(2, 1, False)])
@@ -30,15 +43,43 @@ def _channel(storage="f32", nx=24, ny=10, nz=7, tau=0.7):
     return lat
 
 
-def _blocks(storage="f32", mixed=False):
+def _specs(mixed=False):
     rng = np.random.default_rng(11)
     out = []
     for k, sh in enumerate(SHAPES):
         coll = "mrt_les" if (mixed and k % 3 == 1) else "bgk"
-        out.append(cavity_block(sh, storage=storage, collision=coll, tau=float(rng.uniform(0.51, 0.9)),
-                                u_lid=float(rng.uniform(0.02, 0.1)), u0_amp=1e-3, seed=k))
+        out.append(dict(shape=sh, collision=coll, tau=float(rng.uniform(0.51, 0.9)),
+                        u_lid=float(rng.uniform(0.02, 0.1)), seed=k))
+    return out
+
+
+def _blocks(storage="f32", mixed=False):
+    out = [cavity_block(sp["shape"], storage=storage, collision=sp["collision"], tau=sp["tau"],
+                        u_lid=sp["u_lid"], u0_amp=1e-3, seed=sp["seed"]) for sp in _specs(mixed)]
     out.append(_channel(storage))
     return out
+
+
+def _twins(mixed=False):
+    """Oracle twins of _blocks(): the cavities, then the periodic channel."""
+    tw = [cavity_twin(sp["shape"], collision=sp["collision"], tau=sp["tau"], u_lid=sp["u_lid"])
+          for sp in _specs(mixed)]
+    c = CHANNEL
+    f = _channel_flags(c["nx"], c["ny"], c["nz"]).reshape(-1)
+    tw.append(PeriodicXTwin(19, (c["nx"], c["ny"] + 2, c["nz"] + 2), f, [(2, O.DYN_BGK)],
+                            [(2, 1, O.HWBC_NORMAL)], omega=1.0 / c["tau"]))
+    return tw
+
+
+def _oracle_run(twin, p0, steps):
+    """The twin stepped from p0; returns (pops, cells the block updates)."""
+    p = np.ascontiguousarray(p0.copy())
+    if isinstance(twin, PeriodicXTwin):
+        w = twin.widen(p)
+        twin.run(w, steps)
+        return twin.narrow(w), interior(twin.dims, periodic=(True, False, False))
+    twin.run(p, steps)
+    return p, interior((twin.nx, twin.ny, twin.nz))
 
 
 def _close(bs):
@@ -50,6 +91,7 @@ def _close(bs):
 def test_batch_step_matches_single_blocks(storage, mixed):
     ref, bat = _blocks(storage, mixed), _blocks(storage, mixed)
     try:
+        p0 = [b.pops() for b in bat]
         for b in ref:
             b.step(23)
         with BlockLatticeManager(bat) as m:
@@ -58,6 +100,12 @@ def test_batch_step_matches_single_blocks(storage, mixed):
             m.synchronize()
         for r, b in zip(ref, bat):
             assert np.array_equal(r.pops(), b.pops())
+        if storage in ("f32", "f64"):
+            # the oracle steps every block from the same initial bits
+            for k, (tw, b) in enumerate(zip(_twins(mixed), bat)):
+                want, live = _oracle_run(tw, p0[k], 23)
+                got = b.pops()
+                assert np.array_equal(got[:, live], want[:, live]), f"block {k}"
     finally:
         _close(ref)
         _close(bat)
@@ -83,6 +131,7 @@ def test_batch_launch_shapes(wpb, grid):
 def test_graph_replay_matches_eager_each_time():
     ref, bat = _blocks(), _blocks()
     try:
+        p0 = [b.pops() for b in bat]
         with BlockLatticeManager(bat) as m:
             m.capture(10)
             for rep in range(3):
@@ -95,6 +144,9 @@ def test_graph_replay_matches_eager_each_time():
                     assert np.array_equal(r.pops(), b.pops()), f"replay {rep}"
                     assert np.array_equal(r.moments(), b.moments())
             assert bat[0].info().step == 30
+        for k, (tw, b) in enumerate(zip(_twins(), bat)):
+            want, live = _oracle_run(tw, p0[k], 30)
+            assert np.array_equal(b.pops()[:, live], want[:, live]), f"block {k}"
     finally:
         _close(ref)
         _close(bat)
@@ -157,3 +209,40 @@ def test_batch_rejects_mixed_parity_and_kinds():
     finally:
         other.close()
         _close(bat)
+
+
+def test_config2_workload_graph_against_oracle():
+    """The batched-cavity workload itself: 512 seeded cavities (edges U{16..96}
+    per axis, tau U[0.51, 0.9], lid U[0.02, 0.1], bench.batch_bench's seed),
+    autotuned and replayed as one hipGraph of 500 steps. Eight blocks drawn
+    with a seed (from those of at most 48^3 cells, to bound the oracle's time)
+    equal their oracle twins bit for bit; every block keeps its mass
+    (FP64 sum) to 1e-5 relative and stays finite."""
+    specs = bench.batch_specs()
+    rng = np.random.default_rng(7)
+    small = [k for k, sp in enumerate(specs) if np.prod(sp["shape"]) <= 48 ** 3]
+    picked = sorted(rng.choice(small, size=8, replace=False).tolist())
+    print(f"[config2] blocks {len(specs)}, oracle checks on {picked}", flush=True)
+    blocks = [cavity_block(sp["shape"], tau=sp["tau"], u_lid=sp["u_lid"]) for sp in specs]
+    try:
+        p0 = {k: blocks[k].pops() for k in picked}
+        m0 = np.array([b.mass_momentum()[0] for b in blocks])
+        with BlockLatticeManager(blocks) as m:
+            m.autotune(4)
+            for b in blocks:  # autotune restores the state: back at step 0
+                assert b.info().step == 0
+            m.capture(500)
+            m.replay()
+            m.synchronize()
+        m1 = np.array([b.mass_momentum()[0] for b in blocks])
+        assert np.isfinite(m1).all()
+        drift = np.abs(m1 - m0) / m0
+        print(f"[config2] max relative mass drift {drift.max():.3e}", flush=True)
+        assert drift.max() <= 1e-5
+        for k in picked:
+            sp = specs[k]
+            tw = cavity_twin(sp["shape"], tau=sp["tau"], u_lid=sp["u_lid"])
+            want, live = _oracle_run(tw, p0[k], 500)
+            assert np.array_equal(blocks[k].pops()[:, live], want[:, live]), f"block {k} {sp}"
+    finally:
+        _close(blocks)

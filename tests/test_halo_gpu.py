@@ -1,13 +1,16 @@
 """Same-level block halo: a domain cut into blocks whose one-cell overlap
 layers mirror their neighbours, exchanged after every step, evolves exactly
 as the same domain in one block (bit for bit), with per-block steps, in a
-batch, replayed from a hipGraph, and across a periodic domain axis."""
+batch, replayed from a hipGraph, and across a periodic domain axis. In FP32
+the blocks are also held against the oracle stepping the whole domain (for
+the periodic axis, its periodic twin, tests/twins.py)."""
 import numpy as np
 import pytest
 
 from freelb_amd import (BlockHalo, BlockLattice, BlockLatticeManager, HalfwayBB, Key_TypePair, abi)
 from gpu_case import gpu_lattice
 from oracle import oracle as O
+from tests.twins import PeriodicXTwin
 
 pytestmark = pytest.mark.gpu
 
@@ -47,8 +50,9 @@ def _split_cavity(case, pops, cuts, storage="f32"):
     return blocks, origins, boxes
 
 
-def _compare(glob, blocks, boxes, dims, sel):
-    ref = glob.pops()
+def _compare(ref, blocks, boxes, dims, sel):
+    """Each block's updated cells (sel, overlap cut) against the whole-domain
+    populations ref."""
     for lat, box in zip(blocks, boxes):
         mine = lat.pops().reshape(19, *tuple(b - a for a, b in box)[::-1])
         want = _sub(ref, dims, box, 19).reshape(mine.shape)
@@ -69,6 +73,7 @@ def test_split_cavity_matches_single_block(mode, storage):
     halo = BlockHalo(blocks, origins)
     assert halo.info()["items"] > 0
     sel = case.flags != O.VOID if mode == "hw" else case.flags == O.AABB
+    ref = p.copy() if storage == "f32" else None
     try:
         for n in (1, 1, 3, 4):
             glob.step(n)
@@ -76,7 +81,10 @@ def test_split_cavity_matches_single_block(mode, storage):
                 for b in blocks:
                     b.step(1)
                 halo.exchange()
-            _compare(glob, blocks, boxes, dims, sel)
+            _compare(glob.pops(), blocks, boxes, dims, sel)
+            if ref is not None:
+                case.run(ref, n)
+                _compare(ref, blocks, boxes, dims, sel)
     finally:
         halo.close()
         for b in blocks:
@@ -91,18 +99,23 @@ def test_split_cavity_in_batch_and_graph():
     blocks, origins, boxes = _split_cavity(case, p, ([11, 20], [10], [6, 12]))
     halo = BlockHalo(blocks, origins)
     sel = case.flags != O.VOID
+    ref = p.copy()
     try:
         with BlockLatticeManager(blocks, halo=halo) as m:
             m.step(5)
             glob.step(5)
             m.synchronize()
-            _compare(glob, blocks, boxes, dims, sel)
+            _compare(glob.pops(), blocks, boxes, dims, sel)
+            case.run(ref, 5)
+            _compare(ref, blocks, boxes, dims, sel)
             m.capture(6)
             for _ in range(3):
                 m.replay()
                 glob.step(6)
                 m.synchronize()
-                _compare(glob, blocks, boxes, dims, sel)
+                _compare(glob.pops(), blocks, boxes, dims, sel)
+                case.run(ref, 6)
+                _compare(ref, blocks, boxes, dims, sel)
     finally:
         halo.close()
         for b in blocks:
@@ -145,6 +158,8 @@ def test_periodic_axis_across_blocks():
         origins.append((a - 1, 0, 0))
         boxes.append(xs)
     halo = BlockHalo(blocks, origins, periodic=(True, False, False), extent=(nx, 0, 0))
+    twin = PeriodicXTwin(19, gdims, flags, [(2, O.DYN_BGK)], [(2, 1, O.HWBC_NORMAL)], omega=1 / 0.7)
+    wide = twin.widen(p0)
     try:
         for n in (1, 2, 5):
             glob.step(n)
@@ -152,11 +167,13 @@ def test_periodic_axis_across_blocks():
                 for lat in blocks:
                     lat.step(1)
                 halo.exchange()
-            ref = glob.pops().reshape(19, nz + 2, ny + 2, nx)
-            for lat, xs in zip(blocks, boxes):
-                mine = lat.pops().reshape(19, nz + 2, ny + 2, len(xs))
-                inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
-                assert np.array_equal(mine[inner], ref[:, 1:-1, 1:-1, xs[1:-1]])
+            twin.run(wide, n)
+            for ref in (glob.pops(), twin.narrow(wide)):
+                ref = ref.reshape(19, nz + 2, ny + 2, nx)
+                for lat, xs in zip(blocks, boxes):
+                    mine = lat.pops().reshape(19, nz + 2, ny + 2, len(xs))
+                    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+                    assert np.array_equal(mine[inner], ref[:, 1:-1, 1:-1, xs[1:-1]])
     finally:
         halo.close()
         for lat in blocks:

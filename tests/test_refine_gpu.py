@@ -8,7 +8,7 @@ import pytest
 
 from freelb_amd import BlockLattice, Key_TypePair, Refinement
 from oracle import oracle as O
-from refine_case import compare_masks, fixture_mismatches, gpu_refined, load_fixture
+from refine_case import compare_masks, fixture_mismatches, gpu_hierarchy, gpu_refined, hierarchy_masks, load_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +52,46 @@ def test_refined_cavity_matches_oracle(lattice, n, cell, p0, p1, storage):
         ref.close()
         coarse.close()
         fine.close()
+
+
+# Hierarchies of several patches and levels: (lattice, n, patches), patches as
+# (parent block, origin in the parent's cells, extent in parent cells)
+HIERARCHIES = [
+    # two level-1 patches side by side, a level-2 patch inside the first
+    ("d2q9", 40, [(0, (5, 6), 8), (0, (22, 20), (10, 8)), (1, (7, 6), 4)]),
+    # three levels nested in 3D
+    ("d3q19", 24, [(0, (7, 7, 7), 8), (1, (7, 6, 7), 4)]),
+    # two 3D level-1 patches, one with a level-2 child
+    ("d3q19", 28, [(0, (4, 5, 6), 6), (0, (17, 15, 14), 6), (2, (5, 5, 5), 2)]),
+]
+
+
+@pytest.mark.parametrize("lattice,n,patches", HIERARCHIES, ids=["2d-two-patches-3-levels", "3d-3-levels",
+                                                                "3d-two-patches-3-levels"])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_refined_hierarchy_matches_oracle(lattice, n, patches, storage):
+    """Several patches and three levels (each level steps every 2^(Lmax - L)
+    counts, averages before interpolations): every block bit for bit
+    against the oracle's hierarchy at odd and even counts."""
+    dt = np.float32 if storage == "f32" else np.float64
+    rh = O.RefinedHierarchy(lattice, n, n, n, patches, dtype=dt)
+    print(f"[hierarchy] {lattice} {n} {patches} {storage}", flush=True)
+    blocks, refs, mgr = gpu_hierarchy(rh, storage)
+    masks = hierarchy_masks(rh)
+    try:
+        for k in (1, 2, 3, 7, 13):
+            rh.run(k)
+            mgr.run(k)
+            for j, (b, m) in enumerate(zip(blocks, masks)):
+                assert np.array_equal(b.pops()[:, m], rh.pops[j][:, m]), f"block {j} after {rh.count} counts"
+                fl = rh.blocks[j].flags
+                live = m & (fl == O.AABB)
+                assert np.array_equal(b.download_fields()[:, live], rh.blocks[j].fields()[:, live]), f"fields {j}"
+    finally:
+        for r in refs:
+            r.close()
+        for b in blocks:
+            b.close()
 
 
 def test_refined_cavity_matches_freelb_fixture():
