@@ -390,47 +390,68 @@ def profiled_traffic(storage):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(n=N_EDGE, steps=20):
-    """FreeLB's own benchmarks/cavity3d driver (built from the reference sources
-    into oracle/_ref by oracle/Makefile) on the host cores, bounded sample."""
+def _fullway_reference(n, steps, threads):
+    """FreeLB's own benchmarks/cavity3d driver (fullway walls), MLUPS over the
+    fluid cells, or None when its binary is absent."""
     exe = os.path.join(ROOT, "oracle", "_ref", "cavity3d_bench")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="flb_cpu_")
+    try:
+        ini = (f"[workdir]\nworkdir_ = ./output\n[parallel]\nthread_num = {threads}\nblock_num = 1\n"
+               f"[Mesh]\nNi = {n}\nNj = {n}\nNk = {n}\nCell_Len = 1\n"
+               "[Physical_Property]\nrho_ref = 0.001\nKine_Visc = 0.01\n"
+               "[Init_Conditions]\nU_Ini0 = 0\nU_Ini1 = 0\nU_Ini2 = 0\nU_Max = 0.1\n"
+               "[Boundary_Conditions]\nVelo_Wall0 = 0.1\nVelo_Wall1 = 0\nVelo_Wall2 = 0\n"
+               f"[LB]\nRT = 0.53\n[Simulation_Settings]\nTotalStep = {steps}\nOutputStep = 100000\n"
+               "[tolerance]\ntol = 1e-4\n")
+        with open(os.path.join(work, "cavity3d.ini"), "w") as fh:
+            fh.write(ini)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        out = subprocess.run([exe], cwd=work, env=env, capture_output=True, text=True, timeout=600).stdout
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    for line in out.splitlines():
+        if line.strip().startswith("Average_MLUPs:"):  # utils/timer.h:93
+            return float(line.split(":", 1)[1].split()[0]) * n ** 3 / (n + 2) ** 3
+    return None
+
+
+def cpu_baseline(n=N_EDGE, steps=200, warmup=5):
+    """FreeLB itself on the host cores, on the workload of `value`: the 256^3
+    halfway cavity (BGK<rhoU> on every cavity cell, bounceback::normal /
+    movingwall managers after Stream, boundary/boundary.h:121-128), built
+    from the reference sources with its benchmarks' flags
+    (oracle/_ref/cavity_hw_bench), `steps` timed steps after `warmup`.
+    FreeLB's published fullway benchmark (benchmarks/cavity3d) is timed
+    beside it (fullway_value, 20 steps). Without the binaries, the C oracle
+    (the same algorithm restated) is timed instead."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    exe = os.path.join(ROOT, "oracle", "_ref", "cavity_hw_bench")
     if os.path.exists(exe):
-        work = tempfile.mkdtemp(prefix="flb_cpu_")
-        try:
-            ini = (f"[workdir]\nworkdir_ = ./output\n[parallel]\nthread_num = {threads}\nblock_num = 1\n"
-                   f"[Mesh]\nNi = {n}\nNj = {n}\nNk = {n}\nCell_Len = 1\n"
-                   "[Physical_Property]\nrho_ref = 0.001\nKine_Visc = 0.01\n"
-                   "[Init_Conditions]\nU_Ini0 = 0\nU_Ini1 = 0\nU_Ini2 = 0\nU_Max = 0.1\n"
-                   "[Boundary_Conditions]\nVelo_Wall0 = 0.1\nVelo_Wall1 = 0\nVelo_Wall2 = 0\n"
-                   f"[LB]\nRT = 0.53\n[Simulation_Settings]\nTotalStep = {steps}\nOutputStep = 100000\n"
-                   "[tolerance]\ntol = 1e-4\n")
-            with open(os.path.join(work, "cavity3d.ini"), "w") as fh:
-                fh.write(ini)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-            out = subprocess.run([exe], cwd=work, env=env, capture_output=True, text=True,
-                                 timeout=600).stdout
-        finally:
-            shutil.rmtree(work, ignore_errors=True)
-        mlups_total = None
-        for line in out.splitlines():
-            if line.strip().startswith("Average_MLUPs:"):  # utils/timer.h:93
-                mlups_total = float(line.split(":", 1)[1].split()[0])
-        if mlups_total is not None:
-            active = mlups_total * n ** 3 / (n + 2) ** 3
-            return {"value": active, "unit": "MLUPS", "cores": threads, "kind": "reference",
-                    "sample": f"FreeLB benchmarks/cavity3d (fullway walls), {n}^3 fluid cells, "
-                              f"{steps} timed steps after 10 warm-up, OpenMP {threads} threads; "
-                              f"MLUPS over fluid cells ({mlups_total:.1f} counting the ghost layer)"}
-    # fall back to the C oracle (same algorithm restated), timed the same way
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        out = subprocess.run([exe, "d3q19", "hw", str(n), str(n), str(n), str(steps), "0.53", "0.01", "0.1",
+                              f"time:{warmup}"], env=env, capture_output=True, text=True, timeout=900).stdout
+        fields = out.split()
+        if fields and fields[0] == "MLUPS":
+            rec = {"value": float(fields[1]), "unit": "MLUPS", "cores": threads, "kind": "reference",
+                   "sample": f"FreeLB halfway cavity (BGK<rhoU> + bounceback normal / movingwall managers), "
+                             f"{n}^3 cells, {steps} timed steps after {warmup}, OpenMP {threads} threads, "
+                             f"built with the benchmarks' flags; {float(fields[7]):.1f} s"}
+            full = _fullway_reference(n, 20, threads)
+            if full is not None:
+                rec["fullway_value"] = full
+                rec["fullway_sample"] = (f"FreeLB benchmarks/cavity3d (fullway walls), {n}^3 fluid cells, "
+                                         "20 timed steps after 10 warm-up")
+            return rec
     from oracle import oracle as O
     case, p = O.cavity_case("hw", n, n, n)
     case.L.nthreads = threads
     t0 = time.perf_counter()
-    case.run(p, steps)
+    case.run(p, 20)
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"C oracle, halfway cavity {n}^3, {steps} steps, OpenMP {threads} threads"}
+    return {"value": n ** 3 * 20 / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C oracle, halfway cavity {n}^3, 20 steps, OpenMP {threads} threads"}
 
 
 def main():

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FLB_LIB_PATH") or os.path.join(HERE, "libflb_hip.so")
 
 # ---- enums (include/flb_abi.h) ---------------------------------------------
-FLB_OK, FLB_ERR_ARG, FLB_ERR_STATE, FLB_ERR_HIP, FLB_ERR_UNSUPPORTED, FLB_ERR_LAYOUT, FLB_ERR_NOMEM = range(7)
+(FLB_OK, FLB_ERR_ARG, FLB_ERR_STATE, FLB_ERR_HIP, FLB_ERR_UNSUPPORTED, FLB_ERR_LAYOUT, FLB_ERR_NOMEM,
+ FLB_ERR_NONFINITE) = range(8)
 D2Q9, D3Q19, D2Q5, D3Q7 = 9, 19, 5, 7
 STORE_F32, STORE_F16S, STORE_BF16S, STORE_F64, STORE_F16C = range(5)
 STORAGE = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S, "f64": STORE_F64, "f16c": STORE_F16C}
@@ -31,7 +32,7 @@ EXPORTS = (
     "flb_set_force", "flb_share_velocity", "flb_add_buoyancy", "flb_clear_buoyancy",
     "flb_upload_flags", "flb_set_field_rho", "flb_set_field_u", "flb_upload_pops_f32",
     "flb_upload_pops_f64", "flb_download_pops_f32", "flb_download_pops_f64", "flb_init_equilibrium",
-    "flb_apply_cell_dynamics", "flb_stream", "flb_set_level", "flb_apply_halfway_bc", "flb_step", "flb_moments",
+    "flb_apply_cell_dynamics", "flb_stream", "flb_set_level", "flb_set_watchdog", "flb_apply_halfway_bc", "flb_step", "flb_moments",
     "flb_reduce_mass_momentum", "flb_batch_create", "flb_batch_destroy", "flb_batch_info",
     "flb_batch_set_launch", "flb_batch_step", "flb_batch_capture", "flb_batch_replay",
     "flb_batch_synchronize", "flb_batch_autotune", "flb_batch_set_halo", "flb_set_ghost_overlap",
@@ -135,6 +136,7 @@ def lib() -> C.CDLL:
         "flb_apply_cell_dynamics": [P, C.c_int64],
         "flb_stream": [P, C.c_int64],
         "flb_set_level": [P, C.c_int, C.c_int],
+        "flb_set_watchdog": [P, C.c_int],
         "flb_apply_halfway_bc": [P],
         "flb_step": [P, C.c_int],
         "flb_moments": [P, C.c_uint8, P],

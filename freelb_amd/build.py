@@ -70,7 +70,9 @@ def build(verbose: bool = False, jobs: int = 0, extra_flags=(), lib: str = LIB,
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
-    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs]
+    # roctx: named ranges around the host calls (flbh::Range)
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs, "-L/opt/rocm/lib",
+            "-lrocprofiler-sdk-roctx"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)

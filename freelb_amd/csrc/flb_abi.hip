@@ -88,13 +88,14 @@ hipError_t launch_moments(const flb_block_s* b, const KArgs& a, uint8_t mask, vo
   }
 }
 
-hipError_t launch_reduce(const flb_block_s* b, const KArgs& a, uint8_t mask, double* out) {
+// partials: one double4 per workgroup of the per-cell kernels (cell_blocks(N)).
+hipError_t launch_reduce(const flb_block_s* b, const KArgs& a, uint8_t mask, double* partials, double* out) {
   switch (b->desc.storage) {
-    case FLB_STORE_F32: return flb::launch_reduce_f32(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
-    case FLB_STORE_F16S: return flb::launch_reduce_f16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
-    case FLB_STORE_BF16S: return flb::launch_reduce_bf16s(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
-    case FLB_STORE_F16C: return flb::launch_reduce_f16c(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
-    default: return flb::launch_reduce_f64(b->desc.q, b->parity, a, mask, b->d_partials, out, b->stream);
+    case FLB_STORE_F32: return flb::launch_reduce_f32(b->desc.q, b->parity, a, mask, partials, out, b->stream);
+    case FLB_STORE_F16S: return flb::launch_reduce_f16s(b->desc.q, b->parity, a, mask, partials, out, b->stream);
+    case FLB_STORE_BF16S: return flb::launch_reduce_bf16s(b->desc.q, b->parity, a, mask, partials, out, b->stream);
+    case FLB_STORE_F16C: return flb::launch_reduce_f16c(b->desc.q, b->parity, a, mask, partials, out, b->stream);
+    default: return flb::launch_reduce_f64(b->desc.q, b->parity, a, mask, partials, out, b->stream);
   }
 }
 
@@ -1000,16 +1001,36 @@ int flb_apply_halfway_bc(flb_block b) {
   return FLB_OK;
 }
 
+int flb_set_watchdog(flb_block b, int every) {
+  CHECK_BLOCK(b);
+  if (every < 0) return fail(FLB_ERR_ARG, "negative watchdog interval");
+  b->watchdog = every;
+  return FLB_OK;
+}
+
+static int watchdog_check(flb_block b) {
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  const int rc = flb_reduce_mass_momentum(b, 0xFF, m);
+  if (rc) return rc;
+  for (double v : m)
+    if (!std::isfinite(v))
+      return fail(FLB_ERR_NONFINITE, "non-finite mass / momentum (%g, %g, %g, %g) after step %lld", m[0], m[1],
+                  m[2], m[3], static_cast<long long>(b->step));
+  return FLB_OK;
+}
+
 int flb_step(flb_block b, int n) {
   CHECK_BLOCK(b);
   if (n < 0) return fail(FLB_ERR_ARG, "negative step count");
   if (b->pending) return fail(FLB_ERR_STATE, "step while a Stream is pending");
   DeviceGuard g(b->device);
+  Range range("flb_step");
   for (int s = 0; s < n; ++s) {
-    const int rc = launch_one(b);
+    int rc = launch_one(b);
     if (rc) return rc;
     b->parity ^= 1;
     b->step += 1;
+    if (b->watchdog > 0 && b->step % b->watchdog == 0 && (rc = watchdog_check(b))) return rc;
   }
   return FLB_OK;
 }
@@ -1042,7 +1063,7 @@ int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]) {
     if (hipMalloc(reinterpret_cast<void**>(&b->d_partials), nb * 4 * sizeof(double)) != hipSuccess)
       return fail(FLB_ERR_NOMEM, "reduction scratch allocation failed");
   }
-  HIP_TRY(launch_reduce(b, a, mask, b->d_red));
+  HIP_TRY(launch_reduce(b, a, mask, b->d_partials, b->d_red));
   HIP_TRY(hipMemcpyAsync(out, b->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return FLB_OK;

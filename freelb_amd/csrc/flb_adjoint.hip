@@ -1047,6 +1047,7 @@ int flb_adjoint_forward(flb_adjoint h, int steps, int checkpoint_every, double* 
   if (steps < 1) return fail(FLB_ERR_ARG, "forward needs at least one step");
   if (checkpoint_every < -1) return fail(FLB_ERR_ARG, "checkpoint_every must be -1, 0 or positive");
   DeviceGuard g(h->device);
+  Range range("flb_adjoint_forward");
   return h->real == 8 ? forward_impl<double>(h, steps, checkpoint_every, J)
                       : forward_impl<float>(h, steps, checkpoint_every, J);
 }
@@ -1056,6 +1057,7 @@ int flb_adjoint_backward(flb_adjoint h, double* grad) {
   if (!grad) return fail(FLB_ERR_ARG, "null gradient output");
   if (!h->recorded) return fail(FLB_ERR_STATE, "no recorded forward run (forward with checkpoint_every >= 0 first)");
   DeviceGuard g(h->device);
+  Range range("flb_adjoint_backward");
   return h->real == 8 ? backward_impl<double>(h, grad) : backward_impl<float>(h, grad);
 }
 
@@ -1139,6 +1141,7 @@ int flb_adjoint_batch_forward(flb_adjoint_batch b, int steps, int checkpoint_eve
   if (steps < 1) return fail(FLB_ERR_ARG, "forward needs at least one step");
   if (checkpoint_every < -1) return fail(FLB_ERR_ARG, "checkpoint_every must be -1, 0 or positive");
   DeviceGuard g(b->device);
+  Range range("flb_adjoint_batch_forward");
   return b->real == 8 ? batch_forward_impl<double>(b, steps, checkpoint_every, J)
                       : batch_forward_impl<float>(b, steps, checkpoint_every, J);
 }
@@ -1148,6 +1151,7 @@ int flb_adjoint_batch_backward(flb_adjoint_batch b, double* grad) {
   if (!grad) return fail(FLB_ERR_ARG, "null gradient output");
   if (!b->recorded) return fail(FLB_ERR_STATE, "no recorded batch forward run (checkpoint_every >= 0 first)");
   DeviceGuard g(b->device);
+  Range range("flb_adjoint_batch_backward");
   return b->real == 8 ? batch_backward_impl<double>(b, grad) : batch_backward_impl<float>(b, grad);
 }
 

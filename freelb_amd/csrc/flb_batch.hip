@@ -271,6 +271,7 @@ int flb_batch_step(flb_batch t, int n) {
   CHECK_BATCH(t);
   if (n < 0) return fail(FLB_ERR_ARG, "negative step count");
   DeviceGuard g(t->device);
+  Range range("flb_batch_step");
   int parity = 0;
   int rc = check_state(t, &parity);
   if (rc) return rc;
@@ -344,6 +345,7 @@ int flb_batch_replay(flb_batch t) {
   CHECK_BATCH(t);
   if (!t->exec) return fail(FLB_ERR_STATE, "no captured graph");
   DeviceGuard g(t->device);
+  Range range("flb_batch_replay");
   int parity = 0;
   int rc = check_state(t, &parity);
   if (rc) return rc;

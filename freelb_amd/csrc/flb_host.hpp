@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <string>
@@ -47,6 +48,7 @@ struct flb_block_s {
   // refinement level of the block and the hierarchy's finest level: the
   // count-gated calls step the block when count % 2^(max_level - level) == 0
   int level = 0, max_level = 0;
+  int watchdog = 0;  // flb_step checks mass / momentum every `watchdog` steps (0: off)
   int vec = 0;  // step kernel form: 0 automatic cell form, -1 row segments, -3 linear runs,
                 // 1/2/4 row form with that many cells per thread
   int wpb = 4;   // wavefronts per workgroup
@@ -99,6 +101,15 @@ int fail(int code, const char* fmt, ...);
     if (e_ != hipSuccess)                                                             \
       return fail(FLB_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
   } while (0)
+
+// A roctx range over a host call (its kernels and waits show under it in
+// rocprofv3 traces with --marker-trace; a no-op without a profiler).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 struct DeviceGuard {
   int prev = -1;

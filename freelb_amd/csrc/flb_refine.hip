@@ -561,6 +561,7 @@ int flb_refine_run(flb_refine r, int64_t count0, int n) {
     return fail(FLB_ERR_STATE, "flb_refine_run steps a two-level pair; a deeper hierarchy steps its blocks by count");
   if (r->coarse->pending || r->fine->pending) return fail(FLB_ERR_STATE, "step while a Stream is pending");
   DeviceGuard g(r->device);
+  Range range("flb_refine_run");
   int rc = FLB_OK;
   for (int64_t count = count0; count < count0 + n; ++count) {
     if ((rc = flb_step(r->fine, 1))) return rc;

@@ -250,6 +250,11 @@ class BlockLattice:
     def step(self, n: int = 1) -> None:
         abi.check(abi.lib().flb_step(self._h, int(n)))
 
+    def set_watchdog(self, every: int) -> None:
+        """step() checks the FP64 mass / momentum every `every` steps and
+        raises FLBError (FLB_ERR_NONFINITE) on a NaN / Inf; 0 switches it off."""
+        abi.check(abi.lib().flb_set_watchdog(self._h, int(every)))
+
     # ---- read-out -------------------------------------------------------------
     def moments(self, mask: int = 0xFE) -> np.ndarray:
         """rho, u of cells with flag & mask (moment::rhoU<CELL, true>); [d+1][N]."""

@@ -67,7 +67,8 @@ enum {
   FLB_ERR_HIP = 3,         /* a HIP runtime call failed */
   FLB_ERR_UNSUPPORTED = 4, /* combination not built (e.g. MRT on a storage kind without it) */
   FLB_ERR_LAYOUT = 5,      /* a cell on the outer layer of a non-periodic axis matches a task */
-  FLB_ERR_NOMEM = 6        /* device allocation failed */
+  FLB_ERR_NOMEM = 6,       /* device allocation failed */
+  FLB_ERR_NONFINITE = 7    /* the watchdog found a non-finite mass or momentum (flb_set_watchdog) */
 };
 
 /* ---- enums ------------------------------------------------------------- */
@@ -281,6 +282,12 @@ int flb_set_level(flb_block b, int level, int max_level);
 int flb_apply_halfway_bc(flb_block b);
 /* n complete steps (apply + stream) back to back. */
 int flb_step(flb_block b, int n);
+/* NaN / Inf watchdog of flb_step: after every `every` steps of the block
+ * (counted by its step counter), the FP64 mass / momentum reduction over the
+ * updated cells runs and flb_step returns FLB_ERR_NONFINITE, naming the step,
+ * when any of the four sums is not finite. 0 (the default) switches it off.
+ * The reference has none: it exits only on CUDA errors (utils/cuda_device.h:36-44). */
+int flb_set_watchdog(flb_block b, int every);
 
 /* ---- read-out ------------------------------------------------------------ */
 /* rho and u of cells with (flag & mask) != 0, 0 elsewhere, into host

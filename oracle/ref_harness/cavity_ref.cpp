@@ -9,7 +9,8 @@
 //
 // Usage:
 //   cavity_ref <d3q19|d2q9> <fw|hw|hwfield|mrt|smag|force> Ni Nj Nk steps RT KineVisc Uwall out.bin [Cs]
-// (Nk is ignored for d2q9.)
+// (Nk is ignored for d2q9.) In the hw mode, out = "time:W" runs W untimed
+// steps, then times `steps` steps and prints their MLUPS instead of dumping.
 //
 // Modes
 //   fw      : wall cells collide as BounceBack / BounceBackMovingWall
@@ -40,6 +41,7 @@
 #include "freelb.h"
 #include "freelb.hh"
 
+#include <chrono>
 #include <cstdio>
 #include <type_traits>
 #include <cstdlib>
@@ -63,6 +65,7 @@ struct Params {
   int Ni, Nj, Nk, steps;
   T RT, visc, uwall, cs = T(0);
   std::string out;
+  int time_warmup = -1;  // >= 0: out was "time:<warmup>", time the steps instead of dumping
 };
 
 template <typename BLM, typename FM, typename CELL, typename LatSet>
@@ -206,11 +209,23 @@ int run(const Params& p) {
       using AllTask =
         tmp::Key_TypePair<std::uint8_t(AABBFlag | BouncebackFlag | BBMovingWallFlag), BGKrhoU>;
       using NSTask = TaskSelector<std::uint8_t, CELL, AllTask>;
-      for (int s = 0; s < p.steps; ++s) {
+      auto step = [&]() {
         NSLattice.template ApplyCellDynamics<NSTask>(FlagFM);
         NSLattice.Stream();
         BM.Apply();
+      };
+      if (p.time_warmup >= 0) {
+        // timing mode: warm-up steps, then the timed steps, no dump
+        for (int s = 0; s < p.time_warmup; ++s) step();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int s = 0; s < p.steps; ++s) step();
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double cells = double(p.Ni) * p.Nj * (D == 3 ? p.Nk : 1);
+        std::printf("MLUPS %.3f cells %.0f steps %d seconds %.6f threads %d\n", cells * p.steps / sec / 1e6, cells,
+                    p.steps, sec, Thread_Num);
+        std::exit(0);
       }
+      for (int s = 0; s < p.steps; ++s) step();
     } else if (p.mode == "force") {
       if constexpr (FORCE) {
         using ForceS = force::ConstForce<CELL>;
@@ -279,6 +294,7 @@ int main(int argc, char** argv) {
   p.visc = std::strtof(argv[8], nullptr);
   p.uwall = std::strtof(argv[9], nullptr);
   p.out = argv[10];
+  if (p.out.rfind("time:", 0) == 0) p.time_warmup = std::atoi(p.out.c_str() + 5);
   if (argc == 12) p.cs = std::strtof(argv[11], nullptr);
 #ifdef _OPENMP
   Thread_Num = omp_get_max_threads();

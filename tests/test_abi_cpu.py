@@ -72,6 +72,7 @@ def test_null_arguments():
     for fn in ("flb_apply_cell_dynamics", "flb_stream"):
         assert getattr(L, fn)(None, 0) == abi.FLB_ERR_ARG
     assert L.flb_set_level(None, 0, 0) == abi.FLB_ERR_ARG
+    assert L.flb_set_watchdog(None, 1) == abi.FLB_ERR_ARG
     for fn in ("flb_refine_average", "flb_refine_interpolate", "flb_refine_communicate"):
         assert getattr(L, fn)(None, 0) == abi.FLB_ERR_ARG
     assert L.flb_refine_set_levels(None, 0, 1) == abi.FLB_ERR_ARG

@@ -103,3 +103,28 @@ def test_overlap_wider_than_one(pad, mode, dims):
     inner = 2 if mode == "fw" else 0
     assert sel.sum() == np.prod([v - inner for v in dims if v > 0])
     assert np.array_equal(g[:, sel], ref[:, sel]), f"max |diff| {np.abs(g[:, sel] - ref[:, sel]).max()}"
+
+
+def test_watchdog_stops_on_a_nan_population():
+    """The NaN / Inf watchdog: a NaN planted in one fluid population from the
+    host makes flb_step fail with FLB_ERR_NONFINITE at the first check; a
+    healthy run passes every check."""
+    from freelb_amd import FLBError, abi
+    case, p0 = O.cavity_case("hw", 10, 9, 8)
+    lat = gpu_lattice(case, pops=p0)
+    try:
+        lat.set_watchdog(3)
+        lat.step(12)
+        p = lat.pops()
+        cell = int(np.flatnonzero(case.flags == O.AABB)[17])
+        p[5, cell] = np.nan
+        lat.upload_pops(p)
+        with pytest.raises(FLBError) as e:
+            lat.step(6)
+        assert e.value.code == abi.FLB_ERR_NONFINITE
+        assert "non-finite" in str(e.value)
+        assert lat.info().step == 15
+        lat.set_watchdog(0)
+        lat.step(2)  # switched off: no check
+    finally:
+        lat.close()
