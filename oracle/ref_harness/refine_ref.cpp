@@ -16,8 +16,10 @@
 // movingwall rebuild the wall links.
 //
 // Usage:
-//   refine_ref <d3q19|d2q9> Ni Nj Nk blockcell p0 p1 steps RT KineVisc Uwall out.bin
-// (Nk is ignored for d2q9.)
+//   refine_ref <d3q19|d2q9> Ni Nj Nk blockcell p0 p1 steps RT KineVisc Uwall out.bin [q0 q1]
+// (Nk is ignored for d2q9.) With q0 q1, the block cells whose index lies in
+// [q0, q1) on every axis are refined twice (a third level, BasicBlock::refine(2),
+// geometry/basic_geometry.hh:293-307), the others of [p0, p1) once.
 //
 // Output layout (little endian):
 //   char[8]  "FLBREF02"
@@ -53,6 +55,7 @@ constexpr std::uint8_t BBMovingWallFlag = std::uint8_t(8);
 struct Params {
   std::string lattice;
   int Ni, Nj, Nk, cell, p0, p1, steps;
+  int q0 = 0, q1 = 0;  // doubly refined block cells (none when q0 == q1)
   T RT, visc, uwall;
   std::string out;
 };
@@ -64,12 +67,14 @@ int run(const Params& p, Helper& GeoHelper, const BoxT& cavity, const BoxT& topl
 
   GeoHelper.forEachBlockCell([&](BasicBlock<T, D>& block) {
     const auto idx = block.getIdxBlock().getMin();
-    bool in = true;
+    bool in = true, in2 = p.q0 < p.q1;
     for (int a = 0; a < D; ++a) {
       const int c = (idx[a] - 1) / p.cell;  // the helper's base block starts at index 1
       in = in && c >= p.p0 && c < p.p1;
+      in2 = in2 && c >= p.q0 && c < p.q1;
     }
-    if (in) block.refine();
+    if (in2) block.refine(std::uint8_t(2));
+    else if (in) block.refine();
   });
   GeoHelper.CreateBlocks();
   GeoHelper.LoadBalancing(1);
@@ -169,8 +174,8 @@ int run(const Params& p, Helper& GeoHelper, const BoxT& cavity, const BoxT& topl
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 13) {
-    std::fprintf(stderr, "usage: %s <d3q19|d2q9> Ni Nj Nk blockcell p0 p1 steps RT KineVisc Uwall out\n",
+  if (argc != 13 && argc != 15) {
+    std::fprintf(stderr, "usage: %s <d3q19|d2q9> Ni Nj Nk blockcell p0 p1 steps RT KineVisc Uwall out [q0 q1]\n",
                  argv[0]);
     return 2;
   }
@@ -187,6 +192,10 @@ int main(int argc, char** argv) {
   p.visc = std::strtof(argv[10], nullptr);
   p.uwall = std::strtof(argv[11], nullptr);
   p.out = argv[12];
+  if (argc == 15) {
+    p.q0 = std::atoi(argv[13]);
+    p.q1 = std::atoi(argv[14]);
+  }
   if (p.lattice == "d3q19") {
     AABB<T, 3> cavity(Vector<T, 3>{}, Vector<T, 3>(T(p.Ni), T(p.Nj), T(p.Nk)));
     AABB<T, 3> toplid(Vector<T, 3>(T(1), T(1), T(p.Nk - 1)), Vector<T, 3>(T(p.Ni - 1), T(p.Nj - 1), T(p.Nk)));

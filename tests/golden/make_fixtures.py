@@ -46,6 +46,9 @@ RT, VISC, U = "0.53", "0.01", "0.1"
 # (lattice, Ni=Nj(=Nk), blockcell, p0, p1, counts); FreeLB's 3D refinement builder
 # writes past its fine block, so only D2Q9 is pinned this way
 REFINED = [("d2q9", 16, 4, 1, 3, 41), ("d2q9", 24, 4, 2, 4, 34)]
+# three levels: block cells in [p0, p1) refined once, those in [q0, q1) twice
+# (lattice, Ni, blockcell, p0, p1, q0, q1, counts)
+REFINED3 = [("d2q9", 24, 4, 1, 5, 2, 4, 37), ("d2q9", 32, 4, 1, 7, 3, 5, 44)]
 REF_RT, REF_VISC, REF_U = "0.6", "0.01", "0.05"
 EXE_REF = os.path.join(ROOT, "oracle", "_ref", "refine_ref")
 # FreeLB's own VTK output (oracle/_ref/vtk_ref, vtmo::vtmWriter) of the halfway cavity after the
@@ -141,11 +144,12 @@ def read_natconv(path):
     return out
 
 
-def run_refined(lat, n, cell, p0, p1, counts, tmp):
+def run_refined(lat, n, cell, p0, p1, counts, tmp, q0=None, q1=None):
     out = os.path.join(tmp, f"refine_{lat}_{n}_{cell}_{p0}_{p1}_{counts}.bin")
     nk = n if lat == "d3q19" else 1
+    extra = [] if q0 is None else [str(q0), str(q1)]
     subprocess.run([EXE_REF, lat, str(n), str(n), str(nk), str(cell), str(p0), str(p1), str(counts), REF_RT,
-                    REF_VISC, REF_U, out], check=True, capture_output=True)
+                    REF_VISC, REF_U, out, *extra], check=True, capture_output=True)
     return read_refined(out)
 
 
@@ -251,6 +255,16 @@ def main():
                         arrays[f"b{k}_{key}"] = np.asarray(blk[key])
                 arrays["nblocks"] = np.array(len(r["blocks"]))
                 name = f"refined_{lat}_{n}_cell{cell}_p{p0}{p1}_{counts}.npz"
+                np.savez_compressed(os.path.join(HERE, name), **arrays)
+                print("wrote", name, flush=True)
+            for lat, n, cell, p0, p1, q0, q1, counts in REFINED3:
+                r = run_refined(lat, n, cell, p0, p1, counts, tmp, q0, q1)
+                arrays = {"case": np.array([n, cell, p0, p1, counts, q0, q1]), "uw": r["uw"]}
+                for k, blk in enumerate(r["blocks"]):
+                    for key in ("level", "overlap", "dims", "omega", "min", "flags", "pops", "fields"):
+                        arrays[f"b{k}_{key}"] = np.asarray(blk[key])
+                arrays["nblocks"] = np.array(len(r["blocks"]))
+                name = f"refined3_{lat}_{n}_cell{cell}_p{p0}{p1}_q{q0}{q1}_{counts}.npz"
                 np.savez_compressed(os.path.join(HERE, name), **arrays)
                 print("wrote", name, flush=True)
             if only:

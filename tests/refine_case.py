@@ -18,6 +18,22 @@ def load_fixture(path):
     return dict(n=n, cell=cell, p0=p0, p1=p1, counts=counts, blocks=blocks)
 
 
+def load_fixture3(path):
+    """A three-level FreeLB refined run and the oracle hierarchy of the same
+    case: level-1 patch over block cells [p0, p1), level-2 patch over
+    [q0, q1) (both per axis, `cell` coarse cells per block cell)."""
+    z = np.load(path)
+    n, cell, p0, p1, counts, q0, q1 = (int(v) for v in z["case"])
+    blocks = []
+    for k in range(int(z["nblocks"])):
+        blocks.append({key: z[f"b{k}_{key}"] for key in ("level", "overlap", "dims", "omega", "min", "flags",
+                                                          "pops", "fields")})
+    o1, m1 = 1 + p0 * cell, (p1 - p0) * cell
+    o2, m2 = 2 + 2 * ((1 + q0 * cell) - o1), 2 * (q1 - q0) * cell
+    patches = [(0, (o1, o1), m1), (1, (o2, o2), m2)]
+    return dict(n=n, counts=counts, patches=patches, blocks=blocks)
+
+
 def fixture_mismatches(rc, fx, fine_mask=None):
     """Cells where the oracle's state differs from FreeLB's: every non-void
     cell of each level-0 block (base and overlap, which mirror the coarse
@@ -129,3 +145,88 @@ def hierarchy_masks(rh):
             m &= inner.reshape(-1)
         out.append(m)
     return out
+
+
+def hierarchy_fixture_mismatches(rh, fx):
+    """Cells where the oracle hierarchy's state differs from FreeLB's refined
+    run with several levels. Each FreeLB block of level L is placed on the
+    hierarchy's level-L block by its min corner (FreeLB's level-L blocks
+    tile that block). Compared: every non-void cell of a FreeLB block that no
+    other FreeLB block of its level holds as a base cell (its own base
+    cells, and overlap cells facing a coarser level or the domain edge);
+    overlap cells that mirror another block's base cells are compared in
+    that block. Of an overlap cell that no block owns (it faces a coarser or
+    finer level), only the populations that stream into the block's base
+    cells are compared: FreeLB's communication fills those and leaves the
+    others (which stream into overlap cells) stale, and two blocks holding
+    such a cell then disagree on them. Cells the hierarchy marks COVERED by a
+    finer patch are not compared (FreeLB's overlap cells there hold
+    averages). Populations, and the RHO / VELOCITY fields on fluid cells."""
+    d, q = rh.d, rh.q
+    mins, vox = [np.array([-1.0] * d)], [1.0]
+    for pa, ch, o in rh.patches:
+        v = vox[pa] / 2
+        mins.append(mins[pa] + vox[pa] * np.array(o[:d], dtype=float) - 2 * v)
+        vox.append(v)
+
+    def place(b):
+        L = int(b["level"])
+        nx, ny, nz = (int(v) for v in b["dims"])
+        for k in range(len(rh.blocks)):
+            if rh.levels[k] != L:
+                continue
+            off = np.rint((np.asarray(b["min"][:d], dtype=float) - mins[k]) / vox[k]).astype(int)
+            if all(0 <= off[a] and off[a] + (nx, ny, nz)[a] <= rh.dims[k][a] for a in range(d)):
+                z0 = int(off[2]) if d == 3 else 0
+                return k, (slice(z0, z0 + nz), slice(int(off[1]), int(off[1]) + ny),
+                           slice(int(off[0]), int(off[0]) + nx))
+        raise AssertionError(f"FreeLB block of level {L} at {b['min']} has no place in the hierarchy")
+
+    placed = [place(b) for b in fx["blocks"]]
+    # base cells of every FreeLB block, per hierarchy block
+    owner = [np.full(rh.dims[k][::-1], -1, dtype=int) for k in range(len(rh.blocks))]
+    for j, (b, (k, sl)) in enumerate(zip(fx["blocks"], placed)):
+        ov = int(b["overlap"])
+        base = np.zeros(tuple(s.stop - s.start for s in sl), dtype=bool)
+        inner = tuple(slice(ov, n - ov) if n > 1 else slice(None) for n in base.shape)
+        base[inner] = True
+        owner[k][sl][base] = j
+    c = _velocities(q)
+    bad = checked = 0
+    for j, (b, (k, sl)) in enumerate(zip(fx["blocks"], placed)):
+        nx, ny, nz = (int(v) for v in b["dims"])
+        dims = rh.dims[k]
+        ov = int(b["overlap"])
+        P = b["pops"].reshape((q, nz, ny, nx))
+        Fd = b["fields"].reshape((d + 1, nz, ny, nx))
+        fl = b["flags"].reshape((nz, ny, nx))
+        mine = rh.pops[k].reshape((q,) + dims[::-1])[(slice(None),) + sl]
+        mf = rh.blocks[k].fields().astype(np.float32).reshape((d + 1,) + dims[::-1])[(slice(None),) + sl]
+        mfl = rh.blocks[k].flags.reshape(dims[::-1])[sl]
+        own = owner[k][sl]
+        live = ((fl & O.VOID) == 0) & (mfl != O.COVERED)
+        base = np.zeros((nz, ny, nx), dtype=bool)
+        base[tuple(slice(ov, n - ov) if n > 1 else slice(None) for n in (nz, ny, nx))] = True
+        # per direction: the populations compared in each cell
+        zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        for i in range(q):
+            tx, ty, tz = xx + c[i][0], yy + c[i][1], zz + (c[i][2] if d == 3 else 0)
+            inside = (tx >= 0) & (tx < nx) & (ty >= 0) & (ty < ny) & (tz >= 0) & (tz < nz)
+            feeds = np.zeros_like(base)
+            feeds[inside] = base[tz[inside], ty[inside], tx[inside]]
+            use = live & ((own == j) | ((own < 0) & (base | feeds)))
+            bad += int(((mine[i] != P[i]) & use).sum())
+            checked += int(use.sum())
+        fuse = live & (own == j) & (fl == O.AABB)
+        bad += int((np.any(mf != Fd, axis=0) & fuse).sum())
+    return bad, checked
+
+
+def _velocities(q):
+    """FreeLB's D2Q9 / D3Q19 velocity order (lbm/lattice_set.h:168-194, 259-296)."""
+    if q == 9:
+        return [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (1, 1, 0), (-1, -1, 0), (1, -1, 0),
+                (-1, 1, 0)]
+    return [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0),
+            (-1, -1, 0), (1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, -1, -1), (1, -1, 0), (-1, 1, 0), (1, 0, -1),
+            (-1, 0, 1), (0, 1, -1), (0, -1, 1)]

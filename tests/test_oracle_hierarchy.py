@@ -1,12 +1,40 @@
 """The oracle's multi-level refined hierarchy (RefinedHierarchy): with one
 patch it is the two-level refined cavity that FreeLB's own refined fixtures
-pin, bit for bit; with several patches and levels the rest state stays a
-fixed point on every level, and the level gating steps each block
-2^(level) times per coarse step."""
+pin, bit for bit; with three levels it reproduces FreeLB's own three-level
+refined runs (fixtures written by FreeLB, BasicBlock::refine(2)) bit for
+bit; the rest state stays a fixed point on every level, and the level
+gating steps each block 2^(level) times per coarse step."""
+import glob
+import os
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tests.refine_case import hierarchy_fixture_mismatches, load_fixture, load_fixture3
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "refined3_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_three_levels_match_freelb(path):
+    fx = load_fixture3(path)
+    rh = O.RefinedHierarchy("d2q9", fx["n"], fx["n"], 0, fx["patches"])
+    rh.run(fx["counts"])
+    bad, checked = hierarchy_fixture_mismatches(rh, fx)
+    assert checked > 10000 and bad == 0, (bad, checked)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "refined_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_two_level_fixtures_through_the_hierarchy(path):
+    fx = load_fixture(path)
+    o, m = 1 + fx["p0"] * fx["cell"], (fx["p1"] - fx["p0"]) * fx["cell"]
+    rh = O.RefinedHierarchy("d2q9", fx["n"], fx["n"], 0, [(0, (o, o), m)])
+    rh.run(fx["counts"])
+    bad, checked = hierarchy_fixture_mismatches(rh, fx)
+    assert checked > 1000 and bad == 0, (bad, checked)
 
 
 @pytest.mark.parametrize("lattice,n,cell,p0,p1", [("d2q9", 16, 4, 1, 3), ("d2q9", 24, 4, 2, 4),

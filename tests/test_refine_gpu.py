@@ -8,7 +8,8 @@ import pytest
 
 from freelb_amd import BlockLattice, Key_TypePair, Refinement
 from oracle import oracle as O
-from refine_case import compare_masks, fixture_mismatches, gpu_hierarchy, gpu_refined, hierarchy_masks, load_fixture
+from refine_case import (compare_masks, fixture_mismatches, gpu_hierarchy, gpu_refined, hierarchy_fixture_mismatches,
+                         hierarchy_masks, load_fixture, load_fixture3)
 
 pytestmark = pytest.mark.gpu
 
@@ -87,6 +88,30 @@ def test_refined_hierarchy_matches_oracle(lattice, n, patches, storage):
                 fl = rh.blocks[j].flags
                 live = m & (fl == O.AABB)
                 assert np.array_equal(b.download_fields()[:, live], rh.blocks[j].fields()[:, live]), f"fields {j}"
+    finally:
+        for r in refs:
+            r.close()
+        for b in blocks:
+            b.close()
+
+
+@pytest.mark.parametrize("name", ["refined3_d2q9_24_cell4_p15_q24_37.npz", "refined3_d2q9_32_cell4_p17_q35_44.npz"])
+def test_three_level_hierarchy_matches_freelb_fixture(name):
+    """Three levels on the GPU against FreeLB's own three-level run: the GPU
+    equals the oracle hierarchy bit for bit, and the oracle equals FreeLB's
+    blocks (the oracle is the map between the two block layouts)."""
+    import os
+    fx = load_fixture3(os.path.join(os.path.dirname(__file__), "golden", name))
+    rh = O.RefinedHierarchy("d2q9", fx["n"], fx["n"], 0, fx["patches"])
+    blocks, refs, mgr = gpu_hierarchy(rh)
+    masks = hierarchy_masks(rh)
+    try:
+        rh.run(fx["counts"])
+        mgr.run(fx["counts"])
+        for j, (b, m) in enumerate(zip(blocks, masks)):
+            assert np.array_equal(b.pops()[:, m], rh.pops[j][:, m]), f"block {j}"
+        bad, checked = hierarchy_fixture_mismatches(rh, fx)
+        assert checked > 10000 and bad == 0
     finally:
         for r in refs:
             r.close()
