@@ -204,10 +204,13 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
                 m.replay()
                 graph_ms = timer.stop() / graph_steps
             m.synchronize()
-            mass = [b.mass_momentum()[0] for b in blocks[:4]]
+            mass = np.array([b.mass_momentum()[0] for b in blocks])
     finally:
         for b in blocks:
             b.close()
+    # blocks pairing tau near 0.51 with a lid near 0.1 are unstable under BGK
+    # (the oracle diverges on them too); the rate counts every block's cells
+    diverged = [k for k in range(nblocks) if not np.isfinite(mass[k])]
     bytes_per = 152 if storage == "f32" else 76
     how = f"{graph_steps} eager batch launches" if eager else f"hipGraph of {graph_steps} steps"
     return {"workload": f"{nblocks} cavities, edges U{{16..96}}^3 (seed {seed}), tau U[0.51,0.9], "
@@ -218,7 +221,8 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
             "achieved_gbs": round(cells * bytes_per / (graph_ms / 1e3) / 1e9, 1),
             "per_block_launch_mlups": round(cells / (loop_ms / 1e3) / 1e6, 1),
             "launch": tuned, "setup_s": round(setup_s, 1),
-            "finite": bool(np.isfinite(mass).all())}
+            "finite_blocks": nblocks - len(diverged),
+            "diverged_blocks": [(k, round(specs[k]["tau"], 4), round(specs[k]["u_lid"], 3)) for k in diverged]}
 
 
 # algorithmic HBM bytes per cell and step of the FP32 adjoint kernels: the

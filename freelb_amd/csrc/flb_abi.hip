@@ -209,6 +209,21 @@ KArgs make_args(const flb_block_s* b) {
   if (!a.links) a.bcany = 0;
   a.omega = b->desc.omega;
   for (int j = 0; j < 19; ++j) a.rates[j] = b->rates[j];
+  a.mrt_shear = 0u;
+  if (b->desc.q == 19) {
+    for (int j = 0; j < 19; ++j) {
+      const double r = b->rates[j];
+      if (!(r >= 0.0)) {
+        a.mrt_shear |= 1u << j;
+        a.mrt_sn_f[j] = 0.0f;
+        a.mrt_sn_d[j] = 0.0;
+      } else {
+        const double sn = r / double(flb::m19_norm(j));
+        a.mrt_sn_f[j] = float(sn);
+        a.mrt_sn_d[j] = sn;
+      }
+    }
+  }
   for (int d = 0; d < 3; ++d) a.force[d] = b->force[d];
   const flb_block_s* v = b->vel_src;
   a.u_ext = v && v->d_u ? static_cast<char*>(v->d_u) + flb::kFrontPad * v->real : nullptr;

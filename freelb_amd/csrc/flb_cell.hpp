@@ -514,8 +514,10 @@ __device__ __forceinline__ R smag_omega(const R (&c)[19], R rho, const R (&u)[3]
                rho * (u[2] * u[2] + cs2);
   const R ns = t0 * t0 + t1 * t1 * R(2) + t2 * t2 * R(2) + t3 * t3 + t4 * t4 * R(2) + t5 * t5;
   const R norm = dsqrt(ns);
-  const R coeff =
-      R(double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951 / double(rho_f));
+  // with the usual RHO field of 1 the double division is by 1 and is skipped
+  // (the same bits)
+  const double cd = double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951;
+  const R coeff = rho_f == R(1) ? R(cd) : R(cd / double(rho_f));
   const R rt_mol = R(1) / omega;
   const R rt_tur = R(0.5) * (dsqrt(rt_mol * rt_mol + coeff * norm) - rt_mol);
   return R(1) / (rt_mol + rt_tur);
@@ -595,9 +597,17 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// rates[j] < 0 or NaN marks a shear rate, which takes om
+// The relaxation factor of row j, s_j / |M_j|^2. For a fixed rate it is
+// formed once per block on the host (the rate over the norm in double,
+// rounded to R: sn[j]); a shear row (bit j of `shear`) takes the cell's
+// omega times the rounded reciprocal of its norm.
 template <class R>
-__device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, const double* rates) {
+__device__ __forceinline__ R mrt_factor(int j, const R* sn, uint32_t shear, R om) {
+  return ((shear >> j) & 1u) ? om * R(1.0 / double(m19_norm(j))) : sn[j];
+}
+
+template <class R>
+__device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, const R* sn, uint32_t shear) {
   R e[19];
   feq(e, rho, u);
   R g[19];
@@ -606,14 +616,12 @@ __device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, co
   R k[19];
   static_for<19>([&](auto J) {
     constexpr int j = decltype(J)::value;
-    const double sr = rates[j];
-    const R s = (sr >= 0.0) ? R(sr) : om;
     R dm = R(0);
     static_for<19>([&](auto I) {
       constexpr int i = decltype(I)::value;
       if constexpr (m19(j, i) != 0) dm += R(m19(j, i)) * g[i];
     });
-    k[j] = s * dm / R(m19_norm(j));
+    k[j] = dm * mrt_factor<R>(j, sn, shear, om);
   });
   static_for<19>([&](auto I) {
     constexpr int i = decltype(I)::value;

@@ -71,6 +71,9 @@ struct KArgs {
   uint32_t pop_bytes;    // bytes of the population allocation when below 4 GiB, else 0
   double omega;
   double rates[19];      // MRT rates; < 0 or NaN marks the shear rates
+  float mrt_sn_f[19];    // D3Q19 MRT: rate / |M_j|^2 of the fixed-rate rows (FP32 kinds)
+  double mrt_sn_d[19];   // the same in FP64
+  uint32_t mrt_shear;    // D3Q19 MRT: bit j set for the rows that take omega
   double force[3];       // constant body-force density of DYN_BGK_FORCE
   // advection-diffusion dynamics: slot 0 of the VELOCITY field that advects
   // the scalar (the coupled flow block's), or null for the block's own u_f

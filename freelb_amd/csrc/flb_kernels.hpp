@@ -256,7 +256,14 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
         if (dyn == DYN_SMAG_BGK) {
           bgk_with<Q>(f, rho, u, eo, R(1) - eo);
         } else {
-          mrt(f, rho, u, eo, a.rates);
+          if constexpr (Q == 19) {
+            const R* sn;
+            if constexpr (sizeof(R) == 8) sn = a.mrt_sn_d;
+            else sn = a.mrt_sn_f;
+            mrt(f, rho, u, eo, sn, a.mrt_shear);
+          } else {
+            mrt(f, rho, u, eo, a.rates);
+          }
         }
       }
       break;
@@ -351,6 +358,13 @@ __device__ __forceinline__ void store_row(typename S::E* p, int i, const typenam
 #ifndef FLB_NT_EVEN
 #define FLB_NT_EVEN 1
 #endif
+// In blocks with wide rows, the odd step's directions with c_x = 0 read and
+// write whole aligned rows that no other wavefront touches: non-temporal
+// there too (FLB_NT_ODD_ALIGNED). Its x-shifted directions share their edge
+// lines with the neighbouring wavefront and stay cached.
+#ifndef FLB_NT_ODD_ALIGNED
+#define FLB_NT_ODD_ALIGNED 1
+#endif
 #ifndef FLB_NT_ODD_STORE
 #define FLB_NT_ODD_STORE 0
 #endif
@@ -404,6 +418,108 @@ __device__ __forceinline__ void bst_pop(__amdgpu_buffer_rsrc_t r, uint32_t idx, 
 
 template <int Q, class S, int PAR, int FAM, bool PER, bool BUF = false>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z);
+
+// Buffer access with a wavefront-uniform scalar byte offset so beside the
+// lane's vector byte offset vo.
+template <int PAR, class E>
+__device__ __forceinline__ E bld_pop_s(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  constexpr int aux = (PAR == 0 && FLB_NT_EVEN) ? 2 : 0;
+  if constexpr (sizeof(E) == 8) {
+    const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, aux);
+    return __longlong_as_double(int64_t((uint64_t(w[1]) << 32) | uint64_t(w[0])));
+  } else if constexpr (sizeof(E) == 4) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo, so, aux));
+  } else {
+    return E(__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, aux));
+  }
+}
+template <int PAR, class E>
+__device__ __forceinline__ void bst_pop_s(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, E v) {
+  constexpr int aux = ((PAR == 0 && FLB_NT_EVEN) || (PAR == 1 && FLB_NT_ODD_STORE)) ? 2 : 0;
+  if constexpr (sizeof(E) == 8) {
+    typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+    const uint64_t b = uint64_t(__double_as_longlong(v));
+    const u32x2 w = {uint32_t(b), uint32_t(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, vo, so, aux);
+  } else if constexpr (sizeof(E) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, aux);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(uint16_t(v), r, vo, so, aux);
+  }
+}
+
+// Element offset of the neighbour x + c_k from x (no periodic wrap).
+template <int Q>
+__device__ __forceinline__ int32_t nb_rel(int k, int32_t yp, int32_t zp) {
+  return Lat<Q>::cx(k) + Lat<Q>::cy(k) * yp + Lat<Q>::cz(k) * zp;
+}
+
+// The odd step through the buffer descriptor (no periodic axis). Every
+// access is the lane's base vo = pidx + c0 - B plus a per-direction part: a
+// wavefront-uniform scalar offset m + B, where m is the smaller of the two
+// candidate slots (the neighbour's, or the cell's own across a wall link),
+// and a per-lane delta (candidate - m >= 0) chosen by the lane's link bit.
+// B = zpop + ypop + 1 (2D: ypop + 1) bounds the most negative neighbour offset, so every
+// term is non-negative (rows y, z >= 1 hold pidx >= B). Same selection and
+// arithmetic as the general path, bit for bit. NTA: the c_x = 0 directions
+// (whole aligned rows of wide blocks) load and store non-temporally.
+template <int Q, class S, int FAM, bool NTA>
+__device__ __forceinline__ void odd_buf(const KArgs& a, __amdgpu_buffer_rsrc_t rs, uint32_t pidx, int c0, int s0,
+                                        int dyn, float cs, unsigned ln, unsigned lm) {
+  using R = typename S::R;
+  using E = typename S::E;
+  constexpr uint32_t el = sizeof(E);
+  const int32_t dp = int32_t(a.dpop), yp = a.ypop, zp = int32_t(a.zpop);
+  const int32_t B = (Lat<Q>::D == 3 ? zp : 0) + yp + 1;
+  const uint32_t vo = pidx + uint32_t(c0) - uint32_t(B);
+  const unsigned lk = ln | lm;
+  R f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
+    const int o = opp(i);
+    const int32_t rn = o * dp + nb_rel<Q>(o, yp, zp), rw = i * dp;
+    const int32_t m = rn < rw ? rn : rw;
+    const bool wall = i != 0 && ((lk >> o) & 1u);
+    const uint32_t v = vo + uint32_t((wall ? rw : rn) - m);
+    if (NTA && Lat<Q>::cx(i) == 0)
+      f[i] = S::template from<Q>(bld_pop_s<0, E>(rs, v * el, uint32_t(m + B) * el), i);
+    else
+      f[i] = S::template from<Q>(bld_pop_s<1, E>(rs, v * el, uint32_t(m + B) * el), i);
+  }
+  if (lm) {
+    R rho_f, u_f[3];
+    load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if ((lm >> opp(i)) & 1u) {
+        if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(i, rho_f, f[i]);
+        else f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
+      }
+    }
+  }
+  const R om = R(a.omega);
+  collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
+  R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
+  if (lm) load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
+    const int32_t rn = i * dp + nb_rel<Q>(i, yp, zp), rw = opp(i) * dp;
+    const int32_t m = rn < rw ? rn : rw;
+    const bool wall = i != 0 && ((lk >> i) & 1u);
+    R v = f[i];
+    if (i != 0 && ((lm >> i) & 1u)) {
+      if constexpr (kScalarLat<Q>) v = anti_term<Q>(opp(i), rho_f, v);
+      else v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
+    }
+    const uint32_t w = vo + uint32_t((wall ? rw : rn) - m);
+    if (NTA && Lat<Q>::cx(i) == 0)
+      bst_pop_s<0, E>(rs, w * el, uint32_t(m + B) * el, S::template to<Q>(v, i));
+    else
+      bst_pop_s<1, E>(rs, w * el, uint32_t(m + B) * el, S::template to<Q>(v, i));
+  }
+}
 
 // Wavefront gw of the block (row gw / wpr, x segment gw % wpr) updates its
 // up to 64 cells; lane is the lane id. With vec == -3 (linear form) wavefronts run over the
@@ -485,6 +601,13 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     const uint64_t m = a.links[s0];
     ln = uint32_t(m);
     lm = uint32_t(m >> 32);
+  }
+  if constexpr (BUF && !EVEN && !PER) {
+    // rows of 256 slots and more start on line boundaries: their c_x = 0
+    // directions are whole lines of this wavefront alone
+    if (FLB_NT_ODD_ALIGNED && a.pitch >= 256) odd_buf<Q, S, FAM, true>(a, rs, pidx, c0, s0, dyn, cs, ln, lm);
+    else odd_buf<Q, S, FAM, false>(a, rs, pidx, c0, s0, dyn, cs, ln, lm);
+    return;
   }
   const unsigned lk = ln | lm;
   R f[Q];

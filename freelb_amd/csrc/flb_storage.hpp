@@ -68,27 +68,34 @@ struct StoreF16S {
 // (f - w stays far below 0.25 at lattice speeds). Round to nearest even;
 // larger magnitudes saturate. The E = 0 codes are formed by an FP32 add of
 // 2^-5, whose ulp is exactly 2^-28.
+//
+// The code is the FP32 word of x 2^-109 with its 12 leading significant bits
+// kept: an FP32 number with exponent field E and fraction m 2^12, i.e.
+// (1 + m/2^11) 2^(E-127) for E >= 1 and the subnormal m 2^-137 for E = 0, is
+// exactly the F16C value times 2^-109 (the kernels keep FP32 subnormals).
+// - Decoding places the code's magnitude bits at bit 12 and its sign at bit
+//   31 of an FP32 word; one fused multiply-add by 2^109 (exact) adds w.
+// - Encoding scales f - w by 2^-109, which is exact here: below 2^-17 the
+//   difference of f and w is itself exact (Sterbenz) and has no bit under
+//   2^-31, above it the product stays normal. The integer add of 0x7FF plus
+//   the lowest kept bit rounds the FP32 word to nearest even at bit 12, in
+//   the normal and the subnormal range alike.
 struct StoreF16C {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 4;
-  static constexpr uint32_t kTwoM5 = 122u << 23;  // bits of 2^-5
   template <int Q>
   __device__ __forceinline__ static R from(E e, int i) {
-    const uint32_t m = uint32_t(e) & 0x7FFFu;
-    const float sub = __uint_as_float(kTwoM5 + m) - 0.03125f;
-    const float nor = __uint_as_float((m + (109u << 11)) << 12);
-    const float a = m < 2048u ? sub : nor;
-    return ((e & 0x8000u) ? -a : a) + Lat<Q>::template w<float>(i);
+    const uint32_t c = uint32_t(e);
+    const float x = __uint_as_float(((c & 0x8000u) << 16) | ((c & 0x7FFFu) << 12));
+    return __fmaf_rn(x, 0x1p109f, Lat<Q>::template w<float>(i));
   }
   template <int Q>
   __device__ __forceinline__ static E to(R v, int i) {
-    const uint32_t u = __float_as_uint(v - Lat<Q>::template w<float>(i));
+    const uint32_t u = __float_as_uint((v - Lat<Q>::template w<float>(i)) * 0x1p-109f);
     const uint32_t a = u & 0x7FFFFFFFu;
-    const int cn = int((a + 0x7FFu + ((a >> 12) & 1u)) >> 12) - int(109u << 11);
-    const uint32_t nor = uint32_t(cn < 0x7FFF ? cn : 0x7FFF);
-    const uint32_t sub = __float_as_uint(__uint_as_float(a) + 0.03125f) - kTwoM5;
-    return E(((u >> 16) & 0x8000u) | (a < (110u << 23) ? sub : nor));
+    const uint32_t r = (a + 0x7FFu + ((a >> 12) & 1u)) >> 12;
+    return E(((u >> 16) & 0x8000u) | (r < 0x7FFFu ? r : 0x7FFFu));
   }
 };
 

@@ -18,7 +18,8 @@ static REAL MFN(smag_omega)(int q, const REAL* c, REAL rho, const REAL* u, REAL 
 /* D2Q9: collision::MRT_Feq_RhoU<D2Q9> as unrolled in collisionMRT.ur.h:45-129
  * (closed-form equilibrium moments, rates {0, 1.1, 1.1, 0, 1.1, 0, 1.1, w, w}).
  * D3Q19: m = M f, m_eq = M feq(SecondOrder), f <- f - M^T diag(s_j / |M_j|^2)
- * (m - m_eq) on the orthogonal d'Humieres basis (no reference form). */
+ * (m - m_eq) on the orthogonal d'Humieres basis (no reference form; the
+ * factors s_j / |M_j|^2 are rounded as the GPU kernel rounds them). */
 static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
                           const flbo_lattice* L, REAL* c, size_t id, size_t N) {
   REAL rho, u[3] = {0, 0, 0};
@@ -72,12 +73,16 @@ static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
     MFN(feq_cell)(q, w, rho, u, feq);
     for (int i = 0; i < 19; ++i) g[i] = c[i] - feq[i];
     for (int j = 0; j < 19; ++j) {
+      /* the row's factor s_j / |M_j|^2: a fixed rate over the norm in double,
+         rounded once; a shear row (NaN or negative rate) the cell's omega times
+         the rounded reciprocal of the norm */
       double sj = L->mrt_rates ? L->mrt_rates[j] : mrt19_default_rate(j);
-      const REAL s = isnan(sj) ? om : (REAL)sj;
+      const REAL sn = (isnan(sj) || sj < 0.0) ? om * (REAL)(1.0 / (double)M19_NORM[j])
+                                              : (REAL)(sj / (double)M19_NORM[j]);
       REAL dm = (REAL)0;
       for (int i = 0; i < 19; ++i)
         if (M19[j][i] != 0) dm += (REAL)M19[j][i] * g[i];
-      k[j] = s * dm / (REAL)M19_NORM[j];
+      k[j] = dm * sn;
     }
     for (int i = 0; i < 19; ++i) {
       REAL s = (REAL)0;

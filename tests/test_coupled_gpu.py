@@ -237,15 +237,19 @@ def _outer(dims):
 
 def test_coupling_rejects_blocks_on_different_streams():
     """The scalar step reads the flow's VELOCITY field and the flow step the
-    scalar's RHO field; on two streams nothing would order them."""
-    import torch
+    scalar's RHO field; on two streams nothing would order them. The streams
+    come from the HIP runtime the library itself uses."""
+    import ctypes as C
 
-    s1 = torch.cuda.Stream()
-    s2 = torch.cuda.Stream()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    s1, s2 = C.c_void_p(), C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(s1)) == 0 and hip.hipStreamCreate(C.byref(s2)) == 0
     dims = (8, 7, 6)
-    flow = BlockLattice(19, dims, omega=1.5, stream=s1.cuda_stream)
-    temp = BlockLattice(7, dims, omega=1.2, stream=s2.cuda_stream)
-    same = BlockLattice(7, dims, omega=1.2, stream=s1.cuda_stream)
+    flow = BlockLattice(19, dims, omega=1.5, stream=s1)
+    temp = BlockLattice(7, dims, omega=1.2, stream=s2)
+    same = BlockLattice(7, dims, omega=1.2, stream=s1)
     try:
         with pytest.raises(FLBError, match="share one stream"):
             temp.share_velocity(flow)
@@ -255,4 +259,5 @@ def test_coupling_rejects_blocks_on_different_streams():
     finally:
         for b in (same, temp, flow):
             b.close()
-        torch.cuda.synchronize()
+        hip.hipStreamDestroy(s1)
+        hip.hipStreamDestroy(s2)
