@@ -44,7 +44,7 @@ EXPORTS = (
     "flb_adjoint_init_equilibrium", "flb_adjoint_forward", "flb_adjoint_backward", "flb_adjoint_state",
     "flb_adjoint_timing", "flb_adjoint_batch_create", "flb_adjoint_batch_destroy", "flb_adjoint_batch_forward",
     "flb_adjoint_batch_backward", "flb_adjoint_batch_timing", "flb_bandwidth_probe",
-    "flb_bandwidth_probe_ex",
+    "flb_bandwidth_probe_ex", "flb_reduce_mass_momentum_noalloc",
 )
 
 
@@ -188,6 +188,7 @@ def lib() -> C.CDLL:
         "flb_adjoint_batch_backward": [P, C.POINTER(C.c_double)],
         "flb_adjoint_batch_timing": [P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "flb_bandwidth_probe": [C.c_size_t, C.c_int, C.POINTER(C.c_double)],
+        "flb_reduce_mass_momentum_noalloc": [P, C.c_uint8, C.c_void_p],
         "flb_bandwidth_probe_ex": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_int,
                                    C.POINTER(C.c_double)],
     }

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1081,6 +1082,62 @@ int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]) {
   HIP_TRY(launch_reduce(b, a, mask, b->d_partials, b->d_red));
   HIP_TRY(hipMemcpyAsync(out, b->d_red, 4 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---- scratch of the allocation-free reduction ---------------------------------
+// One double4 of partial sums per workgroup of the per-cell kernels, for
+// blocks of up to kNoAllocMaxWG * 256 = 2^28 host cells: 32 MiB, held by
+// every device the library runs on from the code object's load until the
+// process ends. The array's name is a host-side stand-in; kernels get the
+// device address that hipGetSymbolAddress returns for the current device.
+constexpr int64_t kNoAllocMaxWG = int64_t(1) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[4 * kNoAllocMaxWG];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on device dev, looked up once per device
+// with that device current.
+int scratch_on(int dev, double** out) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(FLB_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    DeviceGuard g(dev);
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FLB_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flb_reduce_mass_momentum_noalloc(flb_block b, uint8_t mask, double* d_out) {
+  CHECK_BLOCK(b);
+  if (!d_out) return fail(FLB_ERR_ARG, "null device output");
+  if (b->pending) return fail(FLB_ERR_STATE, "reduction between ApplyCellDynamics and Stream");
+  const int64_t nb = int64_t(flb::cell_blocks(b->N));
+  if (nb > kNoAllocMaxWG)
+    return fail(FLB_ERR_ARG, "block of %lld host cells exceeds the allocation-free reduction's maximum of %lld",
+                static_cast<long long>(b->N), static_cast<long long>(kNoAllocMaxWG * flb::kMaxThreads));
+  double* partials = nullptr;
+  const int rc = scratch_on(b->device, &partials);
+  if (rc) return rc;
+  DeviceGuard g(b->device);
+  const KArgs a = make_args(b);
+  // every workgroup overwrites its own partials, so nothing needs clearing
+  HIP_TRY(launch_reduce(b, a, mask, partials, d_out));
   return FLB_OK;
 }
 

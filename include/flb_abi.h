@@ -295,6 +295,16 @@ int flb_set_watchdog(flb_block b, int every);
 int flb_moments(flb_block b, uint8_t mask, void* rhou);
 /* FP64 totals over cells with (flag & mask) != 0: out = {mass, Px, Py, Pz}. */
 int flb_reduce_mass_momentum(flb_block b, uint8_t mask, double out[4]);
+/* The same reduction for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No reference
+ * counterpart. It calls no allocating routine: its partial sums live in a
+ * file-scope device array of the library (32 MiB per device, held until the
+ * process ends), so it takes blocks of at most 2^28 host cells (FLB_ERR_ARG
+ * beyond), and it writes the four sums (mass, momentum x/y/z; the same bits as
+ * flb_reduce_mass_momentum) to the caller's device memory d_out, enqueued on the
+ * block's stream without synchronising. Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams, or threads) race. */
+int flb_reduce_mass_momentum_noalloc(flb_block b, uint8_t mask, double* d_out);
 
 /* ---- batches ----------------------------------------------------------------
  * Many blocks of one lattice and storage kind stepped by one launch per time

@@ -76,6 +76,7 @@ def test_null_arguments():
     for fn in ("flb_refine_average", "flb_refine_interpolate", "flb_refine_communicate"):
         assert getattr(L, fn)(None, 0) == abi.FLB_ERR_ARG
     assert L.flb_refine_set_levels(None, 0, 1) == abi.FLB_ERR_ARG
+    assert L.flb_reduce_mass_momentum_noalloc(None, 0xFE, None) == abi.FLB_ERR_ARG
     assert L.flb_step(None, 1) == abi.FLB_ERR_ARG
     assert L.flb_set_tasks(None, None, 0) == abi.FLB_ERR_ARG
     assert "null block" in L.flb_last_error().decode()
