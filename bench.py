@@ -90,6 +90,29 @@ def time_storage(storage, steps, warmup, timer):
             "finite": finite}
 
 
+def mrt_les_speed(timer, storages=("f32", "f16s", "f16c"), n=512, steps=100, warmup=10):
+    """Step rate of the 512^3 MRT+LES cavity (Re 5000, lid 0.05, D3Q19,
+    d'Humieres MRT with the Smagorinsky shear rates): K timed steps after W
+    untimed ones per storage kind, HIP events around the launches."""
+    from freelb_amd.cases import cavity_block
+    out = {}
+    for s in storages:
+        lat = cavity_block(n, storage=s, collision="mrt_les", tau=0.5 + 3 * 0.05 * n / 5000, u_lid=0.05)
+        lat.step(warmup)
+        lat.synchronize()
+        timer.sync()
+        timer.start()
+        lat.step(steps)
+        ms = timer.stop()
+        finite = bool(np.isfinite(lat.mass_momentum()).all())
+        lat.close()
+        bpc = 152 if s == "f32" else (304 if s == "f64" else 76)
+        gbs = n ** 3 * bpc * steps / (ms / 1e3) / 1e9
+        out[s] = {"mlups": round(n ** 3 * steps / (ms / 1e3) / 1e6, 1), "ms_per_step": round(ms / steps, 5),
+                  "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "finite": finite}
+    return out
+
+
 def _vorticity(rhou, n):
     """Central-difference curl of u over the cavity cells whose six
     neighbours are cavity cells; rhou is [4][(n+2)^3]."""
@@ -474,6 +497,8 @@ def main():
     ap.add_argument("--config4", action="store_true",
                     help="512^3 MRT+LES cavity (Re 5000, lid 0.05), 5000 steps, f32/f16s/f16c vs FP64")
     ap.add_argument("--no-cavity", action="store_true", help="with --config4: only the batched adjoint")
+    ap.add_argument("--mrt-les", action="store_true",
+                    help="time the 512^3 MRT+LES cavity (Re 5000) for --steps steps per storage kind instead")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
     ap.add_argument("--no-refined", action="store_true", help="skip the refined-cavity run")
     ap.add_argument("--no-coupled", action="store_true", help="skip the coupled flow + temperature run")
@@ -486,6 +511,11 @@ def main():
               "with a face exchange per step).", file=sys.stderr)
         sys.exit(2)
 
+    if args.mrt_les:
+        st = tuple(x for x in args.storages.split(",") if x != "bf16s")
+        print(json.dumps({"mrt_les_512": mrt_les_speed(HipTimer(), storages=st, steps=args.steps,
+                                                       warmup=args.warmup)}), flush=True)
+        return
     if args.config4:
         # Re = u N / nu = 5000 with u = 0.05, N = 512: tau0 = 0.5 + 3 nu = 0.51536
         if not args.no_cavity:
@@ -543,10 +573,19 @@ def main():
         line["warning"] = "non-finite mass after the timed run"
     if not args.no_probe:
         from freelb_amd import abi
+        # the best 2 GiB -> 2 GiB copy over the probe shapes (16-byte accesses
+        # in flight per lane, plain or non-temporal, one visit per tile)
         gbs = C.c_double()
-        abi.check(abi.lib().flb_bandwidth_probe(C.c_size_t(1 << 30), 20, C.byref(gbs)))
-        line["device_copy_gbs"] = round(gbs.value, 1)
-        line["roofline"]["frac_of_copy"] = round(f32["achieved_gbs"] / gbs.value, 4)
+        best, shape = 0.0, ""
+        for unroll in (1, 2, 4):
+            for nt in (0, 1):
+                abi.check(abi.lib().flb_bandwidth_probe_ex(0, unroll, nt, 0, C.c_size_t(2 << 30), 10,
+                                                           C.byref(gbs)))
+                if gbs.value > best:
+                    best, shape = gbs.value, f"copy unroll {unroll}, {'non-temporal' if nt else 'plain'}"
+        line["device_copy_gbs"] = round(best, 1)
+        line["device_copy_shape"] = shape
+        line["roofline"]["frac_of_copy"] = round(f32["achieved_gbs"] / best, 4)
     if not args.no_batch:
         line["batch"] = batch_bench(timer, eager=args.batch_eager, graph_steps=50 if args.batch_eager else 500)
         print(f"# batch: {line['batch']}", file=sys.stderr, flush=True)

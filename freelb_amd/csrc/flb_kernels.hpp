@@ -602,6 +602,21 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     ln = uint32_t(m);
     lm = uint32_t(m >> 32);
   }
+  if constexpr (BUF && EVEN && !PER) {
+    // the even step touches the cell's own slots only: one vector offset,
+    // the direction in the scalar offset
+    constexpr uint32_t el = sizeof(E);
+    const uint32_t vo = (pidx + uint32_t(c0)) * el;
+    const int32_t dp = int32_t(a.dpop);
+    R f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = S::template from<Q>(bld_pop_s<PAR, E>(rs, vo, uint32_t(i * dp) * el), i);
+    const R om = R(a.omega);
+    collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) bst_pop_s<PAR, E>(rs, vo, uint32_t(opp(i) * dp) * el, S::template to<Q>(f[i], i));
+    return;
+  }
   if constexpr (BUF && !EVEN && !PER) {
     // rows of 256 slots and more start on line boundaries: their c_x = 0
     // directions are whole lines of this wavefront alone
@@ -666,7 +681,9 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 // each is collided on its own, and a pair whose cells are not both updated
 // stores its live half alone. Needs an even row pitch (every row and
 // direction starts on a 4-byte boundary) and row segments, a.wpr counting
-// wavefronts of 128 cells.
+// wavefronts of 128 cells. Below 4 GiB of populations the words go through
+// the buffer descriptor: one lane offset for the pair, the direction's offset
+// in the scalar operand (no per-direction address arithmetic per lane).
 #ifndef FLB_EVEN_PAIR
 #define FLB_EVEN_PAIR 1
 #endif
@@ -688,6 +705,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
   const bool two = c0 + 1 < a.ex;
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
+  const bool bufa = a.pop_bytes != 0;
+  constexpr int aux = (PAR == 0 && FLB_NT_EVEN) ? 2 : 0;  // 2: non-temporal
+  const auto rs = pop_rsrc(a, sizeof(E));
+  const uint32_t vo = (uint32_t(kFrontPad) + uint32_t(a.zpop) * uint32_t(z) + uint32_t(a.ypop) * uint32_t(y) +
+                       uint32_t(c0)) * 2u;
+  const int32_t dp = int32_t(a.dpop);
   float cs0 = 0.0f, cs1 = 0.0f;
   const int d0 = match_task(a, a.flags[s0], cs0);
   const int d1 = two ? match_task(a, a.flags[s0 + 1], cs1) : int(DYN_NONE);
@@ -695,8 +718,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
   R f0[Q], f1[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const uint32_t* p = reinterpret_cast<const uint32_t*>((P + int64_t(i) * a.dpop) + c0);
-    const uint32_t w = ld_pop<PAR>(p);
+    uint32_t w;
+    if (bufa) {
+      w = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, uint32_t(i * dp) * 2u, aux);
+    } else {
+      w = ld_pop<PAR>(reinterpret_cast<const uint32_t*>((P + int64_t(i) * a.dpop) + c0));
+    }
     f0[i] = S::template from<Q>(E(w & 0xffffu), i);
     f1[i] = S::template from<Q>(E(w >> 16), i);
   }
@@ -707,7 +734,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
   for (int i = 0; i < Q; ++i) {
     E* const q = (P + int64_t(opp(i)) * a.dpop) + c0;
     const E e0 = S::template to<Q>(f0[i], i), e1 = S::template to<Q>(f1[i], i);
-    if (d0 != DYN_NONE && d1 != DYN_NONE) {
+    if (bufa && d0 != DYN_NONE && d1 != DYN_NONE) {
+      __builtin_amdgcn_raw_buffer_store_b32(uint32_t(e0) | (uint32_t(e1) << 16), rs, vo,
+                                            uint32_t(opp(i) * dp) * 2u, aux);
+    } else if (d0 != DYN_NONE && d1 != DYN_NONE) {
       st_pop<PAR>(reinterpret_cast<uint32_t*>(q), uint32_t(e0) | (uint32_t(e1) << 16));
     } else if (d0 != DYN_NONE) {
       st_pop<PAR>(q, e0);
