@@ -266,6 +266,89 @@ __device__ __forceinline__ void bgk_with(R (&f)[Q], R rho, const R (&u)[3], R om
   for (int i = 0; i < Q; ++i) f[i] = omega * e[i] + _omega * f[i];
 }
 
+// BGK on the deviations g_i = f_i - w_i that the shifted storage kinds hold
+// (D2Q9 / D3Q19). Same algebra as collision::BGK on f: with sum w_i = 1 and
+// sum c_i w_i = 0,
+//   rho - 1 = sum g_i,   rho u = sum c_i g_i,
+//   feq_i - w_i = w_i (rho (1 + a_i + a_i^2 / 2 - 3 u.u / 2) - 1),  a_i = 3 c_i.u,
+// and g_i <- omega (feq_i - w_i) + (1 - omega) g_i. With d = rho - 1,
+// A = d - rho 3 u.u / 2 and t_i = rho a_i / 2, the bracket of direction i is
+// A + t_i a_i + 2 t_i and that of opp(i) (a = -a_i) A + t_i a_i - 2 t_i: no
+// sum near 1 is formed, so the density keeps the resolution of its deviation.
+// The rounding is not FreeLB's (the FP32 and FP64 kinds keep that order).
+
+// c_i . v with the zero and unit components of c_i folded away.
+template <int Q, class R>
+__device__ __forceinline__ R cdot_folded(int i, const R (&v)[3]) {
+  R r = R(0);
+  bool any = false;
+#pragma unroll
+  for (int d = 0; d < Lat<Q>::D; ++d) {
+    const int c = d == 0 ? Lat<Q>::cx(i) : (d == 1 ? Lat<Q>::cy(i) : Lat<Q>::cz(i));
+    if (c > 0) {
+      r = any ? r + v[d] : v[d];
+      any = true;
+    } else if (c < 0) {
+      r = any ? r - v[d] : -v[d];
+      any = true;
+    }
+  }
+  return r;
+}
+
+template <int Q, class R>
+__device__ __forceinline__ void rhou_dev(const R (&g)[Q], R& d, R& rho, R (&u)[3]) {
+  R s = g[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) s += g[i];
+  d = s;
+  rho = R(1) + s;
+  const R inv = R(1) / rho;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k >= Lat<Q>::D) {
+      u[k] = R(0);
+      continue;
+    }
+    R j = R(0);
+    bool any = false;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int c = k == 0 ? Lat<Q>::cx(i) : (k == 1 ? Lat<Q>::cy(i) : Lat<Q>::cz(i));
+      if (c > 0) {
+        j = any ? j + g[i] : g[i];
+        any = true;
+      } else if (c < 0) {
+        j = any ? j - g[i] : -g[i];
+        any = true;
+      }
+    }
+    u[k] = j * inv;
+  }
+}
+
+template <int Q, class R>
+__device__ __forceinline__ void bgk_dev(R (&g)[Q], R d, R rho, const R (&u)[3], R omega, R _omega) {
+  static_assert(Q == 9 || Q == 19, "flow lattices");
+  R U[3];
+  R uu = u[0] * u[0] + u[1] * u[1];
+  if (Lat<Q>::D == 3) uu += u[2] * u[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) U[k] = R(3) * u[k];
+  const R A = d - rho * (R(1.5) * uu);
+  const R rh = R(0.5) * rho;
+  g[0] = omega * Lat<Q>::template w<R>(0) * A + _omega * g[0];
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const R ai = cdot_folded<Q>(i, U);
+    const R t = rh * ai;
+    const R q = A + t * ai;
+    const R ow = omega * Lat<Q>::template w<R>(i);
+    g[i] = ow * (q + R(2) * t) + _omega * g[i];
+    g[i + 1] = ow * (q - R(2) * t) + _omega * g[i + 1];
+  }
+}
+
 // moment::forcerhoU (forcerhoUImpl<D3Q19> moment.ur.h:639-663, <D2Q9>
 // 564-574): u = (sum c f + F/2) / rho, each component divided on its own.
 template <class R>

@@ -175,9 +175,63 @@ __device__ __forceinline__ void store_fields(const KArgs& a, int slot, R rho, co
   }
 }
 
+// The dynamics of the FAM_FULL family beyond BGK and bounce-back (forced BGK,
+// BGK writing its moments, Smagorinsky BGK, MRT and MRT+LES) on f.
+template <int Q, class R>
+__device__ __forceinline__ void collide_full(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
+                                             R om, R _om) {
+  R rho, u[3], rho_f, u_f[3];
+  if (dyn == DYN_BGK_FORCE) {
+    const R F[3] = {R(a.force[0]), R(a.force[1]), R(a.force[2])};
+    bgk_force<Q>(f, F, om, _om);
+    return;
+  }
+  if (dyn == DYN_BGK_SCALAR_FORCE) {
+    // BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
+    // with SCALARFORCE = 0 + the force::Buoyancy terms (force.h:160-181)
+    // of the coupled scalar blocks, in coupling order
+    R F = R(0);
+    const uint8_t fl = a.flags[slot];
+#pragma unroll
+    for (int k = 0; k < kMaxCouplings; ++k) {
+      if (k < a.ncpl && (fl & a.cpl_mask[k]))
+        F += R(a.cpl_gbeta[k]) * (static_cast<const R*>(a.cpl_rho[k])[slot] - R(a.cpl_rho0[k]));
+    }
+    bgk_scalar_force<Q>(f, F, om, _om, rho, u);
+    store_fields<Q>(a, slot, rho, u);
+    return;
+  }
+  rhou(f, rho, u);
+  if (dyn == DYN_BGK_WRITE) {
+    store_fields<Q>(a, slot, rho, u);
+    bgk_with<Q>(f, rho, u, om, _om);
+    return;
+  }
+  R eo = om;
+  if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
+    load_fields<Q>(a, slot, rho_f, u_f);
+    eo = smag_omega(f, rho, u, om, R(cs), rho_f);
+  }
+  if (dyn == DYN_SMAG_BGK) {
+    bgk_with<Q>(f, rho, u, eo, R(1) - eo);
+  } else {
+    if constexpr (Q == 19) {
+      const R* sn;
+      if constexpr (sizeof(R) == 8) sn = a.mrt_sn_d;
+      else sn = a.mrt_sn_f;
+      mrt(f, rho, u, eo, sn, a.mrt_shear);
+    } else {
+      mrt(f, rho, u, eo, a.rates);
+    }
+  }
+}
+
 // One cell's collision. The RHO / VELOCITY fields of the cell (slot) are read
 // here, by the dynamics that use them, so they hold no registers elsewhere.
-template <int Q, class R, int FAM>
+// DEV: f holds the deviations g_i = f_i - w_i (shifted storage kinds on the
+// flow lattices). BGK then runs on g (bgk_dev), bounce-back is the same on
+// g as on f (w_i = w_opp(i)), and the other dynamics see f = g + w.
+template <int Q, class R, int FAM, bool DEV = false>
 __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
                                         R om, R _om) {
   if constexpr (FAM == FAM_SCALAR) {
@@ -206,12 +260,19 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
   R rho, u[3], rho_f, u_f[3];
   switch (dyn) {
     case DYN_BGK:
-      rhou(f, rho, u);
-      bgk_with<Q>(f, rho, u, om, _om);
+      if constexpr (DEV) {
+        R d;
+        rhou_dev<Q>(f, d, rho, u);
+        bgk_dev<Q>(f, d, rho, u, om, _om);
+      } else {
+        rhou(f, rho, u);
+        bgk_with<Q>(f, rho, u, om, _om);
+      }
       break;
     case DYN_BGK_FIELD:
       load_fields<Q>(a, slot, rho_f, u_f);
-      bgk_with<Q>(f, rho_f, u_f, om, _om);
+      if constexpr (DEV) bgk_dev<Q>(f, rho_f - R(1), rho_f, u_f, om, _om);
+      else bgk_with<Q>(f, rho_f, u_f, om, _om);
       break;
     case DYN_BB:
       bounceback<Q>(f);
@@ -222,48 +283,14 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
       break;
     default:
       if constexpr (FAM == FAM_FULL) {
-        if (dyn == DYN_BGK_FORCE) {
-          const R F[3] = {R(a.force[0]), R(a.force[1]), R(a.force[2])};
-          bgk_force<Q>(f, F, om, _om);
-          break;
-        }
-        if (dyn == DYN_BGK_SCALAR_FORCE) {
-          // BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
-          // with SCALARFORCE = 0 + the force::Buoyancy terms (force.h:160-181)
-          // of the coupled scalar blocks, in coupling order
-          R F = R(0);
-          const uint8_t fl = a.flags[slot];
+        if constexpr (DEV) {
 #pragma unroll
-          for (int k = 0; k < kMaxCouplings; ++k) {
-            if (k < a.ncpl && (fl & a.cpl_mask[k]))
-              F += R(a.cpl_gbeta[k]) * (static_cast<const R*>(a.cpl_rho[k])[slot] - R(a.cpl_rho0[k]));
-          }
-          bgk_scalar_force<Q>(f, F, om, _om, rho, u);
-          store_fields<Q>(a, slot, rho, u);
-          break;
+          for (int i = 0; i < Q; ++i) f[i] += Lat<Q>::template w<R>(i);
         }
-        rhou(f, rho, u);
-        if (dyn == DYN_BGK_WRITE) {
-          store_fields<Q>(a, slot, rho, u);
-          bgk_with<Q>(f, rho, u, om, _om);
-          break;
-        }
-        R eo = om;
-        if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
-          load_fields<Q>(a, slot, rho_f, u_f);
-          eo = smag_omega(f, rho, u, om, R(cs), rho_f);
-        }
-        if (dyn == DYN_SMAG_BGK) {
-          bgk_with<Q>(f, rho, u, eo, R(1) - eo);
-        } else {
-          if constexpr (Q == 19) {
-            const R* sn;
-            if constexpr (sizeof(R) == 8) sn = a.mrt_sn_d;
-            else sn = a.mrt_sn_f;
-            mrt(f, rho, u, eo, sn, a.mrt_shear);
-          } else {
-            mrt(f, rho, u, eo, a.rates);
-          }
+        collide_full<Q, R>(a, slot, dyn, cs, f, om, _om);
+        if constexpr (DEV) {
+#pragma unroll
+          for (int i = 0; i < Q; ++i) f[i] -= Lat<Q>::template w<R>(i);
         }
       }
       break;
@@ -313,15 +340,33 @@ __device__ __forceinline__ void load_flags(const uint8_t* p, uint8_t (&fl)[VEC])
   }
 }
 
+// The step kernels compute on the deviations g_i = f_i - w_i when the
+// storage holds them (the shifted kinds) on the flow lattices: decoding skips
+// the add of w_i and encoding its subtraction, and collide<..., true> runs
+// BGK on g. Every step kernel and launch family takes the same path, so a
+// block's result does not depend on which kernel stepped it.
+template <int Q, class S>
+constexpr bool kDev = S::kShifted && (Q == 9 || Q == 19);
+template <int Q, class S>
+__device__ __forceinline__ typename S::R pop_dec(typename S::E e, int i) {
+  if constexpr (kDev<Q, S>) return S::dev(e);
+  else return S::template from<Q>(e, i);
+}
+template <int Q, class S>
+__device__ __forceinline__ typename S::E pop_enc(typename S::R v, int i) {
+  if constexpr (kDev<Q, S>) return S::undev(v);
+  else return S::template to<Q>(v, i);
+}
+
 // VEC elements of one direction at p (aligned), converted to the compute type.
 template <int Q, class S, int VEC>
 __device__ __forceinline__ void load_row(const typename S::E* p, int i, typename S::R (&v)[VEC]) {
   if constexpr (VEC == 1) {
-    v[0] = S::template from<Q>(p[0], i);
+    v[0] = pop_dec<Q, S>(p[0], i);
   } else {
     const auto w = load_vec<typename S::E, VEC>(p);
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) v[j] = S::template from<Q>(w[j], i);
+    for (int j = 0; j < VEC; ++j) v[j] = pop_dec<Q, S>(w[j], i);
   }
 }
 
@@ -332,17 +377,17 @@ __device__ __forceinline__ void store_row(typename S::E* p, int i, const typenam
   using E = typename S::E;
   constexpr unsigned full = (1u << VEC) - 1u;
   if constexpr (VEC == 1) {
-    if (mask & 1u) p[0] = S::template to<Q>(v[0], i);
+    if (mask & 1u) p[0] = pop_enc<Q, S>(v[0], i);
   } else {
     if (mask == full) {
       vec_t<E, VEC> w;
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) w[j] = S::template to<Q>(v[j], i);
+      for (int j = 0; j < VEC; ++j) w[j] = pop_enc<Q, S>(v[j], i);
       store_vec<E, VEC>(p, w);
     } else if (mask) {
 #pragma unroll
       for (int j = 0; j < VEC; ++j)
-        if ((mask >> j) & 1u) p[j] = S::template to<Q>(v[j], i);
+        if ((mask >> j) & 1u) p[j] = pop_enc<Q, S>(v[j], i);
     }
   }
 }
@@ -454,6 +499,54 @@ __device__ __forceinline__ int32_t nb_rel(int k, int32_t yp, int32_t zp) {
   return Lat<Q>::cx(k) + Lat<Q>::cy(k) * yp + Lat<Q>::cz(k) * zp;
 }
 
+// Halfway wall terms of the odd step (moving walls; anti-bounce-back on the
+// scalar lattices), each applied as one branch around the whole block so that
+// wavefronts without such a cell skip it. In: the population f_i that came
+// back across moving-wall link opp(i) picks up the term before the
+// collision. Out: the population f_i leaving across moving-wall link i picks
+// it up before its store.
+template <int Q, class R>
+__device__ __forceinline__ void wall_terms_in(const KArgs& a, int s0, unsigned lm, R (&f)[Q]) {
+  if (!lm) return;
+  R rho_f, u_f[3];
+  load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if ((lm >> opp(i)) & 1u) {
+      if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(i, rho_f, f[i]);
+      else f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
+    }
+  }
+}
+template <int Q, class R>
+__device__ __forceinline__ void wall_terms_out(const KArgs& a, int s0, unsigned lm, R (&f)[Q]) {
+  if (!lm) return;
+  R rho_f, u_f[3];
+  load_fields<Q>(a, s0, rho_f, u_f);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if ((lm >> i) & 1u) {
+      if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(opp(i), rho_f, f[i]);
+      else f[i] = f[i] + movingwall_term<Q>(opp(i), rho_f, u_f);
+    }
+  }
+}
+
+// Per-lane byte delta of an odd-step access: of the two candidate slots rn
+// (the neighbour's) and rw (the cell's own, taken when bit k of the link
+// word is set), the smaller is in the scalar offset; the lane adds the
+// difference when it takes the larger one. The link bit becomes a sign mask
+// (0 or all ones) ANDed with the scalar difference: no compare and select.
+// lkx is the link word XOR the wavefront-uniform bits of the directions
+// whose wall candidate is the smaller one, so bit k set = larger candidate.
+template <uint32_t el>
+__device__ __forceinline__ uint32_t lane_delta(unsigned lkx, int k, int32_t rn, int32_t rw, bool linked) {
+  if (!linked) return 0u;
+  const uint32_t d = uint32_t(rw > rn ? rw - rn : rn - rw) * el;
+  const uint32_t mask = uint32_t(int32_t(lkx << (31 - k)) >> 31);
+  return mask & d;
+}
+
 // The odd step through the buffer descriptor (no periodic axis). Every
 // access is the lane's base vo = pidx + c0 - B plus a per-direction part: a
 // wavefront-uniform scalar offset m + B, where m is the smaller of the two
@@ -471,8 +564,13 @@ __device__ __forceinline__ void odd_buf(const KArgs& a, __amdgpu_buffer_rsrc_t r
   constexpr uint32_t el = sizeof(E);
   const int32_t dp = int32_t(a.dpop), yp = a.ypop, zp = int32_t(a.zpop);
   const int32_t B = (Lat<Q>::D == 3 ? zp : 0) + yp + 1;
-  const uint32_t vo = pidx + uint32_t(c0) - uint32_t(B);
+  const uint32_t vob = (pidx + uint32_t(c0) - uint32_t(B)) * el;
   const unsigned lk = ln | lm;
+  unsigned flip = 0;  // bit k: the wall candidate of direction k is the smaller slot
+#pragma unroll
+  for (int k = 1; k < Q; ++k)
+    if (int32_t(opp(k) * dp) < int32_t(k * dp + nb_rel<Q>(k, yp, zp))) flip |= 1u << k;
+  const unsigned lkx = lk ^ __builtin_amdgcn_readfirstlane(flip);
   R f[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -480,44 +578,26 @@ __device__ __forceinline__ void odd_buf(const KArgs& a, __amdgpu_buffer_rsrc_t r
     const int o = opp(i);
     const int32_t rn = o * dp + nb_rel<Q>(o, yp, zp), rw = i * dp;
     const int32_t m = rn < rw ? rn : rw;
-    const bool wall = i != 0 && ((lk >> o) & 1u);
-    const uint32_t v = vo + uint32_t((wall ? rw : rn) - m);
+    const uint32_t v = vob + lane_delta<el>(lkx, o, rn, rw, i != 0);
     if (NTA && Lat<Q>::cx(i) == 0)
-      f[i] = S::template from<Q>(bld_pop_s<0, E>(rs, v * el, uint32_t(m + B) * el), i);
+      f[i] = pop_dec<Q, S>(bld_pop_s<0, E>(rs, v, uint32_t(m + B) * el), i);
     else
-      f[i] = S::template from<Q>(bld_pop_s<1, E>(rs, v * el, uint32_t(m + B) * el), i);
+      f[i] = pop_dec<Q, S>(bld_pop_s<1, E>(rs, v, uint32_t(m + B) * el), i);
   }
-  if (lm) {
-    R rho_f, u_f[3];
-    load_fields<Q>(a, s0, rho_f, u_f);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if ((lm >> opp(i)) & 1u) {
-        if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(i, rho_f, f[i]);
-        else f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
-      }
-    }
-  }
+  wall_terms_in<Q>(a, s0, lm, f);
   const R om = R(a.omega);
-  collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
-  R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
-  if (lm) load_fields<Q>(a, s0, rho_f, u_f);
+  collide<Q, R, FAM, kDev<Q, S>>(a, s0, dyn, cs, f, om, R(1) - om);
+  wall_terms_out<Q>(a, s0, lm, f);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
     const int32_t rn = i * dp + nb_rel<Q>(i, yp, zp), rw = opp(i) * dp;
     const int32_t m = rn < rw ? rn : rw;
-    const bool wall = i != 0 && ((lk >> i) & 1u);
-    R v = f[i];
-    if (i != 0 && ((lm >> i) & 1u)) {
-      if constexpr (kScalarLat<Q>) v = anti_term<Q>(opp(i), rho_f, v);
-      else v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
-    }
-    const uint32_t w = vo + uint32_t((wall ? rw : rn) - m);
+    const uint32_t w = vob + lane_delta<el>(lkx, i, rn, rw, i != 0);
     if (NTA && Lat<Q>::cx(i) == 0)
-      bst_pop_s<0, E>(rs, w * el, uint32_t(m + B) * el, S::template to<Q>(v, i));
+      bst_pop_s<0, E>(rs, w, uint32_t(m + B) * el, pop_enc<Q, S>(f[i], i));
     else
-      bst_pop_s<1, E>(rs, w * el, uint32_t(m + B) * el, S::template to<Q>(v, i));
+      bst_pop_s<1, E>(rs, w, uint32_t(m + B) * el, pop_enc<Q, S>(f[i], i));
   }
 }
 
@@ -610,11 +690,11 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     const int32_t dp = int32_t(a.dpop);
     R f[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = S::template from<Q>(bld_pop_s<PAR, E>(rs, vo, uint32_t(i * dp) * el), i);
+    for (int i = 0; i < Q; ++i) f[i] = pop_dec<Q, S>(bld_pop_s<PAR, E>(rs, vo, uint32_t(i * dp) * el), i);
     const R om = R(a.omega);
-    collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
+    collide<Q, R, FAM, kDev<Q, S>>(a, s0, dyn, cs, f, om, R(1) - om);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) bst_pop_s<PAR, E>(rs, vo, uint32_t(opp(i) * dp) * el, S::template to<Q>(f[i], i));
+    for (int i = 0; i < Q; ++i) bst_pop_s<PAR, E>(rs, vo, uint32_t(opp(i) * dp) * el, pop_enc<Q, S>(f[i], i));
     return;
   }
   if constexpr (BUF && !EVEN && !PER) {
@@ -629,46 +709,30 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (EVEN) {
-      f[i] = S::template from<Q>(load(int64_t(i) * a.dpop + c0), i);
+      f[i] = pop_dec<Q, S>(load(int64_t(i) * a.dpop + c0), i);
     } else {
       // upstream neighbour x - c_i a wall: own f*_{opp i}, left in A[i][x]
       const int o = opp(i);
       const bool wall = i != 0 && ((lk >> o) & 1u);
       const int64_t rel = wall ? int64_t(i) * a.dpop + c0
                                : int64_t(o) * a.dpop + c0 + nb_off<Q, PER, true>(a, o, x, y, z);
-      f[i] = S::template from<Q>(load(rel), i);
+      f[i] = pop_dec<Q, S>(load(rel), i);
     }
   }
-  if (!EVEN && lm) {
-    R rho_f, u_f[3];
-    load_fields<Q>(a, s0, rho_f, u_f);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if ((lm >> opp(i)) & 1u) {
-        if constexpr (kScalarLat<Q>) f[i] = anti_term<Q>(i, rho_f, f[i]);
-        else f[i] = f[i] + movingwall_term<Q>(i, rho_f, u_f);
-      }
-    }
-  }
+  if (!EVEN) wall_terms_in<Q>(a, s0, lm, f);
   const R om = R(a.omega);
-  collide<Q, R, FAM>(a, s0, dyn, cs, f, om, R(1) - om);
-  R rho_f = R(1), u_f[3] = {R(0), R(0), R(0)};
-  if (!EVEN && lm) load_fields<Q>(a, s0, rho_f, u_f);
+  collide<Q, R, FAM, kDev<Q, S>>(a, s0, dyn, cs, f, om, R(1) - om);
+  if (!EVEN) wall_terms_out<Q>(a, s0, lm, f);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if (EVEN) {
-      store(int64_t(opp(i)) * a.dpop + c0, S::template to<Q>(f[i], i));
+      store(int64_t(opp(i)) * a.dpop + c0, pop_enc<Q, S>(f[i], i));
     } else {
       // downstream neighbour x + c_i a wall: f_{opp i}(x, t+1) = f*_i (+ wall term)
       const bool wall = i != 0 && ((lk >> i) & 1u);
-      R v = f[i];
-      if (i != 0 && ((lm >> i) & 1u)) {
-        if constexpr (kScalarLat<Q>) v = anti_term<Q>(opp(i), rho_f, v);
-        else v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
-      }
       const int64_t rel = wall ? int64_t(opp(i)) * a.dpop + c0
                                : int64_t(i) * a.dpop + c0 + nb_off<Q, PER, true>(a, i, x, y, z);
-      store(rel, S::template to<Q>(v, i));
+      store(rel, pop_enc<Q, S>(f[i], i));
     }
   }
 }
@@ -724,16 +788,16 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
     } else {
       w = ld_pop<PAR>(reinterpret_cast<const uint32_t*>((P + int64_t(i) * a.dpop) + c0));
     }
-    f0[i] = S::template from<Q>(E(w & 0xffffu), i);
-    f1[i] = S::template from<Q>(E(w >> 16), i);
+    f0[i] = pop_dec<Q, S>(E(w & 0xffffu), i);
+    f1[i] = pop_dec<Q, S>(E(w >> 16), i);
   }
   const R om = R(a.omega);
-  if (d0 != DYN_NONE) collide<Q, R, FAM>(a, s0, d0, cs0, f0, om, R(1) - om);
-  if (d1 != DYN_NONE) collide<Q, R, FAM>(a, s0 + 1, d1, cs1, f1, om, R(1) - om);
+  if (d0 != DYN_NONE) collide<Q, R, FAM, kDev<Q, S>>(a, s0, d0, cs0, f0, om, R(1) - om);
+  if (d1 != DYN_NONE) collide<Q, R, FAM, kDev<Q, S>>(a, s0 + 1, d1, cs1, f1, om, R(1) - om);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     E* const q = (P + int64_t(opp(i)) * a.dpop) + c0;
-    const E e0 = S::template to<Q>(f0[i], i), e1 = S::template to<Q>(f1[i], i);
+    const E e0 = pop_enc<Q, S>(f0[i], i), e1 = pop_enc<Q, S>(f1[i], i);
     if (bufa && d0 != DYN_NONE && d1 != DYN_NONE) {
       __builtin_amdgcn_raw_buffer_store_b32(uint32_t(e0) | (uint32_t(e1) << 16), rs, vo,
                                             uint32_t(opp(i) * dp) * 2u, aux);
@@ -858,13 +922,13 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_row(KArgs a) {
         for (int j = 0; j < VEC; ++j) v[j] = w[j];
       } else if (cx > 0) {
         R nxt = wave_shl(w[0]);
-        if (lane == kWave - 1) nxt = S::template from<Q>(base[VEC], i);
+        if (lane == kWave - 1) nxt = pop_dec<Q, S>(base[VEC], i);
 #pragma unroll
         for (int j = 0; j < VEC - 1; ++j) v[j] = w[j + 1];
         v[VEC - 1] = nxt;
       } else {
         R prv = wave_shr(w[VEC - 1]);
-        if (lane == 0) prv = S::template from<Q>(base[-1], i);
+        if (lane == 0) prv = pop_dec<Q, S>(base[-1], i);
         v[0] = prv;
 #pragma unroll
         for (int j = 1; j < VEC; ++j) v[j] = w[j - 1];
@@ -884,7 +948,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_row(KArgs a) {
 #pragma unroll
       for (int i = 1; i < Q; ++i) {
         if ((lk >> opp(i)) & 1u) {
-          R v = S::template from<Q>((P + int64_t(i) * a.dpop)[c0 + j], i);
+          R v = pop_dec<Q, S>((P + int64_t(i) * a.dpop)[c0 + j], i);
           if ((lm[j] >> opp(i)) & 1u) v = v + movingwall_term<Q>(i, rho_f, u_f);
           f[j][i] = v;
         }
@@ -897,7 +961,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_row(KArgs a) {
   const R _om = R(1) - om;
 #pragma unroll
   for (int j = 0; j < VEC; ++j)
-    if ((live >> j) & 1u) collide<Q, R, FAM>(a, s0 + j, dyn[j], cs[j], f[j], om, _om);
+    if ((live >> j) & 1u) collide<Q, R, FAM, kDev<Q, S>>(a, s0 + j, dyn[j], cs[j], f[j], om, _om);
 
   // --- write ------------------------------------------------------------------------------
   if (PAR == 0) {
@@ -1004,7 +1068,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_row(KArgs a) {
         if ((lk >> i) & 1u) {
           R v = f[j][i];
           if ((lm[j] >> i) & 1u) v = v + movingwall_term<Q>(opp(i), rho_f, u_f);
-          (P + int64_t(opp(i)) * a.dpop)[c0 + j] = S::template to<Q>(v, i);
+          (P + int64_t(opp(i)) * a.dpop)[c0 + j] = pop_enc<Q, S>(v, i);
         }
       }
     }

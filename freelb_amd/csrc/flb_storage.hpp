@@ -7,6 +7,10 @@
 // instead of relative to w_i. All arithmetic is done in the compute type R
 // (float, or double for F64). w_i = w_{opp i}, so the shift of a value does
 // not depend on which of the two AA slots it sits in.
+//
+// from / to convert between a stored element and f_i. The shifted kinds
+// (kShifted) also give dev / undev, the conversion between the element and
+// the deviation g_i = f_i - w_i itself, for kernels that compute on g.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -24,6 +28,7 @@ struct StoreF32 {
   using E = float;  // element in HBM
   using R = float;  // compute type
   static constexpr int kind = 0;
+  static constexpr bool kShifted = false;
   template <int Q>
   __device__ __forceinline__ static R from(E e, int) {
     return e;
@@ -38,6 +43,7 @@ struct StoreF64 {
   using E = double;
   using R = double;
   static constexpr int kind = 3;
+  static constexpr bool kShifted = false;
   template <int Q>
   __device__ __forceinline__ static R from(E e, int) {
     return e;
@@ -52,13 +58,16 @@ struct StoreF16S {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 1;
+  static constexpr bool kShifted = true;
+  __device__ __forceinline__ static R dev(E e) { return __half2float(__ushort_as_half(e)); }
+  __device__ __forceinline__ static E undev(R d) { return __half_as_ushort(__float2half_rn(d)); }
   template <int Q>
   __device__ __forceinline__ static R from(E e, int i) {
-    return __half2float(__ushort_as_half(e)) + Lat<Q>::template w<float>(i);
+    return dev(e) + Lat<Q>::template w<float>(i);
   }
   template <int Q>
   __device__ __forceinline__ static E to(R v, int i) {
-    return __half_as_ushort(__float2half_rn(v - Lat<Q>::template w<float>(i)));
+    return undev(v - Lat<Q>::template w<float>(i));
   }
 };
 
@@ -80,22 +89,34 @@ struct StoreF16S {
 //   2^-31, above it the product stays normal. The integer add of 0x7FF plus
 //   the lowest kept bit rounds the FP32 word to nearest even at bit 12, in
 //   the normal and the subnormal range alike.
+// - undev encodes a deviation computed as such (not as a difference): below
+//   2^-17 its scaled value can round once at 2^-149 before the rounding at
+//   bit 12, so a value within 2^-40 of a tie may round to the other
+//   neighbour (an error of at most half a code step plus 2^-40).
 struct StoreF16C {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 4;
-  template <int Q>
-  __device__ __forceinline__ static R from(E e, int i) {
+  static constexpr bool kShifted = true;
+  // the code as an FP32 number: the value times 2^-109
+  __device__ __forceinline__ static float scaled(E e) {
     const uint32_t c = uint32_t(e);
-    const float x = __uint_as_float(((c & 0x8000u) << 16) | ((c & 0x7FFFu) << 12));
-    return __fmaf_rn(x, 0x1p109f, Lat<Q>::template w<float>(i));
+    return __uint_as_float(((c & 0x8000u) << 16) | ((c & 0x7FFFu) << 12));
   }
-  template <int Q>
-  __device__ __forceinline__ static E to(R v, int i) {
-    const uint32_t u = __float_as_uint((v - Lat<Q>::template w<float>(i)) * 0x1p-109f);
+  __device__ __forceinline__ static R dev(E e) { return scaled(e) * 0x1p109f; }
+  __device__ __forceinline__ static E undev(R d) {
+    const uint32_t u = __float_as_uint(d * 0x1p-109f);
     const uint32_t a = u & 0x7FFFFFFFu;
     const uint32_t r = (a + 0x7FFu + ((a >> 12) & 1u)) >> 12;
     return E(((u >> 16) & 0x8000u) | (r < 0x7FFFu ? r : 0x7FFFu));
+  }
+  template <int Q>
+  __device__ __forceinline__ static R from(E e, int i) {
+    return __fmaf_rn(scaled(e), 0x1p109f, Lat<Q>::template w<float>(i));
+  }
+  template <int Q>
+  __device__ __forceinline__ static E to(R v, int i) {
+    return undev(v - Lat<Q>::template w<float>(i));
   }
 };
 
@@ -103,16 +124,21 @@ struct StoreBF16S {
   using E = uint16_t;
   using R = float;
   static constexpr int kind = 2;
-  template <int Q>
-  __device__ __forceinline__ static R from(E e, int i) {
-    return __uint_as_float(uint32_t(e) << 16) + Lat<Q>::template w<float>(i);
-  }
+  static constexpr bool kShifted = true;
+  __device__ __forceinline__ static R dev(E e) { return __uint_as_float(uint32_t(e) << 16); }
   // Round to nearest even on the upper 16 bits (values are finite here).
-  template <int Q>
-  __device__ __forceinline__ static E to(R v, int i) {
-    const uint32_t u = __float_as_uint(v - Lat<Q>::template w<float>(i));
+  __device__ __forceinline__ static E undev(R d) {
+    const uint32_t u = __float_as_uint(d);
     const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
     return E(r >> 16);
+  }
+  template <int Q>
+  __device__ __forceinline__ static R from(E e, int i) {
+    return dev(e) + Lat<Q>::template w<float>(i);
+  }
+  template <int Q>
+  __device__ __forceinline__ static E to(R v, int i) {
+    return undev(v - Lat<Q>::template w<float>(i));
   }
 };
 
