@@ -582,19 +582,19 @@ __device__ __forceinline__ double dsqrt<double>(double v) {
 // Smagorinsky effective omega (collisionLES.h:36-59): |Pi_neq| from
 // Pi_ab_neq (moment.ur.h:708-718, 750-763) and NormSquare (util.h:418-434);
 // the coefficient is formed in double because std::sqrt(2) is a double.
-template <class R>
+// DEV: c holds the deviations f - w and d = rho - 1; sum c_a^2 w = cs2 and
+// sum c_a c_b w = 0 (a != b), so the diagonal terms subtract rho u_a^2 + d cs2.
+template <bool DEV = false, class R>
 __device__ __forceinline__ R smag_omega(const R (&c)[19], R rho, const R (&u)[3], R omega, R cs,
-                                        R rho_f) {
+                                        R rho_f, R d = R(0)) {
   const R cs2 = R(1) / R(3);
-  const R t0 = c[1] + c[2] + c[7] + c[8] + c[9] + c[10] + c[13] + c[14] + c[15] + c[16] -
-               rho * (u[0] * u[0] + cs2);
+  auto diag = [&](R s, R ua) { return DEV ? s - (rho * (ua * ua) + d * cs2) : s - rho * (ua * ua + cs2); };
+  const R t0 = diag(c[1] + c[2] + c[7] + c[8] + c[9] + c[10] + c[13] + c[14] + c[15] + c[16], u[0]);
   const R t1 = c[7] + c[8] - c[13] - c[14] - rho * (u[0] * u[1]);
   const R t2 = c[9] + c[10] - c[15] - c[16] - rho * (u[0] * u[2]);
-  const R t3 = c[3] + c[4] + c[7] + c[8] + c[11] + c[12] + c[13] + c[14] + c[17] + c[18] -
-               rho * (u[1] * u[1] + cs2);
+  const R t3 = diag(c[3] + c[4] + c[7] + c[8] + c[11] + c[12] + c[13] + c[14] + c[17] + c[18], u[1]);
   const R t4 = c[11] + c[12] - c[17] - c[18] - rho * (u[1] * u[2]);
-  const R t5 = c[5] + c[6] + c[9] + c[10] + c[11] + c[12] + c[15] + c[16] + c[17] + c[18] -
-               rho * (u[2] * u[2] + cs2);
+  const R t5 = diag(c[5] + c[6] + c[9] + c[10] + c[11] + c[12] + c[15] + c[16] + c[17] + c[18], u[2]);
   const R ns = t0 * t0 + t1 * t1 * R(2) + t2 * t2 * R(2) + t3 * t3 + t4 * t4 * R(2) + t5 * t5;
   const R norm = dsqrt(ns);
   // with the usual RHO field of 1 the double division is by 1 and is skipped
@@ -689,31 +689,97 @@ __device__ __forceinline__ R mrt_factor(int j, const R* sn, uint32_t shear, R om
   return ((shear >> j) & 1u) ? om * R(1.0 / double(m19_norm(j))) : sn[j];
 }
 
-template <class R>
-__device__ __forceinline__ void mrt(R (&f)[19], R rho, const R (&u)[3], R om, const R* sn, uint32_t shear) {
-  R e[19];
-  feq(e, rho, u);
-  R g[19];
+// Rows odd in c (j, q, m): M_j(opp i) = -M_j(i); the others are even,
+// M_j(opp i) = M_j(i) (every row is one or the other).
+__host__ __device__ constexpr bool m19_odd(int j) {
+  for (int i = 1; i < 19; i += 2)
+    if (m19(j, i + 1) != -m19(j, i)) return false;
+  return m19(j, 0) == 0;
+}
+__host__ __device__ constexpr bool m19_rows_have_parity() {
+  for (int j = 0; j < 19; ++j)
+    for (int i = 1; i < 19; i += 2)
+      if (m19(j, i + 1) != (m19_odd(j) ? -m19(j, i) : m19(j, i))) return false;
+  return true;
+}
+static_assert(m19_rows_have_parity(), "d'Humieres rows are even or odd in c");
+
+// The equilibrium the MRT and LES of a cell relax towards, in the form the
+// populations are held in: feq_i (FreeLB's evaluation order), or on the
+// deviations feq_i - w_i = w_i (A + t_i a_i +- 2 t_i) as in bgk_dev.
+template <bool DEV, class R>
+__device__ __forceinline__ void mrt_eq(R (&e)[19], R d, R rho, const R (&u)[3]) {
+  if constexpr (!DEV) {
+    feq(e, rho, u);
+  } else {
+    R U[3];
 #pragma unroll
-  for (int i = 0; i < 19; ++i) g[i] = f[i] - e[i];
+    for (int k = 0; k < 3; ++k) U[k] = R(3) * u[k];
+    const R A = d - rho * (R(1.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const R rh = R(0.5) * rho;
+    e[0] = Lat<19>::w<R>(0) * A;
+#pragma unroll
+    for (int i = 1; i < 19; i += 2) {
+      const R ai = cdot_folded<19>(i, U);
+      const R t = rh * ai;
+      const R q = A + t * ai;
+      e[i] = Lat<19>::w<R>(i) * (q + R(2) * t);
+      e[i + 1] = Lat<19>::w<R>(i) * (q - R(2) * t);
+    }
+  }
+}
+
+// D3Q19 MRT on the pairs of opposite directions: with n = f - feq,
+// s_p = n_i + n_opp(i) and d_p = n_i - n_opp(i) (i = 2p + 1), an even row is
+// M_j0 n_0 + sum_p M_ji s_p and an odd row sum_p M_ji d_p; back, with
+// k_j = (M n)_j s_j / |M_j|^2, E_p = sum over even rows of M_ji k_j and O_p
+// over odd rows, f_i -= E_p + O_p and f_opp(i) -= E_p - O_p. Every sum runs
+// over the nonzero entries in ascending index from zero. DEV: f holds the
+// deviations f - w (d = rho - 1), relaxed towards feq - w.
+template <bool DEV, class R>
+__device__ __forceinline__ void mrt(R (&f)[19], R d, R rho, const R (&u)[3], R om, const R* sn, uint32_t shear) {
+  R e[19];
+  mrt_eq<DEV>(e, d, rho, u);
+  const R n0 = f[0] - e[0];
+  R sp[9], dp[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    const R a = f[2 * p + 1] - e[2 * p + 1];
+    const R b = f[2 * p + 2] - e[2 * p + 2];
+    sp[p] = a + b;
+    dp[p] = a - b;
+  }
   R k[19];
   static_for<19>([&](auto J) {
     constexpr int j = decltype(J)::value;
     R dm = R(0);
-    static_for<19>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      if constexpr (m19(j, i) != 0) dm += R(m19(j, i)) * g[i];
+    if constexpr (!m19_odd(j) && m19(j, 0) != 0) dm += R(m19(j, 0)) * n0;
+    static_for<9>([&](auto P) {
+      constexpr int p = decltype(P)::value;
+      constexpr int m = m19(j, 2 * p + 1);
+      if constexpr (m != 0) dm += R(m) * (m19_odd(j) ? dp[p] : sp[p]);
     });
     k[j] = dm * mrt_factor<R>(j, sn, shear, om);
   });
-  static_for<19>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    R acc = R(0);
+  R d0 = R(0);
+  static_for<19>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (!m19_odd(j) && m19(j, 0) != 0) d0 += R(m19(j, 0)) * k[j];
+  });
+  f[0] -= d0;
+  static_for<9>([&](auto P) {
+    constexpr int p = decltype(P)::value;
+    R E = R(0), O = R(0);
     static_for<19>([&](auto J) {
       constexpr int j = decltype(J)::value;
-      if constexpr (m19(j, i) != 0) acc += R(m19(j, i)) * k[j];
+      constexpr int m = m19(j, 2 * p + 1);
+      if constexpr (m != 0) {
+        if constexpr (m19_odd(j)) O += R(m) * k[j];
+        else E += R(m) * k[j];
+      }
     });
-    f[i] -= acc;
+    f[2 * p + 1] -= E + O;
+    f[2 * p + 2] -= E - O;
   });
 }
 

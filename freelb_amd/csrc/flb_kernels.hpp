@@ -175,51 +175,73 @@ __device__ __forceinline__ void store_fields(const KArgs& a, int slot, R rho, co
   }
 }
 
-// The dynamics of the FAM_FULL family beyond BGK and bounce-back (forced BGK,
-// BGK writing its moments, Smagorinsky BGK, MRT and MRT+LES) on f.
+// f_i + w_i and back: the shifted kinds' deviations as populations.
 template <int Q, class R>
+__device__ __forceinline__ void add_w(R (&f)[Q]) {
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] += Lat<Q>::template w<R>(i);
+}
+template <int Q, class R>
+__device__ __forceinline__ void sub_w(R (&f)[Q]) {
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] -= Lat<Q>::template w<R>(i);
+}
+
+// The dynamics of the FAM_FULL family beyond BGK and bounce-back (forced BGK,
+// BGK writing its moments, Smagorinsky BGK, MRT and MRT+LES). DEV (D3Q19
+// only): f holds the deviations f - w; BGK, the Smagorinsky omega and MRT
+// then run on them, the forced dynamics on f = g + w.
+template <int Q, class R, bool DEV>
 __device__ __forceinline__ void collide_full(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
                                              R om, R _om) {
+  static_assert(!DEV || Q == 19, "deviation form of the full family: D3Q19");
   R rho, u[3], rho_f, u_f[3];
-  if (dyn == DYN_BGK_FORCE) {
-    const R F[3] = {R(a.force[0]), R(a.force[1]), R(a.force[2])};
-    bgk_force<Q>(f, F, om, _om);
-    return;
-  }
-  if (dyn == DYN_BGK_SCALAR_FORCE) {
-    // BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
-    // with SCALARFORCE = 0 + the force::Buoyancy terms (force.h:160-181)
-    // of the coupled scalar blocks, in coupling order
-    R F = R(0);
-    const uint8_t fl = a.flags[slot];
+  if (dyn == DYN_BGK_FORCE || dyn == DYN_BGK_SCALAR_FORCE) {
+    if constexpr (DEV) add_w<Q>(f);
+    if (dyn == DYN_BGK_FORCE) {
+      const R F[3] = {R(a.force[0]), R(a.force[1]), R(a.force[2])};
+      bgk_force<Q>(f, F, om, _om);
+    } else {
+      // BGKForce<forcerhoU<CELL, ScalarForce, true>, SecondOrder, ScalarForce>
+      // with SCALARFORCE = 0 + the force::Buoyancy terms (force.h:160-181)
+      // of the coupled scalar blocks, in coupling order
+      R F = R(0);
+      const uint8_t fl = a.flags[slot];
 #pragma unroll
-    for (int k = 0; k < kMaxCouplings; ++k) {
-      if (k < a.ncpl && (fl & a.cpl_mask[k]))
-        F += R(a.cpl_gbeta[k]) * (static_cast<const R*>(a.cpl_rho[k])[slot] - R(a.cpl_rho0[k]));
+      for (int k = 0; k < kMaxCouplings; ++k) {
+        if (k < a.ncpl && (fl & a.cpl_mask[k]))
+          F += R(a.cpl_gbeta[k]) * (static_cast<const R*>(a.cpl_rho[k])[slot] - R(a.cpl_rho0[k]));
+      }
+      bgk_scalar_force<Q>(f, F, om, _om, rho, u);
+      store_fields<Q>(a, slot, rho, u);
     }
-    bgk_scalar_force<Q>(f, F, om, _om, rho, u);
-    store_fields<Q>(a, slot, rho, u);
+    if constexpr (DEV) sub_w<Q>(f);
     return;
   }
-  rhou(f, rho, u);
+  R d = R(0);
+  if constexpr (DEV) rhou_dev<Q>(f, d, rho, u);
+  else rhou(f, rho, u);
   if (dyn == DYN_BGK_WRITE) {
     store_fields<Q>(a, slot, rho, u);
-    bgk_with<Q>(f, rho, u, om, _om);
+    if constexpr (DEV) bgk_dev<Q>(f, d, rho, u, om, _om);
+    else bgk_with<Q>(f, rho, u, om, _om);
     return;
   }
   R eo = om;
   if (dyn == DYN_SMAG_BGK || dyn == DYN_SMAG_MRT) {
     load_fields<Q>(a, slot, rho_f, u_f);
-    eo = smag_omega(f, rho, u, om, R(cs), rho_f);
+    if constexpr (Q == 19) eo = smag_omega<DEV>(f, rho, u, om, R(cs), rho_f, d);
+    else eo = smag_omega(f, rho, u, om, R(cs), rho_f);
   }
   if (dyn == DYN_SMAG_BGK) {
-    bgk_with<Q>(f, rho, u, eo, R(1) - eo);
+    if constexpr (DEV) bgk_dev<Q>(f, d, rho, u, eo, R(1) - eo);
+    else bgk_with<Q>(f, rho, u, eo, R(1) - eo);
   } else {
     if constexpr (Q == 19) {
       const R* sn;
       if constexpr (sizeof(R) == 8) sn = a.mrt_sn_d;
       else sn = a.mrt_sn_f;
-      mrt(f, rho, u, eo, sn, a.mrt_shear);
+      mrt<DEV>(f, d, rho, u, eo, sn, a.mrt_shear);
     } else {
       mrt(f, rho, u, eo, a.rates);
     }
@@ -230,7 +252,7 @@ __device__ __forceinline__ void collide_full(const KArgs& a, int slot, int dyn, 
 // here, by the dynamics that use them, so they hold no registers elsewhere.
 // DEV: f holds the deviations g_i = f_i - w_i (shifted storage kinds on the
 // flow lattices). BGK then runs on g (bgk_dev), bounce-back is the same on
-// g as on f (w_i = w_opp(i)), and the other dynamics see f = g + w.
+// g as on f (w_i = w_opp(i)); collide_full takes the rest.
 template <int Q, class R, int FAM, bool DEV = false>
 __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float cs, R (&f)[Q],
                                         R om, R _om) {
@@ -283,15 +305,11 @@ __device__ __forceinline__ void collide(const KArgs& a, int slot, int dyn, float
       break;
     default:
       if constexpr (FAM == FAM_FULL) {
-        if constexpr (DEV) {
-#pragma unroll
-          for (int i = 0; i < Q; ++i) f[i] += Lat<Q>::template w<R>(i);
-        }
-        collide_full<Q, R>(a, slot, dyn, cs, f, om, _om);
-        if constexpr (DEV) {
-#pragma unroll
-          for (int i = 0; i < Q; ++i) f[i] -= Lat<Q>::template w<R>(i);
-        }
+        // D2Q9's full family (FreeLB's closed-form MRT) runs on f
+        constexpr bool dev19 = DEV && Q == 19;
+        if constexpr (DEV && !dev19) add_w<Q>(f);
+        collide_full<Q, R, dev19>(a, slot, dyn, cs, f, om, _om);
+        if constexpr (DEV && !dev19) sub_w<Q>(f);
       }
       break;
   }

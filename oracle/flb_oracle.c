@@ -58,6 +58,7 @@ static const int INVM9_D[9][9] = {{9, 9, 9, 1, 1, 1, 1, 1, 1},     {9, 36, 18, 6
  * orthogonality, and by mass/momentum conservation. */
 static int M19[19][19];
 static int M19_NORM[19];
+static int M19_ODD[19]; /* row odd in c: M(j, opp i) = -M(j, i); else even */
 static int m19_ready = 0;
 
 static void m19_init(void) {
@@ -89,6 +90,9 @@ static void m19_init(void) {
     int n = 0;
     for (int i = 0; i < 19; ++i) n += M19[j][i] * M19[j][i];
     M19_NORM[j] = n;
+    int odd = M19[j][0] == 0;
+    for (int i = 1; i < 19; i += 2) odd = odd && M19[j][i + 1] == -M19[j][i];
+    M19_ODD[j] = odd;
   }
   m19_ready = 1;
 }

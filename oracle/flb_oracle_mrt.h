@@ -19,7 +19,8 @@ static REAL MFN(smag_omega)(int q, const REAL* c, REAL rho, const REAL* u, REAL 
  * (closed-form equilibrium moments, rates {0, 1.1, 1.1, 0, 1.1, 0, 1.1, w, w}).
  * D3Q19: m = M f, m_eq = M feq(SecondOrder), f <- f - M^T diag(s_j / |M_j|^2)
  * (m - m_eq) on the orthogonal d'Humieres basis (no reference form; the
- * factors s_j / |M_j|^2 are rounded as the GPU kernel rounds them). */
+ * factors s_j / |M_j|^2 are rounded as the GPU kernel rounds them, and both
+ * evaluate the transforms on the pairs of opposite directions). */
 static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
                           const flbo_lattice* L, REAL* c, size_t id, size_t N) {
   REAL rho, u[3] = {0, 0, 0};
@@ -69,9 +70,21 @@ static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
       c[i] -= s;
     }
   } else {
-    REAL feq[19], g[19], k[19];
+    /* On the pairs of opposite directions (i = 2p+1, i+1 = opp i): with
+       n = f - feq, s_p = n_i + n_{i+1}, d_p = n_i - n_{i+1}, an even row is
+       M(j,0) n_0 + sum_p M(j,i) s_p and an odd row sum_p M(j,i) d_p; back,
+       E_p / O_p sum M(j,i) k_j over the even / odd rows, f_i -= E_p + O_p,
+       f_{i+1} -= E_p - O_p. Each sum starts from zero and runs over the
+       nonzero entries in ascending index. */
+    REAL feq[19], sp[9], dp[9], k[19];
     MFN(feq_cell)(q, w, rho, u, feq);
-    for (int i = 0; i < 19; ++i) g[i] = c[i] - feq[i];
+    const REAL n0 = c[0] - feq[0];
+    for (int p = 0; p < 9; ++p) {
+      const REAL a = c[2 * p + 1] - feq[2 * p + 1];
+      const REAL b = c[2 * p + 2] - feq[2 * p + 2];
+      sp[p] = a + b;
+      dp[p] = a - b;
+    }
     for (int j = 0; j < 19; ++j) {
       /* the row's factor s_j / |M_j|^2: a fixed rate over the norm in double,
          rounded once; a shear row (NaN or negative rate) the cell's omega times
@@ -80,15 +93,27 @@ static void MFN(mrt_cell)(int q, const REAL* w, REAL omega, const flbo_task* t,
       const REAL sn = (isnan(sj) || sj < 0.0) ? om * (REAL)(1.0 / (double)M19_NORM[j])
                                               : (REAL)(sj / (double)M19_NORM[j]);
       REAL dm = (REAL)0;
-      for (int i = 0; i < 19; ++i)
-        if (M19[j][i] != 0) dm += (REAL)M19[j][i] * g[i];
+      if (!M19_ODD[j] && M19[j][0] != 0) dm += (REAL)M19[j][0] * n0;
+      for (int p = 0; p < 9; ++p) {
+        const int m = M19[j][2 * p + 1];
+        if (m != 0) dm += (REAL)m * (M19_ODD[j] ? dp[p] : sp[p]);
+      }
       k[j] = dm * sn;
     }
-    for (int i = 0; i < 19; ++i) {
-      REAL s = (REAL)0;
-      for (int j = 0; j < 19; ++j)
-        if (M19[j][i] != 0) s += (REAL)M19[j][i] * k[j];
-      c[i] -= s;
+    REAL d0 = (REAL)0;
+    for (int j = 0; j < 19; ++j)
+      if (!M19_ODD[j] && M19[j][0] != 0) d0 += (REAL)M19[j][0] * k[j];
+    c[0] -= d0;
+    for (int p = 0; p < 9; ++p) {
+      REAL E = (REAL)0, O = (REAL)0;
+      for (int j = 0; j < 19; ++j) {
+        const int m = M19[j][2 * p + 1];
+        if (m == 0) continue;
+        if (M19_ODD[j]) O += (REAL)m * k[j];
+        else E += (REAL)m * k[j];
+      }
+      c[2 * p + 1] -= E + O;
+      c[2 * p + 2] -= E - O;
     }
   }
 }
