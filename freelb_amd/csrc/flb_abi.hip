@@ -148,11 +148,27 @@ bool linear_form(int ex) {
   return 10 * ex < 9 * lanes;
 }
 
+// Whether the step kernels may address the block's populations through
+// wavefront-based buffer descriptors: no periodic axis (the descriptor starts
+// one plane and row before the wavefront's first row, which rows y, z >= 1
+// keep inside the allocation) and every offset from that start below 2 GiB:
+// the lane's row within the wavefront (row segments: 0; linear runs of 64
+// cells: at most 64 rows and planes), its x, and the per-direction scalar
+// offsets of at most two neighbour ranges.
+uint32_t buffer_form_ok(const KArgs& a, int elem, int dims) {
+  if (a.px || a.py || a.pz) return 0u;
+  if (a.ys < 1 || (dims == 3 && a.zs < 1)) return 0u;
+  const uint64_t zp = dims == 3 ? uint64_t(a.zpop) : 0u;
+  const uint64_t B = zp + uint64_t(a.ypop) + 1u;
+  const uint64_t rows = a.vec == -3 ? 64u * (uint64_t(a.ypop) + zp) : 0u;
+  const uint64_t span = rows + uint64_t(a.pitch) + 2u * B + 2u * 18u * uint64_t(a.dpop) + 2u;
+  return span * uint64_t(elem) < (uint64_t(1) << 31) ? 1u : 0u;
+}
+
 KArgs make_args(const flb_block_s* b) {
   KArgs a{};
   a.pops = static_cast<char*>(b->d_pops) + flb::kFrontPad * b->elem;
-  const uint64_t pop_bytes = uint64_t(b->pop_elems) * b->elem;
-  a.pop_bytes = pop_bytes < (uint64_t(1) << 32) ? uint32_t(pop_bytes) : 0u;
+  a.pop_total = uint64_t(b->pop_elems) * b->elem;
   a.dpop = b->dpop;
   a.ypop = b->ypop;
   a.zpop = b->zpop;
@@ -237,6 +253,7 @@ KArgs make_args(const flb_block_s* b) {
     a.cpl_rho0[a.ncpl] = c.rho0;
     ++a.ncpl;
   }
+  a.buf = buffer_form_ok(a, int(b->elem), b->d);
   return a;
 }
 

@@ -78,6 +78,7 @@ KArgs batch_args(const flb_block_s* b) {
   a.groups = a.ex;
   a.wpr = (a.ex + flb::kWave - 1) / flb::kWave;
   a.vec = linear_form(a.ex) ? -3 : 0;
+  a.buf = buffer_form_ok(a, int(b->elem), b->d);
   return a;
 }
 

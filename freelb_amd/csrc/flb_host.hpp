@@ -126,6 +126,7 @@ struct DeviceGuard {
 };
 
 KArgs make_args(const flb_block_s* b);
+uint32_t buffer_form_ok(const KArgs& a, int elem, int dims);
 // True when a count-gated call acts on a block of `level` in a hierarchy whose
 // finest level is max_level (BlockLatticeManager's 2^(maxlevel - level) rule).
 inline bool level_due(int64_t count, int level, int max_level) {

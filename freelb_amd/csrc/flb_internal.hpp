@@ -68,7 +68,8 @@ struct KArgs {
   uint8_t bckind[kMaxBc];
   uint8_t bcany;         // OR of bcmask: cells that may have halfway links
   const uint64_t* links; // slot 0 of the precomputed wall links (normal | moving << 32)
-  uint32_t pop_bytes;    // bytes of the population allocation when below 4 GiB, else 0
+  uint64_t pop_total;    // bytes of the population allocation
+  uint32_t buf;          // 1: the step may address populations through wavefront-based buffer descriptors
   double omega;
   double rates[19];      // MRT rates; < 0 or NaN marks the shear rates
   float mrt_sn_f[19];    // D3Q19 MRT: rate / |M_j|^2 of the fixed-rate rows (FP32 kinds)

@@ -442,48 +442,27 @@ __device__ __forceinline__ void st_pop(E* p, E v) {
   else *p = v;
 }
 
-// Buffer-addressed population access: one descriptor (in scalar registers)
-// for the whole population allocation and a 32-bit byte offset per lane, so
-// an address costs one 32-bit add instead of 64-bit pointer arithmetic. Used
-// when the allocation is below 4 GiB (KArgs::pop_bytes != 0); out-of-range
-// offsets, which the kernels never form, would read 0 and drop stores.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const KArgs& a, size_t elem) {
+// Buffer-addressed population access: a descriptor in scalar registers whose
+// base is a wavefront-uniform element e0 (index from the allocation start),
+// a 32-bit byte offset per lane and a wavefront-uniform scalar offset per
+// direction, so an address costs no 64-bit pointer arithmetic. The range runs
+// from e0 to the end of the allocation, capped at 4 GiB - 1 bytes; the
+// offsets the kernels form stay below 2 GiB (KArgs::buf, checked on the host
+// for the block's strides), so allocations of any size take this form.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const KArgs& a, size_t elem, int64_t e0) {
+  const uint64_t off = uint64_t(e0) * elem;
+  const uint64_t rest = a.pop_total - off;
+  const uint32_t nrec = rest < 0xFFFFFFFFull ? uint32_t(rest) : 0xFFFFFFFFu;
   // word 3 of a raw (unformatted) buffer descriptor on the gfx9 family
-  return __builtin_amdgcn_make_buffer_rsrc(static_cast<char*>(a.pops) - kFrontPad * elem, 0,
-                                           int(a.pop_bytes), 0x00020000);
-}
-template <int PAR, class E>
-__device__ __forceinline__ E bld_pop(__amdgpu_buffer_rsrc_t r, uint32_t idx) {
-  constexpr int aux = (PAR == 0 && FLB_NT_EVEN) ? 2 : 0;  // 2: non-temporal
-  if constexpr (sizeof(E) == 8) {
-    const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, aux);  // two 32-bit words
-    return __longlong_as_double(int64_t((uint64_t(w[1]) << 32) | uint64_t(w[0])));
-  } else if constexpr (sizeof(E) == 4) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4u, 0, aux));
-  } else {
-    return E(__builtin_amdgcn_raw_buffer_load_b16(r, idx * 2u, 0, aux));
-  }
-}
-template <int PAR, class E>
-__device__ __forceinline__ void bst_pop(__amdgpu_buffer_rsrc_t r, uint32_t idx, E v) {
-  constexpr int aux = ((PAR == 0 && FLB_NT_EVEN) || (PAR == 1 && FLB_NT_ODD_STORE)) ? 2 : 0;
-  if constexpr (sizeof(E) == 8) {
-    typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
-    const uint64_t b = uint64_t(__double_as_longlong(v));
-    const u32x2 w = {uint32_t(b), uint32_t(b >> 32)};
-    __builtin_amdgcn_raw_buffer_store_b64(w, r, idx * 8u, 0, aux);
-  } else if constexpr (sizeof(E) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, idx * 4u, 0, aux);
-  } else {
-    __builtin_amdgcn_raw_buffer_store_b16(uint16_t(v), r, idx * 2u, 0, aux);
-  }
+  return __builtin_amdgcn_make_buffer_rsrc(static_cast<char*>(a.pops) - kFrontPad * elem + off, 0, int(nrec),
+                                           0x00020000);
 }
 
 template <int Q, class S, int PAR, int FAM, bool PER, bool BUF = false>
 __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, int z);
 
-// Buffer access with a wavefront-uniform scalar byte offset so beside the
-// lane's vector byte offset vo.
+// One population at lane byte offset vo plus wavefront-uniform scalar byte
+// offset so, and its store.
 template <int PAR, class E>
 __device__ __forceinline__ E bld_pop_s(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
   constexpr int aux = (PAR == 0 && FLB_NT_EVEN) ? 2 : 0;
@@ -565,24 +544,26 @@ __device__ __forceinline__ uint32_t lane_delta(unsigned lkx, int k, int32_t rn, 
   return mask & d;
 }
 
-// The odd step through the buffer descriptor (no periodic axis). Every
-// access is the lane's base vo = pidx + c0 - B plus a per-direction part: a
+// The odd step through the buffer descriptor (no periodic axis). The
+// descriptor starts B = zpop + ypop + 1 (2D: ypop + 1) elements before the
+// wavefront's first row, B bounding the most negative neighbour offset (rows
+// y, z >= 1 keep it inside the allocation). Every access is the lane's base
+// prel + c0 (its row relative to the first) plus a per-direction part: a
 // wavefront-uniform scalar offset m + B, where m is the smaller of the two
 // candidate slots (the neighbour's, or the cell's own across a wall link),
-// and a per-lane delta (candidate - m >= 0) chosen by the lane's link bit.
-// B = zpop + ypop + 1 (2D: ypop + 1) bounds the most negative neighbour offset, so every
-// term is non-negative (rows y, z >= 1 hold pidx >= B). Same selection and
+// and a per-lane delta (candidate - m >= 0) chosen by the lane's link bit, so
+// every term is non-negative. Same selection and
 // arithmetic as the general path, bit for bit. NTA: the c_x = 0 directions
 // (whole aligned rows of wide blocks) load and store non-temporally.
 template <int Q, class S, int FAM, bool NTA>
-__device__ __forceinline__ void odd_buf(const KArgs& a, __amdgpu_buffer_rsrc_t rs, uint32_t pidx, int c0, int s0,
+__device__ __forceinline__ void odd_buf(const KArgs& a, __amdgpu_buffer_rsrc_t rs, uint32_t prel, int c0, int s0,
                                         int dyn, float cs, unsigned ln, unsigned lm) {
   using R = typename S::R;
   using E = typename S::E;
   constexpr uint32_t el = sizeof(E);
   const int32_t dp = int32_t(a.dpop), yp = a.ypop, zp = int32_t(a.zpop);
   const int32_t B = (Lat<Q>::D == 3 ? zp : 0) + yp + 1;
-  const uint32_t vob = (pidx + uint32_t(c0) - uint32_t(B)) * el;
+  const uint32_t vob = (prel + uint32_t(c0)) * el;
   const unsigned lk = ln | lm;
   unsigned flip = 0;  // bit k: the wall candidate of direction k is the smaller slot
 #pragma unroll
@@ -674,21 +655,20 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
   const int x = a.xs + c0;
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
-  // element index of P relative to the allocation start (buffer form)
-  const uint32_t pidx = uint32_t(kFrontPad) + uint32_t(a.zpop) * uint32_t(z) + uint32_t(a.ypop) * uint32_t(y);
+  // population at P + rel (rel: direction, x and neighbour offsets)
+  auto load = [&](int64_t rel) -> E { return ld_pop<PAR>(P + rel); };
+  auto store = [&](int64_t rel, E v) { st_pop<PAR>(P + rel, v); };
+  // buffer form: the descriptor starts B elements before the row of the
+  // wavefront's first active lane (the smallest row of its cells; every lane
+  // of a row segment shares it), prel is the lane's row relative to that row
+  const int32_t B = BUF ? (Lat<Q>::D == 3 ? int32_t(a.zpop) : 0) + a.ypop + 1 : 0;
+  const int yb = __builtin_amdgcn_readfirstlane(y), zb = __builtin_amdgcn_readfirstlane(z);
+  const uint32_t prel = uint32_t(a.zpop) * uint32_t(z - zb) + uint32_t(a.ypop) * uint32_t(y - yb);
   const auto rs = [&] {
-    if constexpr (BUF) return pop_rsrc(a, sizeof(E));
+    if constexpr (BUF)
+      return pop_rsrc(a, sizeof(E), int64_t(kFrontPad) + a.zpop * zb + int64_t(a.ypop) * yb - B);
     else return 0;
   }();
-  // population at P + rel (rel: direction, x and neighbour offsets)
-  auto load = [&](int64_t rel) -> E {
-    if constexpr (BUF) return bld_pop<PAR, E>(rs, pidx + uint32_t(int32_t(rel)));
-    else return ld_pop<PAR>(P + rel);
-  };
-  auto store = [&](int64_t rel, E v) {
-    if constexpr (BUF) bst_pop<PAR, E>(rs, pidx + uint32_t(int32_t(rel)), v);
-    else st_pop<PAR>(P + rel, v);
-  };
   const uint8_t fl = a.flags[s0];
   float cs = 0.0f;
   const int dyn = match_task(a, fl, cs);
@@ -704,22 +684,23 @@ __device__ __forceinline__ void cell_update_at(const KArgs& a, int c0, int y, in
     // the even step touches the cell's own slots only: one vector offset,
     // the direction in the scalar offset
     constexpr uint32_t el = sizeof(E);
-    const uint32_t vo = (pidx + uint32_t(c0)) * el;
+    const uint32_t vo = (prel + uint32_t(c0)) * el;
     const int32_t dp = int32_t(a.dpop);
     R f[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = pop_dec<Q, S>(bld_pop_s<PAR, E>(rs, vo, uint32_t(i * dp) * el), i);
+    for (int i = 0; i < Q; ++i) f[i] = pop_dec<Q, S>(bld_pop_s<PAR, E>(rs, vo, uint32_t(B + i * dp) * el), i);
     const R om = R(a.omega);
     collide<Q, R, FAM, kDev<Q, S>>(a, s0, dyn, cs, f, om, R(1) - om);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) bst_pop_s<PAR, E>(rs, vo, uint32_t(opp(i) * dp) * el, pop_enc<Q, S>(f[i], i));
+    for (int i = 0; i < Q; ++i)
+      bst_pop_s<PAR, E>(rs, vo, uint32_t(B + opp(i) * dp) * el, pop_enc<Q, S>(f[i], i));
     return;
   }
   if constexpr (BUF && !EVEN && !PER) {
     // rows of 256 slots and more start on line boundaries: their c_x = 0
     // directions are whole lines of this wavefront alone
-    if (FLB_NT_ODD_ALIGNED && a.pitch >= 256) odd_buf<Q, S, FAM, true>(a, rs, pidx, c0, s0, dyn, cs, ln, lm);
-    else odd_buf<Q, S, FAM, false>(a, rs, pidx, c0, s0, dyn, cs, ln, lm);
+    if (FLB_NT_ODD_ALIGNED && a.pitch >= 256) odd_buf<Q, S, FAM, true>(a, rs, prel, c0, s0, dyn, cs, ln, lm);
+    else odd_buf<Q, S, FAM, false>(a, rs, prel, c0, s0, dyn, cs, ln, lm);
     return;
   }
   const unsigned lk = ln | lm;
@@ -787,11 +768,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_step_even_pair(KArgs a) {
   const bool two = c0 + 1 < a.ex;
   const int s0 = c0 + a.pitch * y + int(a.plane) * z;
   E* const P = static_cast<E*>(a.pops) + int64_t(a.zpop) * z + int64_t(a.ypop) * y;
-  const bool bufa = a.pop_bytes != 0;
+  const bool bufa = a.buf != 0;
   constexpr int aux = (PAR == 0 && FLB_NT_EVEN) ? 2 : 0;  // 2: non-temporal
-  const auto rs = pop_rsrc(a, sizeof(E));
-  const uint32_t vo = (uint32_t(kFrontPad) + uint32_t(a.zpop) * uint32_t(z) + uint32_t(a.ypop) * uint32_t(y) +
-                       uint32_t(c0)) * 2u;
+  // descriptor from the row's first slot (the row is wavefront-uniform)
+  const auto rs = pop_rsrc(a, sizeof(E), int64_t(kFrontPad) + a.zpop * z + int64_t(a.ypop) * y);
+  const uint32_t vo = uint32_t(c0) * 2u;
   const int32_t dp = int32_t(a.dpop);
   float cs0 = 0.0f, cs1 = 0.0f;
   const int d0 = match_task(a, a.flags[s0], cs0);
@@ -1332,7 +1313,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_links(KArgs a, uint64_t* __rest
 template <int Q, class S, int PAR>
 hipError_t step_cell_q(int family, bool per, const KArgs& a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(kWave * a.wpb);
-  const bool buf = FLB_BUFFER_ADDR && PAR != 0 && a.pop_bytes != 0 && !per;
+  const bool buf = FLB_BUFFER_ADDR && PAR != 0 && a.buf != 0 && !per;
   if (family == FAM_FULL) {
     if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, true>), grid, block, 0, s, a);
     else if (buf) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_FULL, false, true>), grid, block, 0, s, a);
@@ -1428,7 +1409,7 @@ template <int Q, class S, int PAR>
 hipError_t step_scalar_q(const KArgs& a, hipStream_t s) {
   const dim3 grid(a.nblocks), block(kWave * a.wpb);
   const bool per = a.px || a.py || a.pz;
-  const bool buf = FLB_BUFFER_ADDR && PAR != 0 && a.pop_bytes != 0 && !per;
+  const bool buf = FLB_BUFFER_ADDR && PAR != 0 && a.buf != 0 && !per;
   if (per) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, true>), grid, block, 0, s, a);
   else if (buf) hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, false, true>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((k_step_cell<Q, S, PAR, FAM_SCALAR, false>), grid, block, 0, s, a);
