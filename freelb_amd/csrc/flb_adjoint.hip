@@ -14,7 +14,7 @@
 //   solid or outside a non-periodic axis (halfway bounce-back,
 //   boundary/bounce_back_boundary.h:169-181).
 // Objective: J = (1/T) sum_{t=1..T} sum_{x on plane x = outlet} sum_i c_ix f_i(x, t),
-// summed in FP64.
+// summed in FP64 inside the forward step that writes state t (obj_fold).
 //
 // Adjoint. With lambda^t the adjoint of f^t and mu^t = P^T lambda^{t+1} the
 // adjoint of the post-collision populations (mu_i(x) = lambda_i(x + c_i), or
@@ -167,12 +167,17 @@ __global__ __launch_bounds__(kThreads) void k_links(AdjArgs<R> a, uint32_t* __re
 }
 
 // ---- forward step: state src -> dst (push, halfway bounce-back) -------------
-template <class R>
-__device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restrict__ src,
-                                         R* __restrict__ dst, int64_t id) {
-  if (id >= a.N) return;
+// OBJ: also the cell's share of the objective of state dst, the FP64 sum, in
+// direction order, of c_jx times every value it pushes into population j of a
+// cell on the outlet plane (its own, across a wall link, or the neighbour's);
+// every population of the plane's cells is pushed by exactly one cell.
+template <class R, bool OBJ>
+__device__ __forceinline__ double fwd_cell(const AdjArgs<R>& a, const R* __restrict__ src,
+                                           R* __restrict__ dst, int64_t id) {
+  double c = 0.0;
+  if (id >= a.N) return c;
   const uint32_t lk = a.link[id];
-  if (lk & 1u) return;
+  if (lk & 1u) return c;
   int x, y, z;
   cell_xyz(id, a.nx, a.ny, x, y, z);
   R g[Q];
@@ -190,65 +195,72 @@ __device__ __forceinline__ void fwd_cell(const AdjArgs<R>& a, const R* __restric
     const R t2 = R(3) * (R(Lat<Q>::cz(i)) - u[2]) + R(9) * cu * R(Lat<Q>::cz(i));
     const R Fi = Lat<Q>::template w<R>(i) * (F[0] * t0 + F[1] * t1 + F[2] * t2);
     const R v = om * geq(i, drho, rho, u, usq) + _om * g[i] + fom * Fi;
-    if (i != 0 && ((lk >> i) & 1u)) dst[int64_t(opp(i)) * a.N + id] = v;
-    else dst[int64_t(i) * a.N + (i == 0 ? id : open_neighbour(a, x, y, z, i))] = v;
+    if (i != 0 && ((lk >> i) & 1u)) {
+      dst[int64_t(opp(i)) * a.N + id] = v;
+      if (OBJ && Lat<Q>::cx(opp(i)) != 0 && x == a.outlet) c += double(Lat<Q>::cx(opp(i))) * double(v);
+    } else {
+      dst[int64_t(i) * a.N + (i == 0 ? id : open_neighbour(a, x, y, z, i))] = v;
+      if (OBJ && Lat<Q>::cx(i) != 0) {
+        int xd = x + Lat<Q>::cx(i);
+        xd = xd < 0 ? xd + a.nx : (xd >= a.nx ? xd - a.nx : xd);
+        if (xd == a.outlet) c += double(Lat<Q>::cx(i)) * double(v);
+      }
+    }
   }
+  return c;
 }
 
-template <class R>
-__global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
-                                                  R* __restrict__ dst) {
-  fwd_cell(a, src, dst, int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x);
-}
-
-// ---- objective of one state: sum over the outlet plane of sum_i c_ix f_i ----
-// Workgroup z sums the row (x = outlet, z) of the plane into out[z] (a fixed
-// lane order and a fixed LDS tree, so every run gives the same bits);
-// k_obj_final adds the rows of each step in z order. The plane's cells lie
-// nx apart, so every value is a line fetch of its own: the kernel is bound
-// by those scattered reads, not by its parallelism.
-template <class R>
-__device__ __forceinline__ void obj_row(const AdjArgs<R>& a, const R* __restrict__ f, int z,
-                                        double* __restrict__ out) {
-  __shared__ double part[kThreads];
-  double acc = 0.0;
-  for (int y = threadIdx.x; y < a.ny; y += kThreads) {
-    const int64_t id = a.outlet + int64_t(a.nx) * (y + int64_t(a.ny) * z);
-    if (a.solid[id]) continue;
-    double m = 0.0;
+// Fixed-order sum of v over the workgroup (every thread calls it; thread 0
+// gets the result): a shuffle tree within each wavefront, then the
+// wavefronts in order.
+__device__ __forceinline__ double wg_sum(double v) {
+  __shared__ double ws[kThreads / 64];
 #pragma unroll
-    for (int i = 1; i < Q; ++i)
-      if (Lat<Q>::cx(i) != 0) m += double(Lat<Q>::cx(i)) * double(f[int64_t(i) * a.N + id]);
-    acc += m;
-  }
-  part[threadIdx.x] = acc;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[z] = part[0];
+  double s = ws[0];
+#pragma unroll
+  for (int w = 1; w < kThreads / 64; ++w) s += ws[w];
+  __syncthreads();
+  return s;
 }
 
-// one workgroup per z
-template <class R>
-__global__ __launch_bounds__(kThreads) void k_obj(AdjArgs<R> a, const R* __restrict__ f,
-                                                  double* __restrict__ rows) {
-  obj_row(a, f, int(blockIdx.x), rows);
+// Every thread of the workgroup holding chunk `chunk` (256 cells of the box)
+// calls this with its cell's share; the chunk's fixed-order sum goes to
+// partial[chunk]. Nothing is shared between workgroups (no atomics): k_obj_sum
+// adds the chunks of every step after the run, in chunk order, so the sums do
+// not depend on the order in which workgroups run, and a box stepped in a
+// batch (chunks aligned to 256 cells there too) gives the same bits as
+// stepped alone.
+__device__ __forceinline__ void obj_fold(double c, double* __restrict__ partial, unsigned chunk) {
+  const double s = wg_sum(c);
+  if (threadIdx.x == 0) partial[chunk] = s;
 }
 
-// jt[t] = sum over z of rows[t * nz + z], t = 1 .. T
-__device__ __forceinline__ void obj_final(const double* __restrict__ rows, int nz, int T, double* __restrict__ jt,
-                                          int t) {
-  if (t < 1 || t > T) return;
-  double s = 0.0;
-  for (int z = 0; z < nz; ++z) s += rows[int64_t(t) * nz + z];
-  jt[t] = s;
+// Workgroup t (t = 1 .. T): jt[t] = sum of the nchunks partial sums of step t
+// (partial + t * nchunks), each thread over a fixed stride, then wg_sum.
+__device__ __forceinline__ void obj_sum(const double* __restrict__ partial, unsigned nchunks, int t,
+                                        double* __restrict__ jt) {
+  const double* p = partial + int64_t(t) * nchunks;
+  double v = 0.0;
+  for (unsigned k = threadIdx.x; k < nchunks; k += kThreads) v += p[k];
+  const double s = wg_sum(v);
+  if (threadIdx.x == 0) jt[t] = s;
 }
 
-__global__ __launch_bounds__(kThreads) void k_obj_final(const double* __restrict__ rows, int nz, int T,
-                                                         double* __restrict__ jt) {
-  obj_final(rows, nz, T, jt, int(blockIdx.x) * kThreads + int(threadIdx.x));
+__global__ __launch_bounds__(kThreads) void k_obj_sum(const double* __restrict__ partial, unsigned nchunks,
+                                                      double* __restrict__ jt) {
+  obj_sum(partial, nchunks, int(blockIdx.x) + 1, jt);
+}
+
+// partial: the step's chunk sums (OBJ)
+template <class R, bool OBJ>
+__global__ __launch_bounds__(kThreads) void k_fwd(AdjArgs<R> a, const R* __restrict__ src,
+                                                  R* __restrict__ dst, double* __restrict__ partial) {
+  const unsigned wg = xcd_block(blockIdx.x, gridDim.x);
+  const double c = fwd_cell<R, OBJ>(a, src, dst, int64_t(wg) * kThreads + threadIdx.x);
+  if constexpr (OBJ) obj_fold(c, partial, wg);
 }
 
 // ---- equilibrium initial state ----------------------------------------------
@@ -369,8 +381,8 @@ struct Entry {
   R* const* slots;
   R* grad;
   double* jt;
-  double* rows;  // per-step plane row sums, [T + 1][nz]
-  int64_t cell0;
+  double* partial;  // objective partial sums, [T + 1][chunks of 256 cells]
+  int64_t cell0;    // first cell in the batch numbering (a multiple of 256)
 };
 
 template <class R>
@@ -384,28 +396,28 @@ __device__ __forceinline__ int entry_of(const Entry<R>* tab, int n, int64_t k) {
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <class R>
+// One workgroup covers 256 cells of one box (boxes start on multiples of
+// 256 in the batch numbering).
+template <class R, bool OBJ>
 __global__ __launch_bounds__(kThreads) void k_fwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
-                                                    int src, int dst) {
-  const int64_t k = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads + threadIdx.x;
-  if (k >= total) return;
-  const Entry<R>& E = tab[entry_of(tab, n, k)];
-  fwd_cell(E.a, E.slots[src], E.slots[dst], k - E.cell0);
+                                                    int src, int dst, int t) {
+  const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t k0 = int64_t(g) * kThreads;
+  if (k0 >= total) return;
+  const Entry<R>& E = tab[entry_of(tab, n, k0)];
+  const int64_t id0 = k0 - E.cell0;
+  const double c = fwd_cell<R, OBJ>(E.a, E.slots[src], E.slots[dst], id0 + threadIdx.x);
+  if constexpr (OBJ) {
+    const unsigned nchunks = unsigned((E.a.N + kThreads - 1) / kThreads);
+    obj_fold(c, E.partial + int64_t(t) * nchunks, unsigned(id0 / kThreads));
+  }
 }
 
-// grid (max nz, boxes)
+// grid (T, boxes)
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_obj_b(const Entry<R>* __restrict__ tab, int f, int t) {
+__global__ __launch_bounds__(kThreads) void k_obj_sum_b(const Entry<R>* __restrict__ tab) {
   const Entry<R>& E = tab[blockIdx.y];
-  if (int(blockIdx.x) >= E.a.nz) return;
-  obj_row(E.a, E.slots[f], int(blockIdx.x), E.rows + int64_t(t) * E.a.nz);
-}
-
-// grid (ceil(T / kThreads), boxes)
-template <class R>
-__global__ __launch_bounds__(kThreads) void k_obj_final_b(const Entry<R>* __restrict__ tab, int T) {
-  const Entry<R>& E = tab[blockIdx.y];
-  obj_final(E.rows, E.a.nz, T, E.jt, int(blockIdx.x) * kThreads + int(threadIdx.x));
+  obj_sum(E.partial, unsigned((E.a.N + kThreads - 1) / kThreads), int(blockIdx.x) + 1, E.jt);
 }
 
 template <class R>
@@ -448,8 +460,8 @@ struct flb_adjoint_s {
   void* d_grad = nullptr;
   double* d_jt = nullptr;
   int jt_cap = 0;
-  double* d_rows = nullptr;  // [T + 1][nz] plane row sums
-  int rows_cap = 0;          // in steps
+  double* d_partial = nullptr;  // objective partial sums, [T + 1][chunks of 256 cells]
+  int partial_cap = 0;          // in steps
   std::vector<void*> ckpt;  // ckpt[k] = state at step k*K (ckpt[0] aliases d_f0)
   std::vector<void*> tape;  // recompute slots, or every state when K == 0
   int T = 0;
@@ -495,18 +507,11 @@ hipError_t build_links(const flb_adjoint_s* h) {
   return hipGetLastError();
 }
 
-template <class R>
-hipError_t fwd_step(const flb_adjoint_s* h, const void* src, void* dst) {
-  hipLaunchKernelGGL(A::k_fwd<R>, dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
-                     static_cast<const R*>(src), static_cast<R*>(dst));
-  return hipGetLastError();
-}
-
-// plane row sums of state t into rows[t]
-template <class R>
-hipError_t obj_of(const flb_adjoint_s* h, const void* f, int t) {
-  hipLaunchKernelGGL(A::k_obj<R>, dim3(unsigned(h->desc.nz)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
-                     static_cast<const R*>(f), h->d_rows + int64_t(t) * h->desc.nz);
+// One forward step src -> dst; OBJ: the chunk sums of its objective (step t).
+template <class R, bool OBJ>
+hipError_t fwd_step(const flb_adjoint_s* h, const void* src, void* dst, int t) {
+  hipLaunchKernelGGL((A::k_fwd<R, OBJ>), dim3(grid_of(h)), dim3(A::kThreads), 0, h->stream, adj_args<R>(h),
+                     static_cast<const R*>(src), static_cast<R*>(dst), h->d_partial + int64_t(t) * grid_of(h));
   return hipGetLastError();
 }
 
@@ -567,14 +572,14 @@ int prepare_forward(flb_adjoint_s* h, int T, int K) {
       return fail(FLB_ERR_NOMEM, "objective buffer allocation failed");
     h->jt_cap = T + 1;
   }
-  if (h->rows_cap < T + 1) {
-    if (h->d_rows) (void)hipFree(h->d_rows);
-    h->d_rows = nullptr;
-    h->rows_cap = 0;
-    if (hipMalloc(reinterpret_cast<void**>(&h->d_rows), size_t(T + 1) * size_t(h->desc.nz) * sizeof(double)) !=
+  if (h->partial_cap < T + 1) {
+    if (h->d_partial) (void)hipFree(h->d_partial);
+    h->d_partial = nullptr;
+    h->partial_cap = 0;
+    if (hipMalloc(reinterpret_cast<void**>(&h->d_partial), size_t(T + 1) * grid_of(h) * sizeof(double)) !=
         hipSuccess)
-      return fail(FLB_ERR_NOMEM, "objective row buffer allocation failed");
-    h->rows_cap = T + 1;
+      return fail(FLB_ERR_NOMEM, "objective partial-sum buffer allocation failed");
+    h->partial_cap = T + 1;
   }
   return FLB_OK;
 }
@@ -605,13 +610,14 @@ int forward_impl(flb_adjoint_s* h, int T, int K, double* J) {
   const void* src = h->d_f0;
   for (int t = 0; t < T; ++t) {
     void* dst = forward_target(h, t, src);
-    HIP_TRY(fwd_step<R>(h, src, dst));
-    HIP_TRY(obj_of<R>(h, dst, t + 1));
+    HIP_TRY((fwd_step<R, true>(h, src, dst, t + 1)));
     src = dst;
   }
-  hipLaunchKernelGGL(A::k_obj_final, dim3(unsigned((T + A::kThreads) / A::kThreads)), dim3(A::kThreads), 0,
-                     h->stream, static_cast<const double*>(h->d_rows), h->desc.nz, T, h->d_jt);
-  HIP_TRY(hipGetLastError());
+  if (T > 0) {
+    hipLaunchKernelGGL(A::k_obj_sum, dim3(unsigned(T)), dim3(A::kThreads), 0, h->stream, h->d_partial, grid_of(h),
+                       h->d_jt);
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipEventRecord(h->e1, h->stream));
   std::vector<double> jt(size_t(T + 1), 0.0);
   if (T > 0)
@@ -666,7 +672,7 @@ int backward_impl(flb_adjoint_s* h, double* grad) {
       // states t0 .. t1-1 of this segment: tape[0] is the checkpoint itself
       h->tape[0] = h->ckpt[size_t(k)];
       for (int t = t0; t < t1 - 1; ++t)
-        HIP_TRY(fwd_step<R>(h, h->tape[size_t(t - t0)], h->tape[size_t(t - t0 + 1)]));
+        HIP_TRY((fwd_step<R, false>(h, h->tape[size_t(t - t0)], h->tape[size_t(t - t0 + 1)], 0)));
       for (int t = t1 - 1; t >= t0; --t) {
         HIP_TRY(bwd_step<R>(h, h->tape[size_t(t - t0)], h->d_lam[cur], h->d_lam[cur ^ 1], t >= 1));
         cur ^= 1;
@@ -702,7 +708,6 @@ struct flb_adjoint_batch_s {
   std::vector<int64_t> cell0;
   int64_t total = 0;
   int64_t cells = 0;  // sum of member cell counts
-  int max_nz = 1;
   size_t real = 4;
   void* d_tab = nullptr;
   size_t tab_cap = 0;
@@ -771,7 +776,7 @@ int upload_table(flb_adjoint_batch_s* b) {
     tab[m].slots = dslots + m * size_t(nslots);
     tab[m].grad = static_cast<R*>(h->d_grad);
     tab[m].jt = h->d_jt;
-    tab[m].rows = h->d_rows;
+    tab[m].partial = h->d_partial;
     tab[m].cell0 = b->cell0[m];
   }
   HIP_TRY(hipMemcpyAsync(b->d_slots, slots.data(), bytes_slots, hipMemcpyHostToDevice, b->stream));
@@ -785,10 +790,11 @@ unsigned batch_grid(const flb_adjoint_batch_s* b) {
   return unsigned((b->total + A::kThreads - 1) / A::kThreads);
 }
 
-template <class R>
-hipError_t fwd_batch(const flb_adjoint_batch_s* b, int src, int dst) {
-  hipLaunchKernelGGL(A::k_fwd_b<R>, dim3(batch_grid(b)), dim3(A::kThreads), 0, b->stream,
-                     static_cast<const A::Entry<R>*>(b->d_tab), int(b->members.size()), b->total, src, dst);
+// One forward step of every box, slot src -> dst; OBJ: the objectives into jt[t].
+template <class R, bool OBJ>
+hipError_t fwd_batch(const flb_adjoint_batch_s* b, int src, int dst, int t) {
+  hipLaunchKernelGGL((A::k_fwd_b<R, OBJ>), dim3(batch_grid(b)), dim3(A::kThreads), 0, b->stream,
+                     static_cast<const A::Entry<R>*>(b->d_tab), int(b->members.size()), b->total, src, dst, t);
   return hipGetLastError();
 }
 
@@ -812,21 +818,19 @@ int batch_forward_impl(flb_adjoint_batch_s* b, int T, int K, double* J) {
   b->T = T;
   b->K = K;
   if ((rc = upload_table<R>(b))) return rc;
-  const auto* tab = static_cast<const A::Entry<R>*>(b->d_tab);
   const unsigned nm = unsigned(b->members.size());
   HIP_TRY(hipEventRecord(b->e0, b->stream));
   int src = kSlotF0;
   for (int t = 0; t < T; ++t) {
     const int dst = forward_slot(T, K, t, src);
-    HIP_TRY(fwd_batch<R>(b, src, dst));
-    hipLaunchKernelGGL(A::k_obj_b<R>, dim3(unsigned(b->max_nz), nm), dim3(A::kThreads), 0, b->stream, tab, dst,
-                       t + 1);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY((fwd_batch<R, true>(b, src, dst, t + 1)));
     src = dst;
   }
-  hipLaunchKernelGGL(A::k_obj_final_b<R>, dim3(unsigned((T + A::kThreads) / A::kThreads), nm), dim3(A::kThreads), 0,
-                     b->stream, tab, T);
-  HIP_TRY(hipGetLastError());
+  if (T > 0) {
+    hipLaunchKernelGGL(A::k_obj_sum_b<R>, dim3(unsigned(T), nm), dim3(A::kThreads), 0, b->stream,
+                       static_cast<const A::Entry<R>*>(b->d_tab));
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipEventRecord(b->e1, b->stream));
   std::vector<double> jt(size_t(nm) * size_t(T + 1), 0.0);
   for (unsigned m = 0; m < nm; ++m)
@@ -882,7 +886,7 @@ int batch_backward_impl(flb_adjoint_batch_s* b, double* grad) {
     for (int k = nseg - 1; k >= 0; --k) {
       const int t0 = k * K, t1 = std::min((k + 1) * K, T);
       auto state = [&](int t) { return t == t0 ? kSlotCk + k : tb + (t - t0); };
-      for (int t = t0; t < t1 - 1; ++t) HIP_TRY(fwd_batch<R>(b, state(t), state(t + 1)));
+      for (int t = t0; t < t1 - 1; ++t) HIP_TRY((fwd_batch<R, false>(b, state(t), state(t + 1), 0)));
       for (int t = t1 - 1; t >= t0; --t) {
         HIP_TRY(bwd_batch<R>(b, state(t), kSlotL0 + cur, kSlotL0 + (cur ^ 1), t >= 1));
         cur ^= 1;
@@ -952,7 +956,7 @@ int flb_adjoint_destroy(flb_adjoint h) {
   free_vec(h->ckpt, h->d_f0);
   free_vec(h->tape, h->d_f0);
   void* ptrs[] = {h->d_solid, h->d_link, h->d_s, h->d_f0, h->d_work[0], h->d_work[1], h->d_lam[0], h->d_lam[1],
-                  h->d_grad, h->d_jt, h->d_rows};
+                  h->d_grad, h->d_jt, h->d_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->e0) (void)hipEventDestroy(h->e0);
@@ -1111,9 +1115,8 @@ int flb_adjoint_batch_create(const flb_adjoint* members, int n, void* stream, fl
   b->members.assign(members, members + n);
   for (int m = 0; m < n; ++m) {
     b->cell0.push_back(b->total);
-    b->total += (members[m]->N + 63) / 64 * 64;
+    b->total += (members[m]->N + A::kThreads - 1) / A::kThreads * A::kThreads;
     b->cells += members[m]->N;
-    b->max_nz = std::max(b->max_nz, members[m]->desc.nz);
   }
   DeviceGuard g(b->device);
   if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
