@@ -367,56 +367,53 @@ __device__ __forceinline__ void force_rhou(const R (&c)[9], const R (&F)[3], R& 
   u[2] = R(0);
 }
 
-// force::ForcePopImpl (the D3Q19 / D2Q9 forms of force.ur.h; generic form
-// force.h:29-36): Fi = w_i InvCs2 sum_a F_a (c_ia - u_a + InvCs2 (c_i.u) c_ia),
-// with the zero components of c_i left out as the unrolled form does.
-template <class R>
-__device__ __forceinline__ void force_pop(const R (&u)[3], const R (&F)[3], R (&Fi)[19]) {
-  const R I2 = R(3);
-  const R u0 = u[0], u1 = u[1], u2 = u[2], F0 = F[0], F1 = F[1], F2 = F[2];
-  const R T1_u0 = R(1) - u0, _T1_u0 = R(-1) - u0, _u0 = -u0;
-  const R T1_u1 = R(1) - u1, _T1_u1 = R(-1) - u1, _u1 = -u1;
-  const R T1_u2 = R(1) - u2, _T1_u2 = R(-1) - u2, _u2 = -u2;
-  const R c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u2), c7 = I2 * (u0 + u1), c9 = I2 * (u0 + u2),
-          c11 = I2 * (u1 + u2), c13 = I2 * (u0 - u1), c15 = I2 * (u0 - u2), c17 = I2 * (u1 - u2);
-  using L = Lat<19>;
-  Fi[0] = L::template w<R>(0) * I2 * (-(F0 * u0 + F1 * u1 + F2 * u2));
-  Fi[1] = L::template w<R>(1) * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
-  Fi[2] = L::template w<R>(2) * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1) + F2 * (_u2));
-  Fi[3] = L::template w<R>(3) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3) + F2 * (_u2));
-  Fi[4] = L::template w<R>(4) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3) + F2 * (_u2));
-  Fi[5] = L::template w<R>(5) * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (T1_u2 + c5));
-  Fi[6] = L::template w<R>(6) * I2 * (F0 * (_u0) + F1 * (_u1) + F2 * (_T1_u2 + c5));
-  Fi[7] = L::template w<R>(7) * I2 * (F0 * (T1_u0 + c7) + F1 * (T1_u1 + c7) + F2 * (_u2));
-  Fi[8] = L::template w<R>(8) * I2 * (F0 * (_T1_u0 + c7) + F1 * (_T1_u1 + c7) + F2 * (_u2));
-  Fi[9] = L::template w<R>(9) * I2 * (F0 * (T1_u0 + c9) + F1 * (_u1) + F2 * (T1_u2 + c9));
-  Fi[10] = L::template w<R>(10) * I2 * (F0 * (_T1_u0 + c9) + F1 * (_u1) + F2 * (_T1_u2 + c9));
-  Fi[11] = L::template w<R>(11) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c11) + F2 * (T1_u2 + c11));
-  Fi[12] = L::template w<R>(12) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c11) + F2 * (_T1_u2 + c11));
-  Fi[13] = L::template w<R>(13) * I2 * (F0 * (T1_u0 + c13) + F1 * (_T1_u1 - c13) + F2 * (_u2));
-  Fi[14] = L::template w<R>(14) * I2 * (F0 * (_T1_u0 + c13) + F1 * (T1_u1 - c13) + F2 * (_u2));
-  Fi[15] = L::template w<R>(15) * I2 * (F0 * (T1_u0 + c15) + F1 * (_u1) + F2 * (_T1_u2 - c15));
-  Fi[16] = L::template w<R>(16) * I2 * (F0 * (_T1_u0 + c15) + F1 * (_u1) + F2 * (T1_u2 - c15));
-  Fi[17] = L::template w<R>(17) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c17) + F2 * (_T1_u2 - c17));
-  Fi[18] = L::template w<R>(18) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c17) + F2 * (T1_u2 - c17));
+// c_i . u of the first direction of opposite pair p (directions 2p+1, 2p+2),
+// summed the way the unrolled specialisations write it (first nonzero
+// component first): u0, u1, u2, u0+u1, u0+u2, u1+u2, u0-u1, u0-u2, u1-u2
+// (D3Q19), u0, u1, u0+u1, u0-u1 (D2Q9).
+template <int Q, class R>
+__device__ __forceinline__ R pair_cu(int p, const R (&u)[3]) {
+  return cdot_folded<Q>(2 * p + 1, u);
 }
-template <class R>
-__device__ __forceinline__ void force_pop(const R (&u)[3], const R (&F)[3], R (&Fi)[9]) {
+
+// force::ForcePopImpl (the D3Q19 / D2Q9 forms of force.ur.h; generic form
+// force.h:29-36): Fi = w_i InvCs2 sum_a F_a (c_ia - u_a + InvCs2 (c_i.u) c_ia).
+// The unrolled form leaves out the zero components of c_i: a term is
+// -u_a where c_ia = 0, else (+-1 - u_a) +- InvCs2 (c.u) of the pair, with
+// the pair's c.u formed for its first direction (first component +). The
+// terms are generated from c_i in that association; the rest direction is
+// w_0 InvCs2 (-(F.u)).
+template <int Q, class R>
+__device__ __forceinline__ void force_pop(const R (&u)[3], const R (&F)[3], R (&Fi)[Q]) {
+  constexpr int D = Lat<Q>::D;
   const R I2 = R(3);
-  const R u0 = u[0], u1 = u[1], F0 = F[0], F1 = F[1];
-  const R T1_u0 = R(1) - u0, _T1_u0 = R(-1) - u0, _u0 = -u0;
-  const R T1_u1 = R(1) - u1, _T1_u1 = R(-1) - u1, _u1 = -u1;
-  const R c1 = I2 * (u0), c3 = I2 * (u1), c5 = I2 * (u0 + u1), c7 = I2 * (u0 - u1);
-  using L = Lat<9>;
-  Fi[0] = L::template w<R>(0) * I2 * (-(F0 * u0 + F1 * u1));
-  Fi[1] = L::template w<R>(1) * I2 * (F0 * (T1_u0 + c1) + F1 * (_u1));
-  Fi[2] = L::template w<R>(2) * I2 * (F0 * (_T1_u0 + c1) + F1 * (_u1));
-  Fi[3] = L::template w<R>(3) * I2 * (F0 * (_u0) + F1 * (T1_u1 + c3));
-  Fi[4] = L::template w<R>(4) * I2 * (F0 * (_u0) + F1 * (_T1_u1 + c3));
-  Fi[5] = L::template w<R>(5) * I2 * (F0 * (T1_u0 + c5) + F1 * (T1_u1 + c5));
-  Fi[6] = L::template w<R>(6) * I2 * (F0 * (_T1_u0 + c5) + F1 * (_T1_u1 + c5));
-  Fi[7] = L::template w<R>(7) * I2 * (F0 * (T1_u0 + c7) + F1 * (_T1_u1 - c7));
-  Fi[8] = L::template w<R>(8) * I2 * (F0 * (_T1_u0 + c7) + F1 * (T1_u1 - c7));
+  R T1[3], _T1[3], nu[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    T1[a] = R(1) - u[a];
+    _T1[a] = R(-1) - u[a];
+    nu[a] = -u[a];
+  }
+  R fu = F[0] * u[0] + F[1] * u[1];
+  if constexpr (D == 3) fu = fu + F[2] * u[2];
+  Fi[0] = Lat<Q>::template w<R>(0) * I2 * (-fu);
+#pragma unroll
+  for (int p = 0; p < (Q - 1) / 2; ++p) {
+    const R cp = I2 * pair_cu<Q>(p, u);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = 2 * p + 1 + h;  // h = 1: the opposite direction, whose c.u is -cp
+      R acc = R(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const int c = a == 0 ? Lat<Q>::cx(k) : (a == 1 ? Lat<Q>::cy(k) : Lat<Q>::cz(k));
+        const R t = c == 0 ? nu[a] : ((c > 0) == (h == 0) ? (c > 0 ? T1[a] : _T1[a]) + cp
+                                                             : (c > 0 ? T1[a] : _T1[a]) - cp);
+        acc = a == 0 ? F[0] * t : acc + F[a] * t;
+      }
+      Fi[k] = Lat<Q>::template w<R>(k) * I2 * acc;
+    }
+  }
 }
 
 // collision::BGKForce (collision.h:68-87): omega feq + (1 - omega) f +
@@ -441,32 +438,6 @@ __device__ __forceinline__ void bgk_force(R (&f)[Q], const R (&F)[3], R omega, R
 // lattice sets it.
 // ---------------------------------------------------------------------------
 
-// c_i . u of the first direction of opposite pair p (directions 2p+1, 2p+2),
-// summed the way the unrolled specialisations write it: u0, u1, u2, u0+u1,
-// u0+u2, u1+u2, u0-u1, u0-u2, u1-u2 (D3Q19), u0, u1, u0+u1, u0-u1 (D2Q9).
-template <int Q, class R>
-__device__ __forceinline__ R pair_cu(int p, const R (&u)[3]) {
-  if constexpr (Q == 19) {
-    switch (p) {
-      case 0: return u[0];
-      case 1: return u[1];
-      case 2: return u[2];
-      case 3: return u[0] + u[1];
-      case 4: return u[0] + u[2];
-      case 5: return u[1] + u[2];
-      case 6: return u[0] - u[1];
-      case 7: return u[0] - u[2];
-      default: return u[1] - u[2];
-    }
-  } else {
-    switch (p) {
-      case 0: return u[0];
-      case 1: return u[1];
-      case 2: return u[0] + u[1];
-      default: return u[0] - u[1];
-    }
-  }
-}
 
 // forcerhoUImpl's scalar overload (moment.h:423-435, unrolled at
 // moment.ur.h:564-574, 652-663): rho and the sums c f, then F/2 added to the
@@ -579,6 +550,19 @@ __device__ __forceinline__ double dsqrt<double>(double v) {
   return sqrt(v);
 }
 
+// The Smagorinsky omega for |Pi_neq| = norm (collisionLES.h:36-59); the
+// coefficient is formed in double because std::sqrt(2) is a double, and with
+// the usual RHO field of 1 its double division is by 1 and is skipped (the
+// same bits).
+template <class R>
+__device__ __forceinline__ R smag_from_norm(R norm, R omega, R cs, R rho_f) {
+  const double cd = double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951;
+  const R coeff = rho_f == R(1) ? R(cd) : R(cd / double(rho_f));
+  const R rt_mol = R(1) / omega;
+  const R rt_tur = R(0.5) * (dsqrt(rt_mol * rt_mol + coeff * norm) - rt_mol);
+  return R(1) / (rt_mol + rt_tur);
+}
+
 // Smagorinsky effective omega (collisionLES.h:36-59): |Pi_neq| from
 // Pi_ab_neq (moment.ur.h:708-718, 750-763) and NormSquare (util.h:418-434);
 // the coefficient is formed in double because std::sqrt(2) is a double.
@@ -596,14 +580,7 @@ __device__ __forceinline__ R smag_omega(const R (&c)[19], R rho, const R (&u)[3]
   const R t4 = c[11] + c[12] - c[17] - c[18] - rho * (u[1] * u[2]);
   const R t5 = diag(c[5] + c[6] + c[9] + c[10] + c[11] + c[12] + c[15] + c[16] + c[17] + c[18], u[2]);
   const R ns = t0 * t0 + t1 * t1 * R(2) + t2 * t2 * R(2) + t3 * t3 + t4 * t4 * R(2) + t5 * t5;
-  const R norm = dsqrt(ns);
-  // with the usual RHO field of 1 the double division is by 1 and is skipped
-  // (the same bits)
-  const double cd = double(cs * cs * R(3) * R(3) * R(2)) * 1.4142135623730951;
-  const R coeff = rho_f == R(1) ? R(cd) : R(cd / double(rho_f));
-  const R rt_mol = R(1) / omega;
-  const R rt_tur = R(0.5) * (dsqrt(rt_mol * rt_mol + coeff * norm) - rt_mol);
-  return R(1) / (rt_mol + rt_tur);
+  return smag_from_norm(dsqrt(ns), omega, cs, rho_f);
 }
 
 template <class R>
@@ -780,6 +757,121 @@ __device__ __forceinline__ void mrt(R (&f)[19], R d, R rho, const R (&u)[3], R o
     });
     f[2 * p + 1] -= E + O;
     f[2 * p + 2] -= E - O;
+  });
+}
+
+// D3Q19 MRT, with the Smagorinsky shear rates when `les`, on the deviations
+// g = f - w of the shifted storage kinds (d = rho - 1), from the moments
+// themselves: m = M g through the pair sums and differences, less the
+// equilibrium moments in closed form. For the second-order feq, M feq - M w
+// is
+//   e: -11 d + 19 rho u.u      eps: 3 d - 5.5 rho u.u    q_a: -2/3 rho u_a
+//   3pxx: rho (3 ux^2 - u.u)   3pixx: -3pxx / 2          pww: rho (uy^2 - uz^2)
+//   piww: -pww / 2             p_ab: rho u_a u_b          m_a: 0
+// (M w is 1, -11 and 3 in rows rho, e, eps, 0 elsewhere). The conserved rows
+// (rho, j) have zero non-equilibrium part and are not relaxed. The LES
+// stress is read from the same moments: with T = sum c^2 n = dm_e / 19,
+// Pi_xx = (dm_3pxx + T) / 3, Pi_yy, Pi_zz = (T - Pi_xx +- dm_pww) / 2 and
+// Pi_xy, Pi_yz, Pi_xz the p rows, the same |Pi_neq| as smag_omega in exact
+// arithmetic. No feq and no f - feq are formed.
+__host__ __device__ constexpr bool m19_conserved(int j) { return j == 0 || j == 3 || j == 5 || j == 7; }
+
+// The density and velocity come from the same pair sums and differences:
+// rho - 1 = g_0 + sum_p s_p, rho u_a = sum_p c_ai d_p.
+template <class R>
+__device__ __forceinline__ void mrt_dev(R (&g)[19], R om, bool les, R cs, R rho_f, const R* sn, uint32_t shear) {
+  R sp[9], dp[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    sp[p] = g[2 * p + 1] + g[2 * p + 2];
+    dp[p] = g[2 * p + 1] - g[2 * p + 2];
+  }
+  R d = g[0];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) d += sp[p];
+  const R rho = R(1) + d;
+  const R inv = R(1) / rho;
+  R u[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    R j = R(0);
+    bool any = false;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      const int c = k == 0 ? Lat<19>::cx(2 * p + 1) : (k == 1 ? Lat<19>::cy(2 * p + 1) : Lat<19>::cz(2 * p + 1));
+      if (c > 0) {
+        j = any ? j + dp[p] : dp[p];
+        any = true;
+      } else if (c < 0) {
+        j = any ? j - dp[p] : -dp[p];
+        any = true;
+      }
+    }
+    u[k] = j * inv;
+  }
+  R dm[19];
+  static_for<19>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (!m19_conserved(j)) {
+      R m = R(0);
+      if constexpr (!m19_odd(j) && m19(j, 0) != 0) m += R(m19(j, 0)) * g[0];
+      static_for<9>([&](auto P) {
+        constexpr int p = decltype(P)::value;
+        constexpr int c = m19(j, 2 * p + 1);
+        if constexpr (c != 0) m += R(c) * (m19_odd(j) ? dp[p] : sp[p]);
+      });
+      dm[j] = m;
+    }
+  });
+  const R rx = rho * u[0], ry = rho * u[1], rz = rho * u[2];
+  const R xx = rx * u[0], yy = ry * u[1], zz = rz * u[2];
+  const R uu = xx + yy + zz;
+  const R p3xx = R(3) * xx - uu, pww = yy - zz;
+  dm[1] -= R(19) * uu - R(11) * d;
+  dm[2] -= R(3) * d - R(5.5) * uu;
+  dm[4] += R(2.0 / 3.0) * rx;
+  dm[6] += R(2.0 / 3.0) * ry;
+  dm[8] += R(2.0 / 3.0) * rz;
+  dm[9] -= p3xx;
+  dm[10] += R(0.5) * p3xx;
+  dm[11] -= pww;
+  dm[12] += R(0.5) * pww;
+  dm[13] -= rx * u[1];
+  dm[14] -= ry * u[2];
+  dm[15] -= rx * u[2];
+  R eo = om;
+  if (les) {
+    const R T = dm[1] * R(1.0 / 19.0);
+    const R pxx = (dm[9] + T) * R(1.0 / 3.0);
+    const R pyy = R(0.5) * (T - pxx + dm[11]);
+    const R pzz = R(0.5) * (T - pxx - dm[11]);
+    const R ns = pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (dm[13] * dm[13] + dm[14] * dm[14] + dm[15] * dm[15]);
+    eo = smag_from_norm(dsqrt(ns), om, cs, rho_f);
+  }
+  R k[19];
+  static_for<19>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (!m19_conserved(j)) k[j] = dm[j] * mrt_factor<R>(j, sn, shear, eo);
+  });
+  R d0 = R(0);
+  static_for<19>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (!m19_conserved(j) && !m19_odd(j) && m19(j, 0) != 0) d0 += R(m19(j, 0)) * k[j];
+  });
+  g[0] -= d0;
+  static_for<9>([&](auto P) {
+    constexpr int p = decltype(P)::value;
+    R E = R(0), O = R(0);
+    static_for<19>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      constexpr int c = m19(j, 2 * p + 1);
+      if constexpr (c != 0 && !m19_conserved(j)) {
+        if constexpr (m19_odd(j)) O += R(c) * k[j];
+        else E += R(c) * k[j];
+      }
+    });
+    g[2 * p + 1] -= E + O;
+    g[2 * p + 2] -= E - O;
   });
 }
 

@@ -218,6 +218,18 @@ __device__ __forceinline__ void collide_full(const KArgs& a, int slot, int dyn, 
     if constexpr (DEV) sub_w<Q>(f);
     return;
   }
+  if constexpr (DEV) {
+    // MRT and MRT + LES from the moments of the deviations (mrt_dev)
+    if (dyn == DYN_MRT || dyn == DYN_SMAG_MRT) {
+      const R* sn;
+      if constexpr (sizeof(R) == 8) sn = a.mrt_sn_d;
+      else sn = a.mrt_sn_f;
+      rho_f = R(1);
+      if (dyn == DYN_SMAG_MRT) load_fields<Q>(a, slot, rho_f, u_f);
+      mrt_dev(f, om, dyn == DYN_SMAG_MRT, R(cs), rho_f, sn, a.mrt_shear);
+      return;
+    }
+  }
   R d = R(0);
   if constexpr (DEV) rhou_dev<Q>(f, d, rho, u);
   else rhou(f, rho, u);
