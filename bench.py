@@ -459,8 +459,11 @@ def cpu_baseline(n=N_EDGE, steps=200, warmup=5):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
         out = subprocess.run([exe, "d3q19", "hw", str(n), str(n), str(n), str(steps), "0.53", "0.01", "0.1",
                               f"time:{warmup}"], env=env, capture_output=True, text=True, timeout=900).stdout
-        fields = out.split()
-        if fields and fields[0] == "MLUPS":
+        # FreeLB prints its geometry summary first; the harness's result line
+        # reads "MLUPS v cells N steps S seconds T threads n"
+        lines = [ln.split() for ln in out.splitlines() if ln.startswith("MLUPS ")]
+        fields = lines[-1] if lines else []
+        if len(fields) >= 8:
             rec = {"value": float(fields[1]), "unit": "MLUPS", "cores": threads, "kind": "reference",
                    "sample": f"FreeLB halfway cavity (BGK<rhoU> + bounceback normal / movingwall managers), "
                              f"{n}^3 cells, {steps} timed steps after {warmup}, OpenMP {threads} threads, "

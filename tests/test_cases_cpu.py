@@ -1,4 +1,7 @@
-"""Host-side set-up helpers: cavity flags and the vorticity estimator."""
+"""Host-side set-up helpers: cavity flags, the vorticity estimator and the
+CPU baseline of bench.py."""
+import os
+
 import numpy as np
 import pytest
 
@@ -26,3 +29,15 @@ def test_vorticity_of_linear_fields():
     np.testing.assert_allclose(w[0], 0.2, atol=1e-12)
     np.testing.assert_allclose(w[1], 0.1, atol=1e-12)
     np.testing.assert_allclose(w[2], 0.8, atol=1e-12)
+
+
+def test_cpu_baseline_times_the_reference_harness():
+    """bench.py's cpu_baseline runs FreeLB's halfway cavity built from the
+    reference sources (oracle/_ref/cavity_hw_bench) and reads its result line
+    past FreeLB's own geometry printout; the C oracle is only the fallback."""
+    exe = os.path.join(os.path.dirname(bench.__file__), "oracle", "_ref", "cavity_hw_bench")
+    if not os.path.exists(exe):
+        pytest.skip("reference harness not built here")
+    rec = bench.cpu_baseline(n=24, steps=4, warmup=1)
+    assert rec["kind"] == "reference", rec
+    assert rec["value"] > 0 and rec["unit"] == "MLUPS"
