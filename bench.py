@@ -248,21 +248,31 @@ def batch_bench(timer, nblocks=512, seed=2025, graph_steps=500, storage="f32", l
             "diverged_blocks": [(k, round(specs[k]["tau"], 4), round(specs[k]["u_lid"], 3)) for k in diverged]}
 
 
-# algorithmic HBM bytes per cell and step of the FP32 adjoint kernels: the
-# forward push reads and writes 19 populations plus s (4 B) and the solid
-# byte; the adjoint step reads the stored state and 19 adjoint values,
-# writes 19 adjoint values, reads s and the solid byte and updates dJ/ds (8 B)
-ADJ_FWD_BYTES = 19 * 4 * 2 + 4 + 1
-ADJ_BWD_BYTES = 19 * 4 * 3 + 4 + 1 + 8
+# algorithmic HBM bytes per cell of the FP32 adjoint kernels, for boxes
+# without solid cells (their link words come from the cell's place; a box
+# with solid cells also reads a 4-byte link word per cell and kernel):
+# - forward push: reads and writes 19 populations, reads s (4 B);
+# - moment record (rho - 1, u) a recompute step, or a forward step with
+#   checkpoint_every 0, also writes: 16 B;
+# - moment record alone (the last state of each recomputed segment): reads
+#   19 populations and s, writes 16 B;
+# - adjoint step: reads the moment record and 19 adjoint values, writes 19
+#   adjoint values, reads s and updates dJ/ds (8 B).
+ADJ_FWD_BYTES = 19 * 4 * 2 + 4
+ADJ_MOM_BYTES = 4 * 4
+ADJ_MOMONLY_BYTES = 19 * 4 + 4 + 4 * 4
+ADJ_BWD_BYTES = 4 * 4 + 19 * 4 * 2 + 4 + 8
 
 
 def _adjoint_rates(cells, T, K, fwd_ms, bwd_ms):
     """Cell updates per second of the forward pass, of the adjoint pass
-    (T adjoint steps plus the T - ceil(T/K) recomputed forward steps) and of
-    the whole gradient, with the bandwidth each pass reaches."""
-    rec = T - (-(-T // K)) if K > 0 else 0
-    fwd_b = cells * T * ADJ_FWD_BYTES
-    bwd_b = cells * (T * ADJ_BWD_BYTES + rec * ADJ_FWD_BYTES)
+    (T adjoint steps plus the T - ceil(T/K) recomputed forward steps and the
+    ceil(T/K) moment records of segment ends) and of the whole gradient, with
+    the bandwidth each pass reaches."""
+    nseg = -(-T // K) if K > 0 else 0
+    rec = T - nseg if K > 0 else 0
+    fwd_b = cells * T * (ADJ_FWD_BYTES + (ADJ_MOM_BYTES if K == 0 else 0))
+    bwd_b = cells * (T * ADJ_BWD_BYTES + rec * (ADJ_FWD_BYTES + ADJ_MOM_BYTES) + nseg * ADJ_MOMONLY_BYTES)
     return {"forward_mlups": round(cells * T / fwd_ms / 1e3, 1),
             "adjoint_mlups": round(cells * T / bwd_ms / 1e3, 1),
             "gradient_mlups": round(cells * T / (fwd_ms + bwd_ms) / 1e3, 1),
@@ -286,7 +296,8 @@ def adjoint_bench(T=800, K=28):
         J = adj.forward(T, K)
         g = adj.backward()
         fwd_ms, bwd_ms = adj.timing()
-    out = {"workload": f"channel 192x96x96, T {T}, checkpoint every {K}, FP32", "cells": int(s.size),
+    rec = f"checkpoint every {K}" if K > 0 else "moment record of every state (checkpoint_every 0), no recompute"
+    out = {"workload": f"channel 192x96x96, T {T}, {rec}, FP32", "cells": int(s.size),
            "J": J, "grad_finite": bool(np.isfinite(g).all())}
     out.update(_adjoint_rates(s.size, T, K, fwd_ms, bwd_ms))
     return out
@@ -503,6 +514,9 @@ def main():
     ap.add_argument("--mrt-les", action="store_true",
                     help="time the 512^3 MRT+LES cavity (Re 5000) for --steps steps per storage kind instead")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint runs")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint workloads (channel with and without recompute, 64-box batch), "
+                         "e.g. under a profiler")
     ap.add_argument("--no-refined", action="store_true", help="skip the refined-cavity run")
     ap.add_argument("--no-coupled", action="store_true", help="skip the coupled flow + temperature run")
     ap.add_argument("--batch-eager", action="store_true",
@@ -514,6 +528,11 @@ def main():
               "with a face exchange per step).", file=sys.stderr)
         sys.exit(2)
 
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
+        print(json.dumps({"adjoint_no_recompute": adjoint_bench(K=0)}), flush=True)
+        print(json.dumps({"adjoint_batch": adjoint_batch_bench()}), flush=True)
+        return
     if args.mrt_les:
         st = tuple(x for x in args.storages.split(",") if x != "bf16s")
         print(json.dumps({"mrt_les_512": mrt_les_speed(HipTimer(), storages=st, steps=args.steps,
@@ -601,6 +620,9 @@ def main():
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
+        # the same channel without recompute: T moment records, 22.6 GB of HBM
+        line["adjoint_no_recompute"] = adjoint_bench(K=0)
+        print(f"# adjoint, no recompute: {line['adjoint_no_recompute']}", file=sys.stderr, flush=True)
         line["adjoint_batch"] = adjoint_batch_bench()
         print(f"# adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
     if not args.no_cpu:

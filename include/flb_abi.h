@@ -420,7 +420,9 @@ int flb_refine_run(flb_refine r, int64_t count0, int n);
  * J = (1/T) sum_{t=1..T} sum over the plane x = outlet_x of sum_i c_ix f_i,
  * summed in FP64. flb_adjoint_backward returns dJ/ds from the hand-written
  * discrete adjoint of the step, recomputing forward states from
- * checkpoints. No reference counterpart (FreeLB has no adjoint). */
+ * checkpoints. No reference counterpart (FreeLB has no adjoint). A box holds
+ * at most 2^32 bytes per population array (2^30 cells in FP32, 2^29 in
+ * FP64); create fails with FLB_ERR_ARG beyond that. */
 typedef struct flb_adjoint_s* flb_adjoint;
 typedef struct {
   int nx, ny, nz;
@@ -439,9 +441,14 @@ int flb_adjoint_set_fraction(flb_adjoint h, const double* s);         /* N value
 int flb_adjoint_set_fraction_at(flb_adjoint h, int64_t cell, double s);
 /* f^0 = feq(rho, u) from host rhou[4][N]. */
 int flb_adjoint_init_equilibrium(flb_adjoint h, const double* rhou);
-/* T steps from f^0; *J gets the objective. checkpoint_every: k > 0 keeps
- * every k-th state and recomputes the others in the backward pass (k near
- * sqrt(T) balances memory and recompute), 0 keeps every state, -1 records
+/* T steps from f^0; *J gets the objective. The adjoint steps read a forward
+ * state only through its moments (rho - 1 and u: 4 values per cell, against
+ * 19 populations). checkpoint_every: k > 0 keeps every k-th state and, in the
+ * backward pass, recomputes each segment of k states from its checkpoint,
+ * recording their moments (memory: ceil(T/k) + 6 states and k moment
+ * records of 4/19 of a state each; k near sqrt(T) balances memory and
+ * recompute); 0 records the moments of every state during the forward pass
+ * and recomputes nothing (memory: 5 states and T moment records); -1 records
  * nothing (objective only, e.g. for finite differences). */
 int flb_adjoint_forward(flb_adjoint h, int steps, int checkpoint_every, double* J);
 /* dJ/ds for every cell (0 on solid cells) after a recorded forward run. */

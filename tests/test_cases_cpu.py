@@ -41,3 +41,19 @@ def test_cpu_baseline_times_the_reference_harness():
     rec = bench.cpu_baseline(n=24, steps=4, warmup=1)
     assert rec["kind"] == "reference", rec
     assert rec["value"] > 0 and rec["unit"] == "MLUPS"
+
+
+def test_adjoint_byte_accounting():
+    """Bytes behind the adjoint GB/s figures. T = 800, K = 28: 29 segments,
+    so 771 recompute steps of 172 B and 29 moment records of 96 B beside the
+    800 adjoint steps of 180 B; K = 0 recomputes nothing and its forward steps
+    write the 16-byte moment record too."""
+    cells, T = 1000, 800
+    r = bench._adjoint_rates(cells, T, 28, 1.0, 1.0)
+    assert r["recomputed_steps"] == 771
+    assert r["backward_gbs"] == round(cells * (800 * 180 + 771 * 172 + 29 * 96) / 1e6, 1)
+    assert r["forward_gbs"] == round(cells * 800 * 156 / 1e6, 1)
+    r = bench._adjoint_rates(cells, T, 0, 1.0, 1.0)
+    assert r["recomputed_steps"] == 0
+    assert r["backward_gbs"] == round(cells * 800 * 180 / 1e6, 1)
+    assert r["forward_gbs"] == round(cells * 800 * 172 / 1e6, 1)
