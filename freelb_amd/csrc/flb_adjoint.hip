@@ -684,15 +684,12 @@ __device__ __forceinline__ R* slot(const Entry<R>& E, int k) {
   return ((const __attribute__((address_space(4))) P*)E.slots)[k];
 }
 
+// The box of logical workgroup g: the table holds, after its n entries, one
+// int per workgroup of the grid (every workgroup covers 256 cells of one box).
+// A scalar load, in place of a search over the entries' first cells.
 template <class R>
-__device__ __forceinline__ int entry_of(const Entry<R>* tab, int n, int64_t k) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tab[mid].cell0 <= k) lo = mid;
-    else hi = mid - 1;
-  }
-  return __builtin_amdgcn_readfirstlane(lo);
+__device__ __forceinline__ const Entry<R>& entry_of(const Entry<R>* tab, int n, unsigned g) {
+  return tab[((const __attribute__((address_space(4))) int*)(tab + n))[g]];
 }
 
 // One workgroup covers 256 cells of one box (boxes start on multiples of
@@ -704,7 +701,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_b(const Entry<R>* __restrict__
   const unsigned g = xcd_block(blockIdx.x, gridDim.x);
   const int64_t k0 = int64_t(g) * kThreads;
   if (k0 >= total) return;
-  const Entry<R>& E = tab[entry_of(tab, n, k0)];
+  const Entry<R>& E = entry_of(tab, n, g);
   const int64_t id0 = k0 - E.cell0;
   const double c =
       fwd_cell<R, OBJ, MOM>(E.a, slot(E, src), slot(E, dst), MOM ? slot(E, mom) : nullptr, id0 + threadIdx.x);
@@ -726,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void k_seed_b(const Entry<R>* __restrict_
                                                      int lam) {
   const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (k >= total) return;
-  const Entry<R>& E = tab[entry_of(tab, n, k)];
+  const Entry<R>& E = entry_of(tab, n, blockIdx.x);
   seed_cell(E.a, E.slots[lam], k - E.cell0);
 }
 
@@ -735,16 +732,17 @@ __global__ __launch_bounds__(kThreads) void k_mom_b(const Entry<R>* __restrict__
                                                     int src, int mom) {
   const int64_t k0 = int64_t(blockIdx.x) * kThreads;
   if (k0 >= total) return;
-  const Entry<R>& E = tab[entry_of(tab, n, k0)];
+  const Entry<R>& E = entry_of(tab, n, blockIdx.x);
   mom_cell(E.a, slot(E, src), slot(E, mom), k0 - E.cell0 + threadIdx.x);
 }
 
 template <class R>
 __global__ __launch_bounds__(kThreads) void k_bwd_b(const Entry<R>* __restrict__ tab, int n, int64_t total,
                                                     int mom, int lam1, int lam0, int with_obj) {
-  const int64_t k0 = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kThreads;
+  const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t k0 = int64_t(g) * kThreads;
   if (k0 >= total) return;
-  const Entry<R>& E = tab[entry_of(tab, n, k0)];
+  const Entry<R>& E = entry_of(tab, n, g);
   bwd_cell(E.a, slot(E, mom), slot(E, lam1), slot(E, lam0), E.grad, with_obj, k0 - E.cell0 + threadIdx.x);
 }
 
@@ -1097,6 +1095,10 @@ int forward_slot(int T, int K, int t, int src) {
   return src == kSlotW0 ? kSlotW1 : kSlotW0;
 }
 
+unsigned batch_grid(const flb_adjoint_batch_s* b) {
+  return unsigned((b->total + A::kThreads - 1) / A::kThreads);
+}
+
 // Uploads the entry table and every member's slot pointers.
 template <class R>
 int upload_table(flb_adjoint_batch_s* b) {
@@ -1107,7 +1109,9 @@ int upload_table(flb_adjoint_batch_s* b) {
   const int ntape = K > 0 ? std::min(K, T) : (K == 0 ? T : 0);
   const int nslots = tb + ntape;
   size_t bytes_slots = n * size_t(nslots) * sizeof(R*);
-  size_t bytes_tab = n * sizeof(A::Entry<R>);
+  // the entries, then the box of every workgroup (A::entry_of)
+  const size_t nwg = size_t(batch_grid(b));
+  size_t bytes_tab = n * sizeof(A::Entry<R>) + nwg * sizeof(int);
   if (bytes_slots > b->slots_cap) {
     if (b->d_slots) (void)hipFree(b->d_slots);
     b->d_slots = nullptr;
@@ -1145,14 +1149,18 @@ int upload_table(flb_adjoint_batch_s* b) {
     tab[m].cell0 = b->cell0[m];
   }
   HIP_TRY(hipMemcpyAsync(b->d_slots, slots.data(), bytes_slots, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipMemcpyAsync(b->d_tab, tab.data(), bytes_tab, hipMemcpyHostToDevice, b->stream));
+  std::vector<int> wgbox(nwg, 0);
+  for (size_t m = 0; m < n; ++m) {
+    const int64_t w0 = b->cell0[m] / A::kThreads, w1 = (b->cell0[m] + b->members[m]->N + A::kThreads - 1) / A::kThreads;
+    for (int64_t w = w0; w < w1; ++w) wgbox[size_t(w)] = int(m);
+  }
+  std::vector<uint8_t> blob(bytes_tab);
+  std::memcpy(blob.data(), tab.data(), n * sizeof(A::Entry<R>));
+  std::memcpy(blob.data() + n * sizeof(A::Entry<R>), wgbox.data(), nwg * sizeof(int));
+  HIP_TRY(hipMemcpyAsync(b->d_tab, blob.data(), bytes_tab, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->nslots = nslots;
   return FLB_OK;
-}
-
-unsigned batch_grid(const flb_adjoint_batch_s* b) {
-  return unsigned((b->total + A::kThreads - 1) / A::kThreads);
 }
 
 // One forward step of every box, slot src -> dst; OBJ: the objectives into jt[t];
