@@ -184,3 +184,24 @@ def test_layout_rejects_like_create():
     lay = abi.Layout()
     assert L.flb_block_layout(C.byref(_desc(overlap=0)), C.byref(lay)) == abi.FLB_ERR_ARG
     assert "overlap" in L.flb_last_error().decode()
+
+
+@pytest.mark.parametrize("dims,storage", [
+    ((2048, 2048, 2048), "f32"),   # 2^33 cells
+    ((1025, 1024, 1024), "f32"),   # just over 2^30 cells: 4 B x N > 2^32
+    ((513, 1024, 1024), "f64"),    # just over 2^29 cells: 8 B x N > 2^32
+])
+def test_adjoint_rejects_boxes_past_32bit_offsets(dims, storage):
+    """The adjoint steps address a population array with 32-bit byte offsets:
+    creation refuses a box whose arrays exceed 2^32 bytes, before any device
+    work."""
+    L = abi.lib()
+    d = abi.AdjointDesc()
+    d.nx, d.ny, d.nz = dims
+    d.storage = abi.STORAGE[storage]
+    d.omega = 1.0
+    d.outlet_x = 0
+    h = C.c_void_p()
+    assert L.flb_adjoint_create(C.byref(d), C.byref(h)) == abi.FLB_ERR_ARG
+    assert "box too large" in L.flb_last_error().decode()
+    assert not h.value
