@@ -45,6 +45,33 @@ def test_forward_matches_model_fp64():
         np.testing.assert_allclose(adj.state(), fm.numpy(), rtol=0, atol=1e-14)
 
 
+@pytest.mark.parametrize("per", [(True, False, False), (False, False, False), (False, True, True)])
+def test_solid_free_box_matches_model_fp64(per):
+    """Without solid cells the steps take each cell's wall links from its
+    place (the faces of the non-periodic axes) instead of loading link words:
+    objective, final state and gradient against the FP64 model."""
+    N = int(np.prod(DIMS))
+    rng = np.random.default_rng(23)
+    solid = np.zeros(N, dtype=bool)
+    rho = 1 + 0.01 * rng.standard_normal(N)
+    u = 0.02 * rng.standard_normal((3, N))
+    s = rng.uniform(0, 1, N)
+    model = Model(DIMS, omega=1 / TAU, force=G, alpha=ALPHA, periodic=per, solid=solid)
+    f0 = model.equilibrium(rho, u)
+    with Adjoint(DIMS, omega=1 / TAU, force=G, brinkman_alpha=ALPHA, periodic=per, storage="f64") as adj:
+        adj.set_fraction(s)
+        adj.init_equilibrium(rho, u)
+        J = adj.forward(15, 4)
+        state = adj.state()
+        g = adj.backward()
+    st = torch.tensor(s, requires_grad=True)
+    Jm, fm = model.run(f0, st, 15)
+    (gm,) = torch.autograd.grad(Jm, st)
+    assert abs(J - Jm.item()) <= 1e-12 * abs(Jm.item())
+    np.testing.assert_allclose(state, fm.detach().numpy(), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(g, gm.numpy(), rtol=0, atol=1e-9 * np.abs(gm.numpy()).max())
+
+
 @pytest.mark.parametrize("K", [0, 1, 4, 15, 100])
 def test_gradient_matches_autodiff_fp64(K):
     adj, model, f0, s, solid = _setup()
@@ -136,10 +163,12 @@ def test_channel_gradient_against_fd():
 
 
 def _mixed_members(storage):
-    """Three boxes of different sizes and boundary setups."""
+    """Four boxes of different sizes and boundary setups (with and without
+    solid cells)."""
     specs = [((10, 6, 5), (True, False, False), [33, 34, 140, 201]),
              ((7, 9, 4), (False, True, False), [5, 77, 150]),
-             ((13, 5, 6), (True, True, True), [])]
+             ((13, 5, 6), (True, True, True), []),
+             ((9, 7, 6), (False, False, True), [])]
     out = []
     for k, (dims, per, sol) in enumerate(specs):
         N = int(np.prod(dims))
@@ -196,7 +225,7 @@ def test_batch_forward_only_and_rerun():
             assert np.array_equal(J, b.forward(9, -1))
             b.forward(12, 5)
             g = b.backward()
-            assert len(g) == 3 and all(x.size == m.N for x, m in zip(g, mem))
+            assert len(g) == len(mem) and all(x.size == m.N for x, m in zip(g, mem))
     finally:
         for a in mem:
             a.close()
