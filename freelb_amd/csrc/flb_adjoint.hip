@@ -31,7 +31,9 @@
 //
 // Layout: one dense array per direction, [19][N], N = nx*ny*nz,
 // id = x + nx*(y + ny*z) (FreeLB's index order); states are kept whole for
-// checkpoints, so no AA parity is involved. The arrays hold g_i = f_i - w_i
+// checkpoints, so no AA parity is involved. The adjoint step reads a forward
+// state only through its moment record, [4][N]: rho - 1 and u, written by the
+// forward step that reads that state (or by k_mom). The arrays hold g_i = f_i - w_i
 // and all moments are formed from g (rho = 1 + sum g, j = sum c g): the
 // velocities of a slow channel flow (u ~ 1e-5) then keep full precision in
 // FP32 instead of drowning in the rounding of f ~ w.
