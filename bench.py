@@ -303,15 +303,16 @@ def adjoint_bench(T=800, K=28):
     return out
 
 
-def adjoint_batch_bench(n=64, T=400, seed=11, fp64_check=False):
+def adjoint_batch_bench(n=64, T=400, seed=11, fp64_check=False, K=None):
     """n channels with seeded extents uniform in [64,32,32]..[128,64,64]
     (numpy default_rng(seed)), each with its own seeded Brinkman field,
     stepped as one batch (one launch per step over all boxes), T steps with
-    a checkpoint every round(sqrt(T)) steps, then the batched adjoint. With
+    a checkpoint every round(sqrt(T)) steps (or every K; K = 0 records every
+    state's moments and recomputes nothing), then the batched adjoint. With
     fp64_check the same batch runs in FP64 and the FP32 objective and
     gradient errors against it are reported."""
     from freelb_amd.cases import channel_batch, channel_dims
-    K = int(round(T ** 0.5))
+    K = int(round(T ** 0.5)) if K is None else int(K)
     dims = channel_dims(n, seed=seed)
     cells = int(sum(int(np.prod(d)) for d in dims))
     t0 = time.perf_counter()
@@ -327,8 +328,9 @@ def adjoint_batch_bench(n=64, T=400, seed=11, fp64_check=False):
         b.close()
         for m in mem:
             m.close()
+    rec = f"checkpoint every {K}" if K > 0 else "moment record of every state (checkpoint_every 0), no recompute"
     out = {"workload": f"{n} channels, extents U[64..128]xU[32..64]xU[32..64] (seed {seed}), T {T}, "
-                       f"checkpoint every {K}, FP32, one launch per step", "boxes": n, "cells": cells,
+                       f"{rec}, FP32, one launch per step", "boxes": n, "cells": cells,
            "setup_s": round(setup_s, 1), "grad_finite": bool(all(np.isfinite(g).all() for g in g32))}
     out.update(_adjoint_rates(cells, T, K, fwd_ms, bwd_ms))
     if fp64_check:
@@ -532,6 +534,7 @@ def main():
         print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
         print(json.dumps({"adjoint_no_recompute": adjoint_bench(K=0)}), flush=True)
         print(json.dumps({"adjoint_batch": adjoint_batch_bench()}), flush=True)
+        print(json.dumps({"adjoint_batch_no_recompute": adjoint_batch_bench(K=0)}), flush=True)
         return
     if args.mrt_les:
         st = tuple(x for x in args.storages.split(",") if x != "bf16s")
@@ -625,6 +628,9 @@ def main():
         print(f"# adjoint, no recompute: {line['adjoint_no_recompute']}", file=sys.stderr, flush=True)
         line["adjoint_batch"] = adjoint_batch_bench()
         print(f"# adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
+        # the same batch without recompute: T moment records per box, 92 GB of HBM
+        line["adjoint_batch_no_recompute"] = adjoint_batch_bench(K=0)
+        print(f"# adjoint batch, no recompute: {line['adjoint_batch_no_recompute']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
