@@ -79,6 +79,13 @@ class Adjoint:
         abi.check(abi.lib().flb_adjoint_init_equilibrium(self._h, abi.ptr(rhou, C.c_double)))
 
     def forward(self, steps: int, checkpoint_every: int = -1) -> float:
+        """Runs `steps` steps from f^0 and returns the objective J.
+
+        checkpoint_every k > 0 keeps every k-th state; the backward pass then
+        recomputes each segment of k states from its checkpoint (k near
+        sqrt(steps) balances memory and recompute). 0 records every state's
+        moments (rho - 1 and u, 4 values per cell) and recomputes nothing.
+        -1 records nothing: objective only, no backward pass."""
         J = C.c_double()
         abi.check(abi.lib().flb_adjoint_forward(self._h, int(steps), int(checkpoint_every), C.byref(J)))
         return J.value
@@ -131,6 +138,8 @@ class AdjointBatch:
         self.close()
 
     def forward(self, steps: int, checkpoint_every: int = -1) -> np.ndarray:
+        """Every member's objective after `steps` steps (checkpoint_every as
+        for Adjoint.forward)."""
         J = np.empty(len(self.members), dtype=np.float64)
         abi.check(abi.lib().flb_adjoint_batch_forward(self._h, int(steps), int(checkpoint_every),
                                                       abi.ptr(J, C.c_double)))
