@@ -86,16 +86,20 @@ def test_gradient_matches_autodiff_fp64(K):
     np.testing.assert_allclose(g, gm, rtol=0, atol=1e-9 * np.abs(gm).max())
 
 
-def test_checkpoint_modes_agree_bitwise():
-    adj, *_ = _setup()
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_checkpoint_modes_agree_bitwise(storage):
+    """Every checkpoint interval gives the same objective, final state and
+    gradient bit for bit: moment records written by the forward pass (K 0)
+    equal those of the recomputed segments (K 5, 7; K 40 > T is one segment)."""
+    adj, *_ = _setup(storage)
+    out = []
     with adj:
-        adj.forward(23, 0)
-        g0 = adj.backward()
-        adj.forward(23, 5)
-        g5 = adj.backward()
-        adj.forward(23, 7)
-        g7 = adj.backward()
-    assert np.array_equal(g0, g5) and np.array_equal(g0, g7)
+        for K in (0, 5, 7, 40):
+            J = adj.forward(23, K)
+            out.append((J, adj.state(), adj.backward()))
+    J0, s0, g0 = out[0]
+    for J, st, g in out[1:]:
+        assert J == J0 and np.array_equal(st, s0) and np.array_equal(g, g0)
 
 
 def test_gradient_against_fd_fp64():
