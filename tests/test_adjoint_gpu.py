@@ -45,20 +45,24 @@ def test_forward_matches_model_fp64():
         np.testing.assert_allclose(adj.state(), fm.numpy(), rtol=0, atol=1e-14)
 
 
-@pytest.mark.parametrize("per", [(True, False, False), (False, False, False), (False, True, True)])
-def test_solid_free_box_matches_model_fp64(per):
+@pytest.mark.parametrize("dims,per", [
+    (DIMS, (True, False, False)), (DIMS, (False, False, False)), (DIMS, (False, True, True)),
+    # extents of 1: the cell index divides by 1, and both faces of an axis
+    # fall on one cell (or a periodic axis wraps onto the cell itself)
+    ((1, 6, 5), (True, False, False)), ((8, 1, 5), (False, False, True)), ((6, 4, 1), (False, False, False))])
+def test_solid_free_box_matches_model_fp64(dims, per):
     """Without solid cells the steps take each cell's wall links from its
     place (the faces of the non-periodic axes) instead of loading link words:
     objective, final state and gradient against the FP64 model."""
-    N = int(np.prod(DIMS))
+    N = int(np.prod(dims))
     rng = np.random.default_rng(23)
     solid = np.zeros(N, dtype=bool)
     rho = 1 + 0.01 * rng.standard_normal(N)
     u = 0.02 * rng.standard_normal((3, N))
     s = rng.uniform(0, 1, N)
-    model = Model(DIMS, omega=1 / TAU, force=G, alpha=ALPHA, periodic=per, solid=solid)
+    model = Model(dims, omega=1 / TAU, force=G, alpha=ALPHA, periodic=per, solid=solid)
     f0 = model.equilibrium(rho, u)
-    with Adjoint(DIMS, omega=1 / TAU, force=G, brinkman_alpha=ALPHA, periodic=per, storage="f64") as adj:
+    with Adjoint(dims, omega=1 / TAU, force=G, brinkman_alpha=ALPHA, periodic=per, storage="f64") as adj:
         adj.set_fraction(s)
         adj.init_equilibrium(rho, u)
         J = adj.forward(15, 4)
